@@ -1,0 +1,76 @@
+"""Reader for gmsh ASCII ``.msh`` files (format 2.x), the subset Akantu's
+``MeshIOMSH`` (src/io/mesh_io/mesh_io_msh.cc) needs for Tet4 and Hex8 meshes.
+
+gmsh element type 4 is ``_tetrahedron_4`` and 5 is ``_hexahedron_8``; both
+keep gmsh's node order (mesh_io_msh.cc:200-263 leaves them unpermuted).
+Node tags are renumbered to 0-based indices in file order.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .mesh import HEX8, TET4, BoxMesh
+
+_MSH_TYPES = {4: (TET4, 4), 5: (HEX8, 8)}
+
+
+def read_msh(path, etype=None):
+    """Return a ``BoxMesh``-like object with the volume elements of the file.
+
+    ``etype`` selects Tet4 or Hex8 when the file holds both; by default the
+    first volume type found is used."""
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    it = iter(range(len(lines)))
+    nodes, tag_to_idx, elems = [], {}, {}
+    i = 0
+    while i < len(lines):
+        tok = lines[i].strip()
+        if tok == "$MeshFormat":
+            version = float(lines[i + 1].split()[0])
+            if not (2.0 <= version < 3.0):
+                raise ValueError(f"unsupported msh version {version}")
+            i += 2
+        elif tok == "$Nodes":
+            n = int(lines[i + 1])
+            for k in range(n):
+                p = lines[i + 2 + k].split()
+                tag_to_idx[int(p[0])] = k
+                nodes.append([float(p[1]), float(p[2]), float(p[3])])
+            i += 2 + n
+        elif tok == "$Elements":
+            n = int(lines[i + 1])
+            for k in range(n):
+                p = [int(x) for x in lines[i + 2 + k].split()]
+                mtype, ntags = p[1], p[2]
+                if mtype in _MSH_TYPES:
+                    et, nn = _MSH_TYPES[mtype]
+                    c = p[3 + ntags: 3 + ntags + nn]
+                    elems.setdefault(et, []).append([tag_to_idx[t] for t in c])
+            i += 2 + n
+        else:
+            i += 1
+    del it
+    if etype is None:
+        if not elems:
+            raise ValueError("no Tet4/Hex8 elements in " + str(path))
+        etype = sorted(elems)[0]
+    conn = np.asarray(elems[etype], dtype=np.int32)
+    X = np.asarray(nodes, dtype=np.float64)
+    return BoxMesh(etype, X, conn, shape=(0, 0, 0), meta={"file": str(path)})
+
+
+def boundary_nodes(mesh) -> np.ndarray:
+    """Nodes lying on faces that belong to exactly one element (the groups
+    ``Mesh::createBoundaryGroupFromGeometry`` builds from the facets)."""
+    if mesh.etype == TET4:
+        faces = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
+    else:
+        faces = [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5),
+                 (2, 3, 7, 6), (3, 0, 4, 7)]
+    allf = np.concatenate([np.sort(mesh.conn[:, list(f)], axis=1)
+                           for f in faces])
+    uniq, counts = np.unique(allf, axis=0, return_counts=True)
+    on = np.zeros(mesh.nb_nodes, dtype=bool)
+    on[np.unique(uniq[counts == 1].ravel())] = True
+    return on

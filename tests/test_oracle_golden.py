@@ -1,0 +1,385 @@
+"""Pins the CPU oracle (oracle/akantu_oracle.c) before it is trusted as the
+parity reference: against the fixtures the reference's own tests hold, against
+closed forms, and against invariants chosen so that a plausible slip (wrong
+sign, transposed operand, dropped term, wrong qp order) fails one of them.
+
+Reference tests mirrored here:
+- test/test_fe_engine/test_fe_engine_inradius.cc (inradius CSVs, 1e-6)
+- test/test_fe_engine/test_gradient.cc (grad of a linear field 5e-13, grad X = I)
+- test/test_model/patch_tests/test_lumped_mass.cc (sum of masses = rho V)
+- test/test_model/patch_tests/patch_test_linear_elastic_explicit.cc
+  (linear field, 100 explicit steps: E_kin, DOFs 1e-15, grad u / sigma 1e-13,
+  force balance 1e-9)
+- test/test_model/test_common/test_model_solver/test_model_solver_dynamic.cc
+  with test_model_solver_dynamic_explicit.verified (integrator end to end)
+- test/test_model/test_solid_mechanics_model/test_materials/
+  test_elastic_materials.cc (closed-form sigma under rotations, 1e-13)
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from akantu2_amd import mesh as M
+from akantu2_amd import mesh_io
+
+
+def _read(fixtures_dir, name):
+    return mesh_io.read_msh(os.path.join(fixtures_dir, name))
+
+
+def _csv(fixtures_dir, name):
+    d = np.loadtxt(os.path.join(fixtures_dir, name), delimiter=",")
+    out = np.zeros(int(d[:, 0].max()) + 1)
+    out[d[:, 0].astype(int)] = d[:, 1]
+    return out
+
+
+# ---------------------------------------------------------------- FE engine --
+@pytest.mark.parametrize("name,etype", [("tetrahedron_4", oracle.TET4),
+                                        ("hexahedron_8", oracle.HEX8)])
+def test_inradius_matches_reference_csv(fixtures_dir, name, etype):
+    m = _read(fixtures_dir, f"fe_engine_{name}.msh")
+    ref = _csv(fixtures_dir, f"fe_engine_{name}_inradius.csv")
+    assert m.nb_elements == ref.size
+    got = np.array([oracle.inradius(etype, m.nodes[c]) for c in m.conn])
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+
+
+def test_kuhn_tet_stable_dt_closed_form():
+    """Kuhn tet: 2 * inradius = sqrt(2) - 1, c_p = sqrt((lambda+2mu)/rho)."""
+    m = M.box_mesh("_tetrahedron_4", 20, 4, 4)
+    o = oracle.OracleModel(oracle.TET4, m.nodes, m.conn, "elastic", E=1, nu=0.3)
+    o.init_full()
+    lam, mu = 0.3 / (1.3 * 0.4), 1 / 2.6
+    expect = (np.sqrt(2) - 1) / np.sqrt(lam + 2 * mu)
+    assert o.stable_dt() == pytest.approx(expect, rel=1e-14)
+    assert o.stable_dt() == pytest.approx(0.357007, abs=1e-6)
+
+
+@pytest.mark.parametrize("name", ["tetrahedron_4", "hexahedron_8"])
+def test_gradient_of_linear_field_is_exact(fixtures_dir, name):
+    m = _read(fixtures_dir, f"fe_engine_{name}.msh")
+    o = oracle.OracleModel(m.etype, m.nodes, m.conn, "elastic")
+    o.init_full()
+    rng = np.random.default_rng(3)
+    G = rng.uniform(-1, 1, (3, 3))
+    c = rng.uniform(-1, 1, 3)
+    o.u[:] = m.nodes @ G.T + c
+    lib = oracle.lib()
+    import ctypes
+    lib.orc_compute_all_stresses(ctypes.byref(o._s))
+    g = o.grad_u.reshape(-1, 3, 3).transpose(0, 2, 1)  # column-major storage
+    assert np.abs(g - G).max() <= 5e-13
+    # grad X = I
+    o.u[:] = m.nodes
+    lib.orc_compute_all_stresses(ctypes.byref(o._s))
+    g = o.grad_u.reshape(-1, 3, 3).transpose(0, 2, 1)
+    assert np.abs(g - np.eye(3)).max() <= 2e-14
+
+
+def test_hex8_gauss_tables_and_volume():
+    xi, w = oracle.gauss_points(oracle.HEX8)
+    s = 1 / np.sqrt(3)
+    # xi fastest (gauss_integration_tmpl.hh:136-181)
+    assert np.allclose(xi[1], [s, -s, -s]) and np.allclose(xi[2], [-s, s, -s])
+    assert np.allclose(xi[4], [-s, -s, s]) and np.all(w == 1.0)
+    for q in range(8):
+        assert oracle.shapes(oracle.HEX8, xi[q]).sum() == pytest.approx(1.0, abs=1e-15)
+        assert np.abs(oracle.dnds(oracle.HEX8, xi[q]).sum(axis=1)).max() < 1e-16
+    # an affine (parallelepiped) hex: sum of w*detJ = exact volume
+    A = np.array([[2.0, 0.3, 0.1], [0.2, 1.5, -0.2], [0.0, 0.4, 0.7]])
+    m = M.box_mesh("_hexahedron_8", 3, 2, 2)
+    X = m.nodes @ A.T
+    o = oracle.OracleModel(oracle.HEX8, X, m.conn, "elastic")
+    o.init_full()
+    assert o.wJ.sum() == pytest.approx(12 * np.linalg.det(A), rel=1e-14)
+    xi_t, w_t = oracle.gauss_points(oracle.TET4)
+    assert np.all(xi_t == 0.25) and w_t[0] == 1.0 / 6.0
+
+
+def test_negative_jacobian_raises():
+    m = M.box_mesh("_tetrahedron_4", 2, 1, 1)
+    conn = m.conn.copy()
+    conn[3, [1, 2]] = conn[3, [2, 1]]
+    o = oracle.OracleModel(oracle.TET4, m.nodes, conn, "elastic")
+    with pytest.raises(ValueError, match="negative jacobian at quadrature point 3"):
+        o.init_full()
+
+
+# ------------------------------------------------------------------- mass ----
+@pytest.mark.parametrize("name", ["tetrahedron_4", "hexahedron_8"])
+def test_lumped_mass_sums_to_rho_volume(fixtures_dir, name):
+    m = _read(fixtures_dir, f"fe_engine_{name}.msh")
+    o = oracle.OracleModel(m.etype, m.nodes, m.conn, "elastic", rho=7800.0)
+    o.init_full()
+    V = o.wJ.sum()
+    assert V == pytest.approx(1.0, rel=1e-12)  # the fixture meshes are the unit cube
+    for c in range(3):
+        assert o.mass[:, c].sum() == pytest.approx(7800.0 * V, rel=2e-15)
+    assert np.all(o.mass[:, 0] == o.mass[:, 1]) and np.all(o.mass[:, 0] == o.mass[:, 2])
+
+
+def test_lumped_mass_structured_closed_forms():
+    m = M.box_mesh("_tetrahedron_4", 3, 2, 2)
+    o = oracle.OracleModel(oracle.TET4, m.nodes, m.conn, "elastic", rho=2.0)
+    o.init_full()
+    deg = np.bincount(m.conn.ravel(), minlength=m.nb_nodes)
+    np.testing.assert_allclose(o.mass[:, 0], deg * 2.0 / 6.0 / 4.0, rtol=1e-15)
+    m = M.box_mesh("_hexahedron_8", 3, 2, 2, h=0.5)
+    o = oracle.OracleModel(oracle.HEX8, m.nodes, m.conn, "elastic", rho=3.0)
+    o.init_full()
+    deg = np.bincount(m.conn.ravel(), minlength=m.nb_nodes)
+    np.testing.assert_allclose(o.mass[:, 0], deg * 3.0 * 0.125 / 8.0, rtol=1e-14)
+
+
+# ------------------------------------------------------------- materials -----
+def _rot(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q *= np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] *= -1
+    return q
+
+
+def test_elastic_closed_form_under_rotations():
+    rng = np.random.default_rng(7)
+    E, nu = 2.1e11, 0.3
+    lam, mu = oracle.lame(E, nu)
+    assert lam == pytest.approx(nu * E / ((1 + nu) * (1 - 2 * nu)), rel=1e-15)
+    assert mu == pytest.approx(E / (2 * (1 + nu)), rel=1e-15)
+    for _ in range(20):
+        R = _rot(rng)
+        eps_p = np.diag(rng.uniform(-1e-3, 1e-3, 3))
+        g = R @ eps_p @ R.T
+        sig, _, _ = oracle.law_stress("elastic", lam, mu, g)
+        expect = R @ (lam * np.trace(eps_p) * np.eye(3) + 2 * mu * eps_p) @ R.T
+        assert np.abs(sig - expect).max() / np.abs(expect).max() < 1e-13
+        # the antisymmetric part of grad u carries no stress
+        W = rng.normal(size=(3, 3)) * 1e-3
+        sig2, _, _ = oracle.law_stress("elastic", lam, mu, g + (W - W.T))
+        assert np.abs(sig2 - sig).max() / np.abs(expect).max() < 1e-12
+
+
+def test_neohookean_closed_form_and_objectivity():
+    rng = np.random.default_rng(11)
+    lam, mu = oracle.lame(1.0, 0.3)
+    for _ in range(10):
+        st = rng.uniform(0.8, 1.25, 3)
+        g = np.diag(st - 1.0)
+        S, _, _ = oracle.law_stress("neohookean", lam, mu, g)
+        J = st.prod()
+        expect = np.diag(mu + (lam * np.log(J) - mu) / st**2)
+        np.testing.assert_allclose(S, expect, rtol=1e-13, atol=1e-15)
+        # F -> R F leaves C, hence S, unchanged
+        R = _rot(rng)
+        F = np.eye(3) + rng.normal(size=(3, 3)) * 0.05
+        S1, _, _ = oracle.law_stress("neohookean", lam, mu, F - np.eye(3))
+        S2, _, _ = oracle.law_stress("neohookean", lam, mu, R @ F - np.eye(3))
+        assert np.abs(S1 - S2).max() < 1e-13
+    # rigid rotation: S = 0
+    S, _, _ = oracle.law_stress("neohookean", lam, mu, _rot(rng) - np.eye(3))
+    assert np.abs(S).max() < 1e-14
+
+
+def test_elastic_finite_deformation_is_svk():
+    rng = np.random.default_rng(5)
+    lam, mu = oracle.lame(1.0, 0.25)
+    g = rng.normal(size=(3, 3)) * 0.1
+    S, _, _ = oracle.law_stress("elastic_fd", lam, mu, g)
+    Eg = 0.5 * (g.T @ g + g + g.T)
+    np.testing.assert_allclose(S, lam * np.trace(Eg) * np.eye(3) + 2 * mu * Eg,
+                               rtol=1e-13, atol=1e-16)
+
+
+def test_plastic_radial_return_closed_form():
+    lam, mu = oracle.lame(1.0, 0.3)
+    sy, h = 2e-3, 0.05
+    # below yield: purely elastic
+    g = np.zeros((3, 3)); g[0, 1] = 1e-3
+    sig, iso, eps = oracle.law_stress("plastic_linear_isotropic_hardening", lam, mu,
+                                      g, sigma_y=sy, h=h)
+    np.testing.assert_allclose(sig, mu * (g + g.T), rtol=1e-14)
+    assert iso == 0.0 and np.all(eps == 0.0)
+    # simple shear beyond yield: one radial-return step
+    g[0, 1] = 1e-2
+    sig, iso, eps = oracle.law_stress("plastic_linear_isotropic_hardening", lam, mu,
+                                      g, sigma_y=sy, h=h)
+    tau_tr = mu * 1e-2
+    eff_tr = np.sqrt(3.0) * tau_tr
+    dp = (eff_tr - sy) / (3 * mu + h)
+    assert iso == pytest.approx(h * dp, rel=1e-13)
+    dev = sig - np.trace(sig) / 3 * np.eye(3)
+    eff = np.sqrt(1.5 * np.sum(dev * dev))
+    assert eff == pytest.approx(sy + iso, rel=1e-12)  # back on the yield surface
+    assert sig[0, 1] == pytest.approx(tau_tr * (1 - 3 * mu * dp / eff_tr), rel=1e-12)
+    assert np.trace(eps) == pytest.approx(0.0, abs=1e-18)  # isochoric flow
+    # continuing from that state with no increment changes nothing
+    sig2, iso2, eps2 = oracle.law_stress("plastic_linear_isotropic_hardening", lam, mu,
+                                         g, sigma_y=sy, h=h, prev_grad_u=g,
+                                         prev_sigma=sig, prev_iso=iso, prev_eps_p=eps)
+    np.testing.assert_allclose(sig2, sig, rtol=1e-14, atol=1e-18)
+    assert iso2 == pytest.approx(iso, rel=1e-14)
+
+
+# ---------------------------------------------------------------- assembly ---
+@pytest.mark.parametrize("etype,law", [(oracle.TET4, "elastic"), (oracle.HEX8, "elastic"),
+                                       (oracle.TET4, "neohookean"),
+                                       (oracle.HEX8, "neohookean"),
+                                       (oracle.HEX8, "elastic_fd")])
+def test_internal_force_invariants(etype, law):
+    m = M.box_mesh(etype, 4, 3, 2, jitter=0.1, seed=2)
+    o = oracle.OracleModel(etype, m.nodes, m.conn, law)
+    o.init_full()
+    rng = np.random.default_rng(1)
+    o.u[:] = rng.uniform(-1e-3, 1e-3, o.u.shape)
+    o.internal_forces()
+    f = o.f_int.copy()
+    scale = np.abs(f).max()
+    assert scale > 0
+    assert np.abs(f.sum(axis=0)).max() / scale < 1e-13  # translation invariance
+    # rigid translation adds no force
+    o.u[:] += np.array([0.3, -0.2, 0.1])
+    o.internal_forces()
+    assert np.abs(o.f_int - f).max() / scale < 1e-9
+    # rigid rotation: zero force (exactly for infinitesimal rotations in small
+    # strain, for finite rotations in finite deformation)
+    R = _rot(np.random.default_rng(4))
+    if law == "elastic":
+        W = np.array([[0, -1e-3, 2e-3], [1e-3, 0, -5e-4], [-2e-3, 5e-4, 0]])
+        o.u[:] = m.nodes @ W.T
+    else:
+        o.u[:] = m.nodes @ R.T - m.nodes
+    o.internal_forces()
+    assert np.abs(o.f_int).max() < 1e-12
+    if law == "elastic":  # linear in u, with the minus sign of material.cc:508
+        o.u[:] = 0
+        o.u[1] = [1e-3, 0, 0]
+        o.internal_forces()
+        assert o.f_int[1, 0] < 0  # pulling a node creates a restoring force
+
+
+def _patch(fixtures_dir, name):
+    m = _read(fixtures_dir, f"patch_{name}.msh")
+    o = oracle.OracleModel(m.etype, m.nodes, m.conn, "elastic", rho=7800.0,
+                           E=2.1e11, nu=0.3)
+    o.init_full()
+    alpha = np.array([[0.01, 0.02, 0.03, 0.04], [0.05, 0.06, 0.07, 0.08],
+                      [0.09, 0.10, 0.11, 0.12]])
+    lin = alpha[:, 0] + m.nodes @ alpha[:, 1:].T
+    bnd = mesh_io.boundary_nodes(m)
+    o.blocked[bnd] = 1
+    o.u[:] = lin
+    o.set_time_step(0.8 * o.stable_dt())
+    return m, o, alpha, lin, bnd
+
+
+@pytest.mark.parametrize("name", ["tetrahedron_4", "hexahedron_8"])
+def test_patch_test_linear_elastic_explicit(fixtures_dir, name):
+    m, o, alpha, lin, bnd = _patch(fixtures_dir, name)
+    assert (~bnd).sum() > 0  # the patch has interior nodes
+    o.solve_steps(100)
+    ekin = 0.5 * np.sum(o.mass * o.v * o.v)
+    assert ekin < 1e-16
+    assert np.abs(o.u - lin).max() < 1e-15
+    G = alpha[:, 1:]
+    g = o.grad_u.reshape(-1, 3, 3).transpose(0, 2, 1)
+    assert np.abs(g - G).max() / np.abs(G).max() < 1e-13
+    E, nu = 2.1e11, 0.3
+    lam, mu = nu * E / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    eps = 0.5 * (G + G.T)
+    sig_ref = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+    s = o.stress.reshape(-1, 3, 3).transpose(0, 2, 1)
+    assert np.abs(s - sig_ref).max() / np.abs(sig_ref).max() < 1e-13
+    f = o.f_int
+    fmax = np.abs(f).max()
+    assert np.abs(f.sum(axis=0)).max() / fmax < 1e-9
+    assert np.abs(f[~bnd]).max() / fmax < 1e-9  # interior equilibrium
+
+
+# -------------------------------------------------------------- integrator ---
+def test_model_solver_dynamic_explicit_verified(fixtures_dir):
+    """The reference's 1-D truss (test_model_solver_my_model.hh:46-445) driven
+    through _dynamic_lumped + _central_difference (_acceleration), compared
+    with its .verified stdout (7 significant digits)."""
+    nn, dt, pw, amp, F = 201, 0.001, 0.2, 0.01, -9.81
+    X = np.arange(nn) * (1.0 / (nn - 1))
+    L = np.abs(X[1:] - X[:-1])
+    k, c = 2 * np.pi / pw, 1.0
+    u, v, a = np.zeros(nn), np.zeros(nn), np.zeros(nn)
+    fext = np.zeros(nn)
+    fext[-1] = F
+    blocked = np.zeros(nn, np.uint8)
+
+    def bc(nodes, t):
+        x = X[nodes]
+        v[nodes] = k * c * amp * np.sin(k * (x - c * t))
+        u[nodes] = amp * np.cos(k * (x - c * t))
+
+    bc(np.arange(nn), 0.0)
+    blocked[[0, nn - 1]] = 1
+    mass = np.zeros(nn)
+    for e in range(nn - 1):
+        mass[e] += L[e] / 2
+        mass[e + 1] += L[e] / 2
+    st = {}
+
+    def residual(r):
+        strain = (u[1:] - u[:-1]) / L
+        fn = strain  # E = A = 1
+        fi = np.zeros(nn)
+        for e in range(nn - 1):
+            fi[e] += -fn[e]
+            fi[e + 1] += fn[e]
+        r += fext
+        r += -1.0 * fi
+        st["fi"], st["strain"] = fi, strain
+
+    rows = [(0.0, 0.0, 0.0, 0.0, 0.0, max(0, u.max()), min(0, u.min()))]
+    wext = 0.0
+    for i in range(1, 401):
+        bc(np.array([0, nn - 1]), dt * (i - 1))
+        oracle.lumped_step_cb(u, v, a, blocked, mass, dt, residual)
+        epot = np.sum(st["strain"] ** 2 * L) / 2
+        ekin = np.sum(mass * v * v) / 2
+        wext += np.sum(np.where(blocked == 1, -st["fi"], fext) * v) * dt
+        rows.append((dt * i, wext, epot, ekin, ekin + epot - wext,
+                     max(0, u.max()), min(0, u.min())))
+    ref = np.loadtxt(os.path.join(fixtures_dir, "model_solver_dynamic_explicit.verified"),
+                     delimiter=",", skiprows=1)
+    got = np.array(rows)
+    assert got.shape == ref.shape
+    # printed with 7 significant digits: half a unit in the last place
+    np.testing.assert_allclose(got, ref, rtol=5.01e-7, atol=1e-24)
+
+
+def test_central_difference_single_dof_closed_form():
+    """x'' = -w^2 x: the lumped central-difference iterate satisfies
+    x_{n+1} - 2 x_n + x_{n-1} = -(w dt)^2 x_n, i.e. x_n = cos(n theta) with
+    cos(theta) = 1 - (w dt)^2 / 2 for x_0 = 1, v_0 = 0, a_0 = -w^2."""
+    w2, dt = 4.0, 0.1
+    u, v, a = np.array([1.0]), np.array([0.0]), np.array([-w2])
+    mass, blocked = np.array([1.0]), np.zeros(1, np.uint8)
+    theta = np.arccos(1 - w2 * dt * dt / 2)
+    for n in range(1, 200):
+        oracle.lumped_step_cb(u, v, a, blocked, mass, dt,
+                              lambda r: r.__setitem__(0, -w2 * u[0]))
+        assert u[0] == pytest.approx(np.cos(n * theta), abs=1e-12)
+
+
+def test_c1_energy_is_conserved():
+    m = M.box_mesh("_tetrahedron_4", 20, 4, 4)
+    o = oracle.OracleModel(oracle.TET4, m.nodes, m.conn, "elastic")
+    o.init_full()
+    o.blocked[:] = M.clamped_face_x0(m)
+    o.v[:] = M.initial_velocity(m, 1e-3, 0)
+    o.set_time_step(0.5 * o.stable_dt())
+    e0 = 0.5 * np.sum(o.mass * o.v * o.v)
+    es = []
+    for _ in range(10):
+        o.solve_steps(100)
+        # potential energy of a linear elastic body: -1/2 u . f_int(u)
+        es.append(0.5 * np.sum(o.mass * o.v * o.v) - 0.5 * np.sum(o.u * o.f_int))
+    drift = np.abs(np.array(es) - e0).max() / e0
+    assert drift < 2e-2
