@@ -1,0 +1,534 @@
+// akx_kernels.hpp -- gfx950 kernels of the explicit internal-force step.
+//
+// Device layout (structure of arrays, all coalesced on the fast index):
+//   nodal fields     X, u, v, a, f_int, f_ext : [3][nn] doubles
+//   lumped mass      M                         : [nn] doubles (the three
+//                                                components are equal)
+//   blocked DOFs     bits 0..2 of a byte       : [nn]
+//   connectivity                               : [nel][nnpe] int32 (16 B loads)
+//   elemental force  r_e                       : [slot][3], slot = p*nnpe + a
+//                    where p is the element's position in the assembly order
+//   gather table     rowptr [nn+1], slots [nel*nnpe] (node -> slot list in
+//                    ascending assembly order, i.e. the reference's
+//                    sequential scatter order of dof_manager.cc:92-114)
+#pragma once
+
+#include "akx_math.hpp"
+#include <stdint.h>
+
+namespace akx {
+
+constexpr int ET_TET4 = 0;
+constexpr int ET_HEX8 = 1;
+constexpr int LAW_ELASTIC = 0;
+constexpr int LAW_ELASTIC_FD = 1;
+constexpr int LAW_NEOHOOKEAN = 2;
+constexpr int LAW_PLASTIC = 3;
+
+template <int ET> struct Elem;
+template <> struct Elem<ET_TET4> {
+  static constexpr int nnpe = 4, nqp = 1;
+};
+template <> struct Elem<ET_HEX8> {
+  static constexpr int nnpe = 8, nqp = 8;
+};
+
+// Hex8 natural derivatives, shapes and weights at the 8 Gauss points,
+// computed on the host with the reference's formulas
+// (element_class_hexahedron_8_inline_impl.hh:111-188,
+// gauss_integration_tmpl.hh:136-181) and uploaded once.
+struct Hex8Tables {
+  double dnds[8][3][8];
+  double N[8][8];
+  double w[8];
+  float dnds_f[8][3][8];
+};
+extern __constant__ Hex8Tables c_hex8;
+
+// blockIdx -> logical block so that each of the 8 XCDs walks a contiguous
+// range of elements/nodes (blocks are dealt round-robin to the XCDs; the
+// nodes shared by neighbouring elements then stay in one XCD's L2).
+__device__ __forceinline__ int64_t xcd_block(int64_t nblocks_padded) {
+  int64_t b = blockIdx.x;
+  int64_t per = nblocks_padded >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Jacobian at one quadrature point and the columns of dN/dX.
+// Reference: shape_lagrange_inline_impl.hh:61-83 (J = dnds X^T,
+// dN/dX = J^{-1} dnds), element_class_tmpl.hh:297-346, integrator_gauss
+// multiplyJacobiansByWeights (integrator_gauss_inline_impl.hh:379-398).
+// X is given per node, X[n][j].  jacobian_at returns w*detJ and J^{-1};
+// bcol gives column n of dN/dX, b[i] = dN_n/dX_i, recomputed where it is
+// needed instead of being kept (the bits are the same every time).
+// ---------------------------------------------------------------------------
+template <int ET, class R> struct Shape;
+
+template <> struct Shape<ET_TET4, double> {
+  __device__ static __forceinline__ double
+  jacobian_at(const double (&X)[4][3], int, double (&Ji)[3][3], double &detJ) {
+    double J[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      J[0][j] = X[3][j] - X[0][j]; // d/dxi   : nodes (-1, 0, 0, 1)
+      J[1][j] = X[1][j] - X[0][j]; // d/deta  : nodes (-1, 1, 0, 0)
+      J[2][j] = X[2][j] - X[0][j]; // d/dzeta : nodes (-1, 0, 1, 0)
+    }
+    detJ = jacobian_det_inv(J, Ji);
+    return detJ * (1. / 6.);
+  }
+  __device__ static __forceinline__ void bcol(const double (&Ji)[3][3], int,
+                                              int n, double (&b)[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      b[i] = n == 0 ? -Ji[i][0] + -Ji[i][1] + -Ji[i][2]
+                    : (n == 1 ? Ji[i][1] : (n == 2 ? Ji[i][2] : Ji[i][0]));
+  }
+};
+
+template <> struct Shape<ET_HEX8, double> {
+  __device__ static __forceinline__ double
+  jacobian_at(const double (&X)[8][3], int q, double (&Ji)[3][3], double &detJ) {
+    double J[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.;
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+          s += c_hex8.dnds[q][i][n] * X[n][j];
+        J[i][j] = s;
+      }
+    detJ = jacobian_det_inv(J, Ji);
+    return detJ * c_hex8.w[q];
+  }
+  __device__ static __forceinline__ void bcol(const double (&Ji)[3][3], int q,
+                                              int n, double (&b)[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double s = 0.;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        s += Ji[i][k] * c_hex8.dnds[q][k][n];
+      b[i] = s;
+    }
+  }
+};
+
+// FP32 form: coordinates relative to node 0 (exact FP64 differences rounded
+// once), so the FP32 Jacobian does not lose the mesh offset.
+template <> struct Shape<ET_TET4, float> {
+  __device__ static __forceinline__ float
+  jacobian_at(const float (&dX)[4][3], int, float (&Ji)[3][3], float &detJ) {
+    float J[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      J[0][j] = dX[3][j];
+      J[1][j] = dX[1][j];
+      J[2][j] = dX[2][j];
+    }
+    detJ = jacobian_det_inv(J, Ji);
+    return detJ * (1.f / 6.f);
+  }
+  __device__ static __forceinline__ void bcol(const float (&Ji)[3][3], int,
+                                              int n, float (&b)[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      b[i] = n == 0 ? -Ji[i][0] - Ji[i][1] - Ji[i][2]
+                    : (n == 1 ? Ji[i][1] : (n == 2 ? Ji[i][2] : Ji[i][0]));
+  }
+};
+
+template <> struct Shape<ET_HEX8, float> {
+  __device__ static __forceinline__ float
+  jacobian_at(const float (&dX)[8][3], int q, float (&Ji)[3][3], float &detJ) {
+    float J[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        float s = 0.f;
+#pragma unroll
+        for (int n = 1; n < 8; ++n)
+          s += c_hex8.dnds_f[q][i][n] * dX[n][j];
+        J[i][j] = s;
+      }
+    detJ = jacobian_det_inv(J, Ji);
+    return detJ; // Hex8 Gauss weights are all 1
+  }
+  __device__ static __forceinline__ void bcol(const float (&Ji)[3][3], int q,
+                                              int n, float (&b)[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      float s = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        s += Ji[i][k] * c_hex8.dnds_f[q][k][n];
+      b[i] = s;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Force of one quadrature point, r_q[n][k] (before the w*detJ factor).
+// Small strain: B^T sigma (shape_lagrange_inline_impl.hh:373-410).
+// Finite deformation: (B_L + B_NL(grad u))^T voigt(S) (material.cc:525-584,
+// aka_voigthelper_tmpl.hh:99-140,180-207), written out term by term in the
+// Voigt order xx yy zz yz xz xy (aka_voigthelper.hh:57-70).
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ __forceinline__ void btd_small(const R (&S)[3][3], const R (&b)[3],
+                                          R (&r)[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    R s = R(0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      s += S[i][j] * b[j];
+    r[i] = s;
+  }
+}
+
+template <class R>
+__device__ __forceinline__ void btd_finite(const R (&g)[3][3], const R (&S)[3][3],
+                                           const R (&b)[3], R (&r)[3]) {
+  const R S00 = S[0][0], S11 = S[1][1], S22 = S[2][2];
+  const R S12 = S[1][2], S02 = S[0][2], S01 = S[0][1];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    // rows 0..2: B_L(I, 3a+k) = [k == I] B(I, a); B_NL = g(k, I) B(I, a)
+    R t0 = ((k == 0) ? b[0] : R(0)) + g[k][0] * b[0];
+    R t1 = ((k == 1) ? b[1] : R(0)) + g[k][1] * b[1];
+    R t2 = ((k == 2) ? b[2] : R(0)) + g[k][2] * b[2];
+    // row 3 (yz): B_L = [k==1] dndz + [k==2] dndy
+    R l3 = (k == 1) ? b[2] : ((k == 2) ? b[1] : R(0));
+    R n3 = g[k][1] * b[2] + g[k][2] * b[1];
+    // row 4 (xz): B_L = [k==0] dndz + [k==2] dndx
+    R l4 = (k == 0) ? b[2] : ((k == 2) ? b[0] : R(0));
+    R n4 = g[k][0] * b[2] + g[k][2] * b[0];
+    // row 5 (xy): B_L = [k==0] dndy + [k==1] dndx
+    R l5 = (k == 0) ? b[1] : ((k == 1) ? b[0] : R(0));
+    R n5 = g[k][0] * b[1] + g[k][1] * b[0];
+    R s = t0 * S00;
+    s += t1 * S11;
+    s += t2 * S22;
+    s += (l3 + n3) * S12;
+    s += (l4 + n4) * S02;
+    s += (l5 + n5) * S01;
+    r[k] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element pass.  One thread per element of one material.
+// ---------------------------------------------------------------------------
+struct ElemArgs {
+  int64_t n;              // elements in this launch
+  int64_t nel;            // elements in the mesh (connectivity stride)
+  const int32_t *filter;  // element ids, or nullptr (ids first + i)
+  int64_t first;
+  int64_t pos_offset;     // assembly position of the first element
+  const int32_t *conn;    // [nel][nnpe]
+  const double *X;        // [3][nn]
+  const double *u;        // [3][nn]
+  int64_t nn;
+  double lambda, mu, sigma_y, h;
+  const double *lam_el, *mu_el, *sy_el, *h_el; // per launch element, or null
+  void *fe;               // [slot][3] of FT
+  // materialised internals [9][n*nqp] (or null), column-major 3x3 index
+  double *grad_u, *stress;
+  // plastic history, [9][n*nqp] and [n*nqp]: previous (read) / current (write)
+  const double *p_grad, *p_sig, *p_eps, *p_iso;
+  double *c_grad, *c_sig, *c_eps, *c_iso;
+  int64_t nblocks_padded;
+};
+
+template <int ET, int LAW, class CT, class FT>
+__global__ void __launch_bounds__(256)
+k_element_forces(ElemArgs A) {
+  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
+  const int64_t i = xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
+  if (i >= A.n)
+    return;
+  const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+
+  int32_t nodes[NN];
+  {
+    const int4 *c4 = reinterpret_cast<const int4 *>(A.conn + e * NN);
+#pragma unroll
+    for (int k = 0; k < NN / 4; ++k) {
+      int4 v = c4[k];
+      nodes[4 * k] = v.x;
+      nodes[4 * k + 1] = v.y;
+      nodes[4 * k + 2] = v.z;
+      nodes[4 * k + 3] = v.w;
+    }
+  }
+
+  CT X[NN][3], U[NN][3];
+  if constexpr (sizeof(CT) == 8) {
+#pragma unroll
+    for (int n = 0; n < NN; ++n)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        X[n][c] = A.X[(int64_t)c * A.nn + nodes[n]];
+        U[n][c] = A.u[(int64_t)c * A.nn + nodes[n]];
+      }
+  } else {
+    double X0[3], U0[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      X0[c] = A.X[(int64_t)c * A.nn + nodes[0]];
+      U0[c] = A.u[(int64_t)c * A.nn + nodes[0]];
+      X[0][c] = CT(0);
+      U[0][c] = CT(0);
+    }
+#pragma unroll
+    for (int n = 1; n < NN; ++n)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        X[n][c] = CT(A.X[(int64_t)c * A.nn + nodes[n]] - X0[c]);
+        U[n][c] = CT(A.u[(int64_t)c * A.nn + nodes[n]] - U0[c]);
+      }
+  }
+
+  CT lambda, mu, sy = CT(A.sigma_y), hh = CT(A.h);
+  if (A.lam_el) {
+    lambda = CT(A.lam_el[i]);
+    mu = CT(A.mu_el[i]);
+  } else {
+    lambda = CT(A.lambda);
+    mu = CT(A.mu);
+  }
+  if (A.sy_el)
+    sy = CT(A.sy_el[i]);
+  if (A.h_el)
+    hh = CT(A.h_el[i]);
+
+  CT re[NN][3];
+#pragma unroll
+  for (int n = 0; n < NN; ++n)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      re[n][k] = CT(0);
+
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    CT Ji[3][3], detJ;
+    const CT wJ = Shape<ET, CT>::jacobian_at(X, q, Ji, detJ);
+    // grad u (i, j) = sum_n u(i, n) B(j, n), summed over n in order
+    CT g[3][3];
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+        g[ii][jj] = CT(0);
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+      CT b[3];
+      Shape<ET, CT>::bcol(Ji, q, n, b);
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+          g[ii][jj] += U[n][ii] * b[jj];
+    }
+    const int64_t iq = i * NQ + q;
+    const int64_t nq_all = A.n * NQ;
+    CT S[3][3];
+    if constexpr (LAW == LAW_ELASTIC) {
+      elastic_stress(lambda, mu, g, CT(-0.0), S);
+    } else if constexpr (LAW == LAW_ELASTIC_FD) {
+      CT E[3][3];
+      green_strain(g, E);
+      elastic_stress(lambda, mu, E, CT(-0.0), S);
+    } else if constexpr (LAW == LAW_NEOHOOKEAN) {
+      neohookean_pk2(lambda, mu, g, S);
+    } else {
+      CT pg[3][3], ps[3][3], pe[3][3], eps[3][3], iso;
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
+          pg[ii][jj] = CT(A.p_grad[o]);
+          ps[ii][jj] = CT(A.p_sig[o]);
+          pe[ii][jj] = CT(A.p_eps[o]);
+        }
+      plastic_lih(lambda, mu, sy, hh, g, pg, ps, CT(A.p_iso[iq]), pe, S, iso,
+                  eps);
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
+          A.c_grad[o] = double(g[ii][jj]);
+          A.c_sig[o] = double(S[ii][jj]);
+          A.c_eps[o] = double(eps[ii][jj]);
+        }
+      A.c_iso[iq] = double(iso);
+    }
+    if (A.grad_u) {
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
+          A.grad_u[o] = double(g[ii][jj]);
+          A.stress[o] = double(S[ii][jj]);
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+      CT r[3], b[3];
+      Shape<ET, CT>::bcol(Ji, q, n, b);
+      if constexpr (LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN)
+        btd_finite(g, S, b, r);
+      else
+        btd_small(S, b, r);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        re[n][k] += r[k] * wJ;
+    }
+  }
+
+  FT *out = reinterpret_cast<FT *>(A.fe) + (A.pos_offset + i) * NN * 3;
+#pragma unroll
+  for (int n = 0; n < NN; ++n)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      out[n * 3 + k] = FT(re[n][k]);
+}
+
+// ---------------------------------------------------------------------------
+// Nodal gather + residual + lumped solve + corrector (+ the next step's
+// predictor).  One thread per node.
+//   f_int  = sum over the node's slots, in assembly order, of -1 * r_e
+//   r      = (0 + f_ext) + f_int + M * (0 + -1 * a)
+//   x      = 1 * (r / M)                  (unblocked DOFs only)
+//   a     += 1 * x ; v += (beta dt) * x    (u += 0 * x is the identity)
+//   [predictor] u += dt v + (.5 dt)(dt a) ; v = v + dt a
+// References: dof_manager.cc:65-120 (scatter order), solid_mechanics_model.cc
+// :235-249, integration_scheme_2nd_order.cc:70-93, dof_manager_default.cc
+// :260-278, non_linear_solver_lumped.cc:76-89, newmark-beta.cc:47-77,164-197.
+// ---------------------------------------------------------------------------
+struct GatherArgs {
+  int64_t nn;
+  const int32_t *rowptr; // [nn+1]
+  const int32_t *slots;  // [nel*nnpe]
+  const void *fe;        // [slot][3]
+  double *fint;          // [3][nn]
+  const double *fext;    // [3][nn] or null (zero external force)
+  const double *mass;    // [nn]
+  const uint8_t *blocked;
+  double *u, *v, *a;     // [3][nn]
+  float *cu, *cv;        // compensation terms [3][nn] (FP32 form) or null
+  double dt;
+  int64_t nblocks_padded;
+};
+
+__device__ __forceinline__ void kahan_add(double &s, float &c, double inc) {
+  double y = inc - double(c);
+  double t = s + y;
+  c = float((t - s) - y);
+  s = t;
+}
+
+template <class FT, bool UPDATE, bool PREDICT, bool COMP>
+__global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
+  const int64_t n = xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
+  if (n >= A.nn)
+    return;
+  const FT *fe = reinterpret_cast<const FT *>(A.fe);
+  double f0 = 0., f1 = 0., f2 = 0.;
+  const int32_t s0 = A.rowptr[n], s1 = A.rowptr[n + 1];
+  for (int32_t s = s0; s < s1; ++s) {
+    const int64_t slot = A.slots[s];
+    const FT *p = fe + slot * 3;
+    f0 = f0 + -1. * double(p[0]);
+    f1 = f1 + -1. * double(p[1]);
+    f2 = f2 + -1. * double(p[2]);
+  }
+  A.fint[n] = f0;
+  A.fint[A.nn + n] = f1;
+  A.fint[2 * A.nn + n] = f2;
+  if constexpr (!UPDATE)
+    return;
+  const double M = A.mass[n];
+  const uint8_t blk = A.blocked[n];
+  const double dd = .5 * A.dt; // beta * dt
+  double f[3] = {f0, f1, f2};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if ((blk >> c) & 1)
+      continue;
+    const int64_t o = (int64_t)c * A.nn + n;
+    double fx = A.fext ? A.fext[o] : 0.;
+    double a = A.a[o], v = A.v[o];
+    double r = 0.;
+    r += 1. * fx;
+    r += 1. * f[c];
+    double cache = 0.;
+    cache += -1. * a;
+    r += M * cache;
+    const double x = 1. * (r / M);
+    a += 1. * x;
+    if constexpr (COMP) {
+      float cv = A.cv[o];
+      kahan_add(v, cv, dd * x);
+      if constexpr (PREDICT) {
+        double u = A.u[o];
+        float cu = A.cu[o];
+        const double dt_a = A.dt * a;
+        kahan_add(u, cu, A.dt * v + (.5 * A.dt) * dt_a);
+        kahan_add(v, cv, dt_a);
+        A.u[o] = u;
+        A.cu[o] = cu;
+      }
+      A.cv[o] = cv;
+    } else {
+      v += dd * x;
+      if constexpr (PREDICT) {
+        const double dt_a = A.dt * a;
+        A.u[o] = A.u[o] + (A.dt * v + (.5 * A.dt) * dt_a);
+        v = v + dt_a;
+      }
+    }
+    A.a[o] = a;
+    A.v[o] = v;
+  }
+}
+
+// Predictor alone (first step of a solve_step call): newmark-beta.cc:47-77
+template <bool COMP>
+__global__ void __launch_bounds__(256) k_predictor(GatherArgs A) {
+  const int64_t n = xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
+  if (n >= A.nn)
+    return;
+  const uint8_t blk = A.blocked[n];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if ((blk >> c) & 1)
+      continue;
+    const int64_t o = (int64_t)c * A.nn + n;
+    const double a = A.a[o];
+    double v = A.v[o], u = A.u[o];
+    const double dt_a = A.dt * a;
+    if constexpr (COMP) {
+      float cu = A.cu[o], cv = A.cv[o];
+      kahan_add(u, cu, A.dt * v + (.5 * A.dt) * dt_a);
+      kahan_add(v, cv, dt_a);
+      A.cu[o] = cu;
+      A.cv[o] = cv;
+    } else {
+      u = u + (A.dt * v + (.5 * A.dt) * dt_a);
+      v = v + dt_a;
+    }
+    A.u[o] = u;
+    A.v[o] = v;
+  }
+}
+
+} // namespace akx

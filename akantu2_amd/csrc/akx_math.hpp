@@ -1,0 +1,283 @@
+// akx_math.hpp -- per-quadrature-point arithmetic of the explicit internal
+// force, written for gfx950 device code (and usable on the host).
+//
+// The FP64 instantiation reproduces the reference's operation order so that
+// the plain path matches the CPU oracle bit for bit, apart from the last bit
+// of log() in the neo-Hookean law (device libm vs glibc).  All code here is
+// compiled with -ffp-contract=off; the FP32 precision form enables
+// contraction in its own kernels.
+//
+// Tensors are 3x3 arrays t[i][j] (row index i, column index j).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define AKX_HD __host__ __device__ __forceinline__
+
+namespace akx {
+
+template <class R> struct M3 {
+  R m[3][3];
+};
+
+// ---------------------------------------------------------------------------
+// Jacobian inverse and determinant through a partial-pivot LU.  The
+// reference's Jacobian is a dynamic-size Eigen matrix
+// (element_class_tmpl.hh:327, :345-346, :381-385), so Eigen evaluates
+// det and inverse with PartialPivLU: row pivoting on the largest magnitude
+// (first one on ties), det = sign * prod(diag U), inverse by a permuted unit
+// lower solve followed by an upper solve that scales by the reciprocal
+// pivot.  Indices are compile-time after unrolling; row swaps are selects.
+// ---------------------------------------------------------------------------
+template <class R> AKX_HD R akx_abs(R x) { return x < R(0) ? -x : x; }
+
+template <class R> AKX_HD void swap_rows_if(R (&a)[3][3], int r0, int r1, bool c) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    R x = a[r0][j], y = a[r1][j];
+    a[r0][j] = c ? y : x;
+    a[r1][j] = c ? x : y;
+  }
+}
+
+template <class R>
+AKX_HD void jacobian_lu(const R (&A)[3][3], R (&lu)[3][3], int (&perm)[3],
+                        int &nb_transp) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      lu[i][j] = A[i][j];
+  nb_transp = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int p = k;
+    R big = akx_abs(lu[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < 3; ++i) {
+      R v = akx_abs(lu[i][k]);
+      bool gt = v > big;
+      big = gt ? v : big;
+      p = gt ? i : p;
+    }
+    perm[k] = p;
+    if (big != R(0)) {
+#pragma unroll
+      for (int i = k + 1; i < 3; ++i) {
+        swap_rows_if(lu, k, i, p == i);
+        nb_transp += (p == i) ? 1 : 0;
+      }
+#pragma unroll
+      for (int i = k + 1; i < 3; ++i)
+        lu[i][k] = lu[i][k] / lu[k][k];
+    }
+#pragma unroll
+    for (int i = k + 1; i < 3; ++i)
+#pragma unroll
+      for (int j = k + 1; j < 3; ++j)
+        lu[i][j] = lu[i][j] - lu[i][k] * lu[k][j];
+  }
+}
+
+// det J and J^{-1} of the element Jacobian
+template <class R>
+AKX_HD R jacobian_det_inv(const R (&J)[3][3], R (&Ji)[3][3]) {
+  R lu[3][3];
+  int perm[3], nt;
+  jacobian_lu(J, lu, perm, nt);
+  R det = ((nt & 1) ? R(-1) : R(1)) * (lu[0][0] * lu[1][1] * lu[2][2]);
+  R x[3][3] = {{R(1), R(0), R(0)}, {R(0), R(1), R(0)}, {R(0), R(0), R(1)}};
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int i = k + 1; i < 3; ++i)
+      swap_rows_if(x, k, i, perm[k] == i);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int r = i + 1; r < 3; ++r)
+        x[r][j] = x[r][j] - x[i][j] * lu[r][i];
+#pragma unroll
+    for (int i = 2; i >= 0; --i) {
+      R a = R(1) / lu[i][i];
+      x[i][j] = x[i][j] * a;
+#pragma unroll
+      for (int r = 0; r < i; ++r)
+        x[r][j] = x[r][j] - x[i][j] * lu[r][i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      Ji[i][j] = x[i][j];
+  return det;
+}
+
+template <class R> AKX_HD R jacobian_det(const R (&J)[3][3]) {
+  R lu[3][3];
+  int perm[3], nt;
+  jacobian_lu(J, lu, perm, nt);
+  return ((nt & 1) ? R(-1) : R(1)) * (lu[0][0] * lu[1][1] * lu[2][2]);
+}
+
+// Fixed-size 3x3 determinant and inverse (Eigen's closed forms for
+// Matrix<Real,3,3>): used on F and C by the neo-Hookean law
+// (material_neohookean_inline_impl.hh:45-48).
+template <class R> AKX_HD R det3(const R (&m)[3][3]) {
+  R a = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]);
+  R b = m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]);
+  R c = m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  return a - b + c;
+}
+
+template <class R> AKX_HD R cof3(const R (&m)[3][3], int i, int j) {
+  int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+  return m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+}
+
+template <class R> AKX_HD void inv3(const R (&m)[3][3], R (&r)[3][3]) {
+  R c0 = cof3(m, 0, 0), c1 = cof3(m, 1, 0), c2 = cof3(m, 2, 0);
+  R det = c0 * m[0][0] + c1 * m[1][0] + c2 * m[2][0];
+  R invdet = R(1) / det;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      r[i][j] = cof3(m, j, i) * invdet;
+}
+
+// ---------------------------------------------------------------------------
+// Lame constants: material_elastic.cc:53-62
+template <class R> AKX_HD void lame(R E, R nu, R &lambda, R &mu) {
+  lambda = nu * E / ((R(1) + nu) * (R(1) - R(2) * nu));
+  mu = E / (R(2) * (R(1) + nu));
+}
+
+// Linear elastic law: material_elastic_inline_impl.hh:33-49 with
+// eps = .5 (g^T + g) (material_inline_impl.hh:109-113)
+template <class R>
+AKX_HD void elastic_stress(R lambda, R mu, const R (&g)[3][3], R sigma_th,
+                           R (&s)[3][3]) {
+  R trace = g[0][0] + g[1][1] + g[2][2];
+  R two_mu = R(2) * mu;
+  R sd = lambda * trace + sigma_th;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      R eps = R(.5) * (g[j][i] + g[i][j]);
+      s[i][j] = two_mu * eps + sd * (i == j ? R(1) : R(0));
+    }
+}
+
+// Green strain: material_inline_impl.hh:125-130
+template <class R> AKX_HD void green_strain(const R (&g)[3][3], R (&E)[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      R p = R(0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        p += g[k][i] * g[k][j];
+      E[i][j] = (p + g[j][i] + g[i][j]) / R(2);
+    }
+}
+
+template <class R> AKX_HD R akx_log(R x);
+template <> AKX_HD double akx_log<double>(double x) { return log(x); }
+template <> AKX_HD float akx_log<float>(float x) { return logf(x); }
+template <class R> AKX_HD R akx_sqrt(R x);
+template <> AKX_HD double akx_sqrt<double>(double x) { return sqrt(x); }
+template <> AKX_HD float akx_sqrt<float>(float x) { return sqrtf(x); }
+
+// Compressible neo-Hookean PK2: material_neohookean_inline_impl.hh:39-50
+template <class R>
+AKX_HD void neohookean_pk2(R lambda, R mu, const R (&g)[3][3], R (&S)[3][3]) {
+  R F[3][3], C[3][3], Ci[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      F[i][j] = (i == j ? R(1) : R(0)) + g[i][j];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      R s = R(0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        s += F[k][i] * F[k][j];
+      C[i][j] = s;
+    }
+  R J = det3(F) * R(1); // * sqrt(C33), C33 = 1 in 3-D
+  inv3(C, Ci);
+  R t = lambda * akx_log(J) - mu;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      S[i][j] = (i == j ? R(1) : R(0)) * mu + t * Ci[i][j];
+}
+
+// Linear isotropic hardening, small strain, radial return:
+// material_linear_isotropic_hardening_inline_impl.hh:31-86 and
+// material_plastic.hh:65-87.  Reads the previous state, writes the new one.
+template <class R>
+AKX_HD void plastic_lih(R lambda, R mu, R sigma_y, R h, const R (&g)[3][3],
+                        const R (&pg)[3][3], const R (&ps)[3][3], R piso,
+                        const R (&pe)[3][3], R (&sigma)[3][3], R &iso,
+                        R (&eps)[3][3]) {
+  R dg[3][3], str[3][3], dev[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      dg[i][j] = g[i][j] - pg[i][j];
+  elastic_stress(lambda, mu, dg, R(0), str);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      str[i][j] += ps[i][j];
+  R tr = str[0][0] + str[1][1] + str[2][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      dev[i][j] = str[i][j] - (i == j ? R(1) : R(0)) * tr / R(3);
+  // sum of squares in column-major order (cwiseProduct().sum())
+  R s = R(0);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      s += dev[i][j] * dev[i][j];
+  R eff = akx_sqrt(R(3) / R(2) * s);
+  bool yielding = ((eff - piso - sigma_y) > R(0));
+  R dp = yielding ? (eff - sigma_y - piso) / (R(3) * mu + h) : R(0);
+  iso = piso + h * dp;
+  R tol = sizeof(R) == 8 ? R(1e2 * 2.220446049250313e-16) : R(1e2 * 1.1920929e-07f);
+  bool flow = akx_abs(eff) > akx_sqrt(s) * tol;
+  R dge[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      R de = flow ? dev[i][j] * R(3) / R(2) * dp / eff : R(0);
+      eps[i][j] = pe[i][j] + de;
+      dge[i][j] = (g[i][j] - pg[i][j]) - de;
+    }
+  R sel[3][3];
+  elastic_stress(lambda, mu, dge, R(0), sel);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      sigma[i][j] = ps[i][j] + sel[i][j];
+}
+
+} // namespace akx

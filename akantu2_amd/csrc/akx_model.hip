@@ -1,0 +1,1387 @@
+// akx_model.hip -- host runtime and C ABI of the MI355X explicit internal
+// force path (declared in include/akantu_mi355x.h).
+//
+// One akx_model = one SolidMechanicsModel on one device and one stream
+// (default priority).  The per-step work is two launches:
+//   element pass (one kernel per material)  ->  nodal gather + update
+// with the predictor of step k+1 fused into the gather of step k when
+// several steps are queued together.
+#include "../../include/akantu_mi355x.h"
+#include "akx_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace akx {
+
+__constant__ Hex8Tables c_hex8;
+
+// ---------------------------------------------------------------------------
+// Errors: C++ exceptions inside, status codes across the ABI.
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+static thread_local std::string g_last_error;
+
+#define HIPCHECK(x)                                                            \
+  do {                                                                         \
+    hipError_t err_ = (x);                                                     \
+    if (err_ != hipSuccess)                                                    \
+      throw Error(err_ == hipErrorOutOfMemory ? AKX_ERR_MEMORY : AKX_ERR_HIP,  \
+                  std::string(#x) + ": " + hipGetErrorString(err_));           \
+  } while (0)
+
+static void require(bool c, const std::string &msg) {
+  if (!c)
+    throw Error(AKX_ERR_INVALID, msg);
+}
+
+// ---------------------------------------------------------------------------
+// Device buffer
+// ---------------------------------------------------------------------------
+template <class T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p)
+      return;
+    release();
+    if (count == 0)
+      return;
+    HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---------------------------------------------------------------------------
+// Layout kernels (array-of-structures at the API, structure-of-arrays on the
+// device)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_aos_to_soa(const T *aos, T *soa, int64_t n, int comps) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * comps)
+    return;
+  int64_t r = i / comps, c = i % comps;
+  soa[c * n + r] = aos[i];
+}
+
+template <class T>
+__global__ void k_soa_to_aos(const T *soa, T *aos, int64_t n, int comps) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * comps)
+    return;
+  int64_t r = i / comps, c = i % comps;
+  aos[i] = soa[c * n + r];
+}
+
+__global__ void k_bits_from_bool3(const uint8_t *b3, uint8_t *bits, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  bits[i] = (b3[3 * i] ? 1 : 0) | (b3[3 * i + 1] ? 2 : 0) | (b3[3 * i + 2] ? 4 : 0);
+}
+
+__global__ void k_bool3_from_bits(const uint8_t *bits, uint8_t *b3, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  uint8_t b = bits[i];
+  b3[3 * i] = b & 1;
+  b3[3 * i + 1] = (b >> 1) & 1;
+  b3[3 * i + 2] = (b >> 2) & 1;
+}
+
+__global__ void k_mass_to3(const double *m, double *m3, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  m3[3 * i] = m[i];
+  m3[3 * i + 1] = m[i];
+  m3[3 * i + 2] = m[i];
+}
+
+__global__ void k_fill_f32(float *p, int64_t n, float v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = v;
+}
+__global__ void k_fill_f64(double *p, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = v;
+}
+
+// Lame constants per element from E, nu (material_elastic.cc:53-62)
+__global__ void k_lame(const double *E, const double *nu, double E0, double nu0,
+                       double *lam, double *mu, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  double l, m;
+  lame(E ? E[i] : E0, nu ? nu[i] : nu0, l, m);
+  lam[i] = l;
+  mu[i] = m;
+}
+
+// ---------------------------------------------------------------------------
+// Init kernels: jacobian check, lumped mass, stable time step
+// ---------------------------------------------------------------------------
+struct InitArgs {
+  int64_t n, nel, first, pos_offset, nn;
+  const int32_t *filter, *conn;
+  const double *X, *u;
+  double rho;
+  double lambda, mu;
+  const double *lam_el, *mu_el;
+  double *me; // [slot] mass contributions
+  unsigned long long *min_bits;
+  long long *first_bad;
+};
+
+template <int ET>
+__device__ __forceinline__ void load_element_X(const InitArgs &A, int64_t e,
+                                               bool add_u,
+                                               double (&X)[Elem<ET>::nnpe][3]) {
+#pragma unroll
+  for (int n = 0; n < Elem<ET>::nnpe; ++n) {
+    int32_t node = A.conn[e * Elem<ET>::nnpe + n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double x = A.X[(int64_t)c * A.nn + node];
+      X[n][c] = add_u ? x + A.u[(int64_t)c * A.nn + node] : x;
+    }
+  }
+}
+
+// IntegratorGauss::checkJacobians over every element of the mesh
+template <int ET> __global__ void k_check_jacobians(InitArgs A) {
+  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= A.nel)
+    return;
+  double X[NN][3];
+  load_element_X<ET>(A, e, false, X);
+  for (int q = 0; q < NQ; ++q) {
+    double Ji[3][3], detJ;
+    Shape<ET, double>::jacobian_at(X, q, Ji, detJ);
+    if (detJ < 0.) {
+      atomicMin(A.first_bad, (long long)(e * NQ + q));
+      return;
+    }
+  }
+}
+
+// Lumped mass contributions m_e(a) = sum_q (rho N_a(q)) wJ_q
+// (fe_engine_template_tmpl_field.hh:112-138)
+template <int ET> __global__ void k_mass_elements(InitArgs A) {
+  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n)
+    return;
+  int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+  double X[NN][3];
+  load_element_X<ET>(A, e, false, X);
+  double m[NN];
+#pragma unroll
+  for (int n = 0; n < NN; ++n)
+    m[n] = 0.;
+  for (int q = 0; q < NQ; ++q) {
+    double Ji[3][3], detJ;
+    double wJ = Shape<ET, double>::jacobian_at(X, q, Ji, detJ);
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+      double N = (ET == ET_TET4) ? .25 : c_hex8.N[q][n];
+      m[n] += (A.rho * N) * wJ;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < NN; ++n)
+    A.me[(A.pos_offset + i) * NN + n] = m[n];
+}
+
+// Inradius: Tet4 2 * Math::tetrahedron_inradius (aka_math_tmpl.hh:71-124),
+// Hex8 min of the 12 edges (element_class_hexahedron_8_inline_impl.hh:194-212)
+__device__ __forceinline__ double dist3(const double (&a)[3], const double (&b)[3]) {
+  double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ __forceinline__ double tet_volume(const double (&c1)[3],
+                                             const double (&c2)[3],
+                                             const double (&c3)[3],
+                                             const double (&c4)[3]) {
+  double xx[3][3];
+  auto setcol = [&](int k, const double (&c)[3]) {
+    xx[0][k] = c[0];
+    xx[1][k] = c[1];
+    xx[2][k] = c[2];
+  };
+  setcol(0, c2);
+  setcol(1, c3);
+  setcol(2, c4);
+  double vol = det3(xx);
+  setcol(0, c1);
+  vol -= det3(xx);
+  setcol(1, c2);
+  vol += det3(xx);
+  setcol(2, c3);
+  vol -= det3(xx);
+  vol /= 6;
+  return vol;
+}
+
+template <int ET>
+__device__ __forceinline__ double inradius(const double (&X)[Elem<ET>::nnpe][3]) {
+  if constexpr (ET == ET_TET4) {
+    double l12 = dist3(X[0], X[1]), l13 = dist3(X[0], X[2]), l14 = dist3(X[0], X[3]);
+    double l23 = dist3(X[1], X[2]), l24 = dist3(X[1], X[3]), l34 = dist3(X[2], X[3]);
+    double s1 = (l12 + l23 + l13) * 0.5;
+    s1 = sqrt(s1 * (s1 - l12) * (s1 - l23) * (s1 - l13));
+    double s2 = (l12 + l24 + l14) * 0.5;
+    s2 = sqrt(s2 * (s2 - l12) * (s2 - l24) * (s2 - l14));
+    double s3 = (l23 + l34 + l24) * 0.5;
+    s3 = sqrt(s3 * (s3 - l23) * (s3 - l34) * (s3 - l24));
+    double s4 = (l13 + l34 + l14) * 0.5;
+    s4 = sqrt(s4 * (s4 - l13) * (s4 - l34) * (s4 - l14));
+    double volume = tet_volume(X[0], X[1], X[2], X[3]);
+    return 2. * (3 * volume / (s1 + s2 + s3 + s4));
+  } else {
+    const int ed[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
+                           {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}};
+    double p = dist3(X[0], X[1]);
+#pragma unroll
+    for (int k = 1; k < 12; ++k) {
+      double l = dist3(X[ed[k][0]], X[ed[k][1]]);
+      p = (l < p) ? l : p;
+    }
+    return p;
+  }
+}
+
+// min_e h_e / c_e over one material (solid_mechanics_model.cc:423-473)
+template <int ET> __global__ void __launch_bounds__(256) k_stable_dt(InitArgs A) {
+  constexpr int NN = Elem<ET>::nnpe;
+  __shared__ double red[256];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double dt = 1.7976931348623157e308;
+  if (i < A.n) {
+    int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+    double X[NN][3];
+    load_element_X<ET>(A, e, true, X);
+    double lambda = A.lam_el ? A.lam_el[i] : A.lambda;
+    double mu = A.mu_el ? A.mu_el[i] : A.mu;
+    double h = inradius<ET>(X);
+    double c = sqrt((lambda + 2 * mu) / A.rho);
+    dt = h / c;
+  }
+  red[threadIdx.x] = dt;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // positive doubles order like their bit patterns
+    atomicMin(A.min_bits, (unsigned long long)__double_as_longlong(red[0]));
+  }
+}
+
+// Lumped mass gather: M = 0 + sum 1 * m_e over the node's slots in order
+__global__ void k_gather_mass(int64_t nn, const int32_t *rowptr,
+                              const int32_t *slots, const double *me,
+                              double *mass) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn)
+    return;
+  double m = 0.;
+  for (int32_t s = rowptr[n]; s < rowptr[n + 1]; ++s)
+    m += 1. * me[slots[s]];
+  mass[n] = m;
+}
+
+} // namespace akx
+
+using namespace akx;
+
+// ---------------------------------------------------------------------------
+// Model
+// ---------------------------------------------------------------------------
+struct Material {
+  int law = 0;
+  double rho = 0, E = 0, nu = 0, sigma_y = 0, h = 0;
+  double lambda = 0, mu = 0;
+  int64_t n = 0;          // elements in the filter
+  bool contiguous = true; // filter == first..first+n-1
+  int64_t first = 0;
+  int64_t pos_offset = 0; // assembly position of its first element
+  std::vector<int32_t> filter_host;
+  DevBuf<int32_t> filter;
+  DevBuf<double> lam_el, mu_el, sy_el, h_el;
+  bool per_element = false;
+  // materialised internals and plastic history [2][9][n*nqp] / [2][n*nqp]
+  DevBuf<double> grad, stress; // materialised by compute_all_stresses
+  DevBuf<double> hgrad[2], hsig[2], heps[2], hiso[2];
+  int cur = 0; // index of the current plastic state
+};
+
+struct akx_model {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int etype = -1, nnpe = 0, nqp = 0;
+  int64_t nn = 0, nel = 0;
+  int precision = AKX_PRECISION_FP64;
+  bool initialized = false;
+  bool has_fext = false;
+  double dt = 0.;
+  std::vector<int32_t> conn_host; // AoS copy for filters and the gather table
+  DevBuf<double> X, u, v, a, fint, fext, mass;
+  DevBuf<uint8_t> blocked;
+  DevBuf<float> cu, cv;
+  DevBuf<int32_t> conn, rowptr, slots;
+  DevBuf<unsigned char> fe; // [slots][3] of double or float
+  DevBuf<unsigned char> staging;
+  DevBuf<unsigned long long> scratch_u64;
+  DevBuf<long long> scratch_i64;
+  std::vector<std::unique_ptr<Material>> mats;
+  // kernel timing
+  bool timing = false;
+  double t_ms[3] = {0, 0, 0};
+  int64_t t_n[3] = {0, 0, 0};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[3];
+  std::vector<hipEvent_t> event_pool;
+
+  ~akx_model() {
+    drain_timing();
+    for (auto e : event_pool)
+      (void)hipEventDestroy(e);
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+
+  void set_device() { HIPCHECK(hipSetDevice(device)); }
+
+  hipEvent_t get_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+
+  void drain_timing() {
+    for (int k = 0; k < 3; ++k) {
+      for (auto &pr : pending[k]) {
+        if (hipEventSynchronize(pr.second) == hipSuccess) {
+          float ms = 0.f;
+          if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+            t_ms[k] += ms;
+            t_n[k] += 1;
+          }
+        }
+        event_pool.push_back(pr.first);
+        event_pool.push_back(pr.second);
+      }
+      pending[k].clear();
+    }
+  }
+
+  template <class F> void timed(int cls, F &&launch) {
+    if (!timing) {
+      launch();
+      return;
+    }
+    hipEvent_t e0 = get_event(), e1 = get_event();
+    HIPCHECK(hipEventRecord(e0, stream));
+    launch();
+    HIPCHECK(hipEventRecord(e1, stream));
+    pending[cls].push_back({e0, e1});
+    if (pending[cls].size() > 4096)
+      drain_timing();
+  }
+
+  size_t fe_elem_size() const {
+    return precision == AKX_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+  }
+};
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+inline int64_t padded_blocks(int64_t n) {
+  int64_t nb = (n + TPB - 1) / TPB;
+  return (nb + 7) / 8 * 8;
+}
+
+void upload_hex8_tables() {
+  // element_class_hexahedron_8_inline_impl.hh:111-188 at the tensor-product
+  // Gauss points (xi fastest), gauss_integration.cc:42-43
+  static bool done_device[64] = {false};
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 64 && done_device[dev])
+    return;
+  Hex8Tables t;
+  double pos[2] = {-1. / std::sqrt(3.), 1. / std::sqrt(3.)};
+  for (int q = 0; q < 8; ++q) {
+    double c[3] = {pos[q & 1], pos[(q >> 1) & 1], pos[(q >> 2) & 1]};
+    double(*d)[8] = t.dnds[q];
+    d[0][0] = -.125 * (1 - c[1]) * (1 - c[2]);
+    d[0][1] = .125 * (1 - c[1]) * (1 - c[2]);
+    d[0][2] = .125 * (1 + c[1]) * (1 - c[2]);
+    d[0][3] = -.125 * (1 + c[1]) * (1 - c[2]);
+    d[0][4] = -.125 * (1 - c[1]) * (1 + c[2]);
+    d[0][5] = .125 * (1 - c[1]) * (1 + c[2]);
+    d[0][6] = .125 * (1 + c[1]) * (1 + c[2]);
+    d[0][7] = -.125 * (1 + c[1]) * (1 + c[2]);
+    d[1][0] = -.125 * (1 - c[0]) * (1 - c[2]);
+    d[1][1] = -.125 * (1 + c[0]) * (1 - c[2]);
+    d[1][2] = .125 * (1 + c[0]) * (1 - c[2]);
+    d[1][3] = .125 * (1 - c[0]) * (1 - c[2]);
+    d[1][4] = -.125 * (1 - c[0]) * (1 + c[2]);
+    d[1][5] = -.125 * (1 + c[0]) * (1 + c[2]);
+    d[1][6] = .125 * (1 + c[0]) * (1 + c[2]);
+    d[1][7] = .125 * (1 - c[0]) * (1 + c[2]);
+    d[2][0] = -.125 * (1 - c[0]) * (1 - c[1]);
+    d[2][1] = -.125 * (1 + c[0]) * (1 - c[1]);
+    d[2][2] = -.125 * (1 + c[0]) * (1 + c[1]);
+    d[2][3] = -.125 * (1 - c[0]) * (1 + c[1]);
+    d[2][4] = .125 * (1 - c[0]) * (1 - c[1]);
+    d[2][5] = .125 * (1 + c[0]) * (1 - c[1]);
+    d[2][6] = .125 * (1 + c[0]) * (1 + c[1]);
+    d[2][7] = .125 * (1 - c[0]) * (1 + c[1]);
+    t.N[q][0] = .125 * (1 - c[0]) * (1 - c[1]) * (1 - c[2]);
+    t.N[q][1] = .125 * (1 + c[0]) * (1 - c[1]) * (1 - c[2]);
+    t.N[q][2] = .125 * (1 + c[0]) * (1 + c[1]) * (1 - c[2]);
+    t.N[q][3] = .125 * (1 - c[0]) * (1 + c[1]) * (1 - c[2]);
+    t.N[q][4] = .125 * (1 - c[0]) * (1 - c[1]) * (1 + c[2]);
+    t.N[q][5] = .125 * (1 + c[0]) * (1 - c[1]) * (1 + c[2]);
+    t.N[q][6] = .125 * (1 + c[0]) * (1 + c[1]) * (1 + c[2]);
+    t.N[q][7] = .125 * (1 - c[0]) * (1 + c[1]) * (1 + c[2]);
+    double wq = 1.;
+    wq *= 1.;
+    wq *= 1.;
+    wq *= 1.;
+    t.w[q] = wq;
+    for (int i = 0; i < 3; ++i)
+      for (int n = 0; n < 8; ++n)
+        t.dnds_f[q][i][n] = (float)d[i][n];
+  }
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_hex8), &t, sizeof(t)));
+  if (dev < 64)
+    done_device[dev] = true;
+}
+
+template <class T>
+void upload_soa(akx_model *m, const T *host_aos, T *dev_soa, int64_t n, int comps) {
+  size_t bytes = (size_t)n * comps * sizeof(T);
+  m->staging.alloc(std::max(m->staging.n, bytes));
+  HIPCHECK(hipMemcpyAsync(m->staging.p, host_aos, bytes, hipMemcpyHostToDevice,
+                          m->stream));
+  hipLaunchKernelGGL(k_aos_to_soa<T>, dim3(grid_for(n * comps)), dim3(TPB), 0,
+                     m->stream, (const T *)m->staging.p, dev_soa, n, comps);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(m->stream));
+}
+
+template <class T>
+void download_soa(akx_model *m, const T *dev_soa, T *host_aos, int64_t n, int comps) {
+  size_t bytes = (size_t)n * comps * sizeof(T);
+  m->staging.alloc(std::max(m->staging.n, bytes));
+  hipLaunchKernelGGL(k_soa_to_aos<T>, dim3(grid_for(n * comps)), dim3(TPB), 0,
+                     m->stream, dev_soa, (T *)m->staging.p, n, comps);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(host_aos, m->staging.p, bytes, hipMemcpyDeviceToHost,
+                          m->stream));
+  HIPCHECK(hipStreamSynchronize(m->stream));
+}
+
+void zero_async(akx_model *m, double *p, int64_t n) {
+  if (n <= 0)
+    return;
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n)), dim3(TPB), 0, m->stream, p, n, 0.);
+  HIPCHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------- launches ----
+ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
+  ElemArgs A{};
+  A.n = mat.n;
+  A.nel = m->nel;
+  A.filter = mat.contiguous ? nullptr : mat.filter.p;
+  A.first = mat.first;
+  A.pos_offset = mat.pos_offset;
+  A.conn = m->conn.p;
+  A.X = m->X.p;
+  A.u = m->u.p;
+  A.nn = m->nn;
+  A.lambda = mat.lambda;
+  A.mu = mat.mu;
+  A.sigma_y = mat.sigma_y;
+  A.h = mat.h;
+  A.lam_el = mat.per_element ? mat.lam_el.p : nullptr;
+  A.mu_el = mat.per_element ? mat.mu_el.p : nullptr;
+  A.sy_el = mat.sy_el.p;
+  A.h_el = mat.h_el.p;
+  A.fe = m->fe.p;
+  if (materialize && mat.law != AKX_LAW_PLASTIC_LIH) {
+    A.grad_u = mat.grad.p;
+    A.stress = mat.stress.p;
+  }
+  if (mat.law == AKX_LAW_PLASTIC_LIH) {
+    int p = mat.cur, c = 1 - mat.cur;
+    A.p_grad = mat.hgrad[p].p;
+    A.p_sig = mat.hsig[p].p;
+    A.p_eps = mat.heps[p].p;
+    A.p_iso = mat.hiso[p].p;
+    A.c_grad = mat.hgrad[c].p;
+    A.c_sig = mat.hsig[c].p;
+    A.c_eps = mat.heps[c].p;
+    A.c_iso = mat.hiso[c].p;
+  }
+  A.nblocks_padded = padded_blocks(mat.n);
+  return A;
+}
+
+template <int ET, class CT, class FT>
+void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
+  dim3 grid((unsigned)A.nblocks_padded), block(TPB);
+  switch (mat.law) {
+  case AKX_LAW_ELASTIC:
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_ELASTIC, CT, FT>), grid, block,
+                       0, m->stream, A);
+    break;
+  case AKX_LAW_ELASTIC_FD:
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_ELASTIC_FD, CT, FT>), grid,
+                       block, 0, m->stream, A);
+    break;
+  case AKX_LAW_NEOHOOKEAN:
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_NEOHOOKEAN, CT, FT>), grid,
+                       block, 0, m->stream, A);
+    break;
+  case AKX_LAW_PLASTIC_LIH:
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_PLASTIC, CT, FT>), grid, block,
+                       0, m->stream, A);
+    break;
+  default:
+    throw Error(AKX_ERR_UNSUPPORTED, "unknown law");
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_element_pass(akx_model *m, bool materialize) {
+  for (auto &mp : m->mats) {
+    Material &mat = *mp;
+    if (mat.n == 0)
+      continue;
+    ElemArgs A = elem_args(m, mat, materialize);
+    m->timed(0, [&] {
+      bool f64 = m->precision == AKX_PRECISION_FP64;
+      if (m->etype == AKX_TET4) {
+        if (f64)
+          launch_elem_law<ET_TET4, double, double>(m, mat, A);
+        else
+          launch_elem_law<ET_TET4, float, float>(m, mat, A);
+      } else {
+        if (f64)
+          launch_elem_law<ET_HEX8, double, double>(m, mat, A);
+        else
+          launch_elem_law<ET_HEX8, float, float>(m, mat, A);
+      }
+    });
+  }
+}
+
+void flip_plastic_state(akx_model *m) {
+  for (auto &mp : m->mats)
+    if (mp->law == AKX_LAW_PLASTIC_LIH)
+      mp->cur = 1 - mp->cur;
+}
+
+GatherArgs gather_args(akx_model *m) {
+  GatherArgs G{};
+  G.nn = m->nn;
+  G.rowptr = m->rowptr.p;
+  G.slots = m->slots.p;
+  G.fe = m->fe.p;
+  G.fint = m->fint.p;
+  G.fext = m->has_fext ? m->fext.p : nullptr;
+  G.mass = m->mass.p;
+  G.blocked = m->blocked.p;
+  G.u = m->u.p;
+  G.v = m->v.p;
+  G.a = m->a.p;
+  G.cu = m->cu.p;
+  G.cv = m->cv.p;
+  G.dt = m->dt;
+  G.nblocks_padded = padded_blocks(m->nn);
+  return G;
+}
+
+void launch_gather(akx_model *m, bool update, bool predict) {
+  GatherArgs G = gather_args(m);
+  dim3 grid((unsigned)G.nblocks_padded), block(TPB);
+  bool f64 = m->precision == AKX_PRECISION_FP64;
+  m->timed(1, [&] {
+    if (!update) {
+      if (f64)
+        hipLaunchKernelGGL((k_gather_update<double, false, false, false>), grid,
+                           block, 0, m->stream, G);
+      else
+        hipLaunchKernelGGL((k_gather_update<float, false, false, false>), grid,
+                           block, 0, m->stream, G);
+    } else if (f64) {
+      if (predict)
+        hipLaunchKernelGGL((k_gather_update<double, true, true, false>), grid,
+                           block, 0, m->stream, G);
+      else
+        hipLaunchKernelGGL((k_gather_update<double, true, false, false>), grid,
+                           block, 0, m->stream, G);
+    } else {
+      if (predict)
+        hipLaunchKernelGGL((k_gather_update<float, true, true, true>), grid,
+                           block, 0, m->stream, G);
+      else
+        hipLaunchKernelGGL((k_gather_update<float, true, false, true>), grid,
+                           block, 0, m->stream, G);
+    }
+  });
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_predictor(akx_model *m) {
+  GatherArgs G = gather_args(m);
+  dim3 grid((unsigned)G.nblocks_padded), block(TPB);
+  m->timed(2, [&] {
+    if (m->precision == AKX_PRECISION_FP64)
+      hipLaunchKernelGGL((k_predictor<false>), grid, block, 0, m->stream, G);
+    else
+      hipLaunchKernelGGL((k_predictor<true>), grid, block, 0, m->stream, G);
+  });
+  HIPCHECK(hipGetLastError());
+}
+
+InitArgs init_args(akx_model *m, Material *mat) {
+  InitArgs A{};
+  A.nel = m->nel;
+  A.nn = m->nn;
+  A.conn = m->conn.p;
+  A.X = m->X.p;
+  A.u = m->u.p;
+  if (mat) {
+    A.n = mat->n;
+    A.first = mat->first;
+    A.pos_offset = mat->pos_offset;
+    A.filter = mat->contiguous ? nullptr : mat->filter.p;
+    A.rho = mat->rho;
+    A.lambda = mat->lambda;
+    A.mu = mat->mu;
+    A.lam_el = mat->per_element ? mat->lam_el.p : nullptr;
+    A.mu_el = mat->per_element ? mat->mu_el.p : nullptr;
+  }
+  return A;
+}
+
+void check_ready(akx_model *m) {
+  require(m != nullptr, "null model");
+  require(m->initialized, "initFull has not been called");
+}
+
+// Gather table: for each node the slots (position * nnpe + local node) in
+// ascending assembly order -- materials in creation order, each material's
+// filter in order -- which is the order of the reference's sequential
+// scatter (dof_manager.cc:92-114 called per material by
+// material.cc:98-120).  MeshUtils::buildNode2Elements (mesh_utils.cc:37-82)
+// builds the same node -> element CSR.
+void build_gather_table(akx_model *m) {
+  int64_t nslots = 0;
+  for (auto &mp : m->mats)
+    nslots += mp->n * m->nnpe;
+  require(nslots < (int64_t)INT32_MAX, "too many element slots for int32 indices");
+  std::vector<int32_t> rowptr(m->nn + 1, 0), slots(nslots);
+  for (auto &mp : m->mats)
+    for (int64_t i = 0; i < mp->n; ++i) {
+      int64_t e = mp->contiguous ? mp->first + i : mp->filter_host[i];
+      for (int a = 0; a < m->nnpe; ++a)
+        rowptr[m->conn_host[e * m->nnpe + a] + 1]++;
+    }
+  for (int64_t n = 0; n < m->nn; ++n)
+    rowptr[n + 1] += rowptr[n];
+  std::vector<int32_t> fill(rowptr.begin(), rowptr.end() - 1);
+  for (auto &mp : m->mats)
+    for (int64_t i = 0; i < mp->n; ++i) {
+      int64_t e = mp->contiguous ? mp->first + i : mp->filter_host[i];
+      int64_t pos = mp->pos_offset + i;
+      for (int a = 0; a < m->nnpe; ++a) {
+        int32_t node = m->conn_host[e * m->nnpe + a];
+        slots[fill[node]++] = (int32_t)(pos * m->nnpe + a);
+      }
+    }
+  m->rowptr.alloc(m->nn + 1);
+  m->slots.alloc(std::max<int64_t>(nslots, 1));
+  HIPCHECK(hipMemcpy(m->rowptr.p, rowptr.data(), rowptr.size() * sizeof(int32_t),
+                     hipMemcpyHostToDevice));
+  if (nslots)
+    HIPCHECK(hipMemcpy(m->slots.p, slots.data(), nslots * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+}
+
+void alloc_internals(akx_model *m, Material &mat) {
+  int64_t nq = mat.n * m->nqp;
+  if (mat.law == AKX_LAW_PLASTIC_LIH) {
+    for (int s = 0; s < 2; ++s) {
+      mat.hgrad[s].alloc(9 * nq);
+      mat.hsig[s].alloc(9 * nq);
+      mat.heps[s].alloc(9 * nq);
+      mat.hiso[s].alloc(nq);
+      zero_async(m, mat.hgrad[s].p, 9 * nq);
+      zero_async(m, mat.hsig[s].p, 9 * nq);
+      zero_async(m, mat.heps[s].p, 9 * nq);
+      zero_async(m, mat.hiso[s].p, nq);
+    }
+    mat.cur = 0;
+  }
+}
+
+template <class F> int guard(F &&f) {
+  try {
+    f();
+    return AKX_OK;
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return AKX_ERR_MEMORY;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return AKX_ERR_INVALID;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return AKX_ERR_INVALID;
+  }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *akx_last_error(void) { return g_last_error.c_str(); }
+
+int akx_version(void) { return 100; }
+
+int akx_device_count(int *count) {
+  return guard([&] {
+    require(count != nullptr, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+      n = 0;
+    int good = 0;
+    for (int d = 0; d < n; ++d) {
+      hipDeviceProp_t p;
+      if (hipGetDeviceProperties(&p, d) == hipSuccess &&
+          std::strncmp(p.gcnArchName, "gfx950", 6) == 0)
+        ++good;
+    }
+    *count = good;
+  });
+}
+
+int akx_model_create(int device, akx_model **out) {
+  return guard([&] {
+    require(out != nullptr, "null output pointer");
+    auto m = std::make_unique<akx_model>();
+    m->device = device;
+    m->set_device();
+    hipDeviceProp_t p;
+    HIPCHECK(hipGetDeviceProperties(&p, device));
+    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+      throw Error(AKX_ERR_UNSUPPORTED,
+                  std::string("device is not gfx950: ") + p.gcnArchName);
+    HIPCHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    m->scratch_u64.alloc(4);
+    m->scratch_i64.alloc(4);
+    *out = m.release();
+  });
+}
+
+int akx_model_destroy(akx_model *m) {
+  return guard([&] {
+    if (!m)
+      return;
+    (void)hipSetDevice(m->device);
+    if (m->stream)
+      (void)hipStreamSynchronize(m->stream);
+    delete m;
+  });
+}
+
+int akx_model_set_mesh(akx_model *m, int element_type, int64_t nb_nodes,
+                       const double *nodes, int64_t nb_elements,
+                       const int32_t *connectivity) {
+  return guard([&] {
+    require(m != nullptr, "null model");
+    require(!m->initialized, "mesh already initialised");
+    require(element_type == AKX_TET4 || element_type == AKX_HEX8,
+            "element type must be _tetrahedron_4 or _hexahedron_8");
+    require(nb_nodes > 0 && nb_elements > 0, "empty mesh");
+    require(nb_nodes < INT32_MAX, "too many nodes for int32 connectivity");
+    require(nodes && connectivity, "null mesh arrays");
+    m->set_device();
+    m->etype = element_type;
+    m->nnpe = element_type == AKX_TET4 ? 4 : 8;
+    m->nqp = element_type == AKX_TET4 ? 1 : 8;
+    m->nn = nb_nodes;
+    m->nel = nb_elements;
+    m->conn_host.assign(connectivity, connectivity + nb_elements * m->nnpe);
+    for (int32_t c : m->conn_host)
+      require(c >= 0 && c < nb_nodes, "connectivity refers to a missing node");
+    m->X.alloc(3 * nb_nodes);
+    m->u.alloc(3 * nb_nodes);
+    m->v.alloc(3 * nb_nodes);
+    m->a.alloc(3 * nb_nodes);
+    m->fint.alloc(3 * nb_nodes);
+    m->mass.alloc(nb_nodes);
+    m->blocked.alloc(nb_nodes);
+    m->conn.alloc(nb_elements * m->nnpe);
+    upload_soa(m, nodes, m->X.p, nb_nodes, 3);
+    HIPCHECK(hipMemcpy(m->conn.p, connectivity,
+                       nb_elements * m->nnpe * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+    zero_async(m, m->u.p, 3 * nb_nodes);
+    zero_async(m, m->v.p, 3 * nb_nodes);
+    zero_async(m, m->a.p, 3 * nb_nodes);
+    zero_async(m, m->fint.p, 3 * nb_nodes);
+    zero_async(m, m->mass.p, nb_nodes);
+    HIPCHECK(hipMemsetAsync(m->blocked.p, 0, nb_nodes, m->stream));
+    if (m->etype == AKX_HEX8)
+      upload_hex8_tables();
+    HIPCHECK(hipStreamSynchronize(m->stream));
+  });
+}
+
+int akx_material_law_by_name(const char *name, int finite_deformation, int *law) {
+  return guard([&] {
+    require(name && law, "null argument");
+    std::string s(name);
+    if (s == "elastic")
+      *law = finite_deformation ? AKX_LAW_ELASTIC_FD : AKX_LAW_ELASTIC;
+    else if (s == "neohookean")
+      *law = AKX_LAW_NEOHOOKEAN;
+    else if (s == "plastic_linear_isotropic_hardening")
+      *law = AKX_LAW_PLASTIC_LIH;
+    else
+      throw Error(AKX_ERR_UNSUPPORTED, "unknown material law: " + s);
+  });
+}
+
+int akx_model_add_material(akx_model *m, int law, const double *params,
+                           int nparams, const int32_t *filter,
+                           int64_t nb_filter, int *material_id) {
+  return guard([&] {
+    require(m != nullptr && m->etype >= 0, "set the mesh first");
+    require(!m->initialized, "materials must be added before initFull");
+    require(law >= AKX_LAW_ELASTIC && law <= AKX_LAW_PLASTIC_LIH, "unknown law");
+    require(params && nparams >= 3, "params = {rho, E, nu[, sigma_y, h]}");
+    m->set_device();
+    auto mat = std::make_unique<Material>();
+    mat->law = law;
+    mat->rho = params[0];
+    mat->E = params[1];
+    mat->nu = params[2];
+    mat->sigma_y = nparams > 3 ? params[3] : 0.;
+    mat->h = nparams > 4 ? params[4] : 0.;
+    require(mat->rho > 0, "rho must be positive");
+    lame(mat->E, mat->nu, mat->lambda, mat->mu);
+    int64_t pos = 0;
+    for (auto &o : m->mats)
+      pos += o->n;
+    mat->pos_offset = pos;
+    if (filter == nullptr) {
+      mat->n = m->nel;
+      mat->first = 0;
+      mat->contiguous = true;
+    } else {
+      require(nb_filter >= 0, "negative filter size");
+      mat->n = nb_filter;
+      mat->filter_host.assign(filter, filter + nb_filter);
+      bool contig = true;
+      for (int64_t i = 0; i < nb_filter; ++i) {
+        require(filter[i] >= 0 && filter[i] < m->nel, "filter refers to a missing element");
+        if (filter[i] != filter[0] + i)
+          contig = false;
+      }
+      mat->contiguous = contig;
+      mat->first = nb_filter ? filter[0] : 0;
+      if (!contig && nb_filter) {
+        mat->filter.alloc(nb_filter);
+        HIPCHECK(hipMemcpy(mat->filter.p, filter, nb_filter * sizeof(int32_t),
+                           hipMemcpyHostToDevice));
+      }
+    }
+    if (material_id)
+      *material_id = (int)m->mats.size();
+    m->mats.push_back(std::move(mat));
+  });
+}
+
+int akx_model_set_element_parameters(akx_model *m, int material_id,
+                                     const double *E, const double *nu,
+                                     const double *sigma_y, const double *h) {
+  return guard([&] {
+    require(m != nullptr, "null model");
+    require(material_id >= 0 && material_id < (int)m->mats.size(), "bad material id");
+    m->set_device();
+    Material &mat = *m->mats[material_id];
+    int64_t n = mat.n;
+    if (n == 0)
+      return;
+    auto up = [&](DevBuf<double> &b, const double *src) {
+      b.alloc(n);
+      HIPCHECK(hipMemcpy(b.p, src, n * sizeof(double), hipMemcpyHostToDevice));
+    };
+    if (E || nu) {
+      DevBuf<double> dE, dnu;
+      if (E)
+        up(dE, E);
+      if (nu)
+        up(dnu, nu);
+      mat.lam_el.alloc(n);
+      mat.mu_el.alloc(n);
+      hipLaunchKernelGGL(k_lame, dim3(grid_for(n)), dim3(TPB), 0, m->stream,
+                         (const double *)dE.p, (const double *)dnu.p, mat.E,
+                         mat.nu, mat.lam_el.p, mat.mu_el.p, n);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(m->stream));
+      mat.per_element = true;
+    }
+    if (sigma_y)
+      up(mat.sy_el, sigma_y);
+    if (h)
+      up(mat.h_el, h);
+  });
+}
+
+int akx_model_set_precision(akx_model *m, int precision) {
+  return guard([&] {
+    require(m != nullptr, "null model");
+    require(!m->initialized, "precision must be set before initFull");
+    require(precision == AKX_PRECISION_FP64 || precision == AKX_PRECISION_FP32,
+            "unknown precision");
+    m->precision = precision;
+  });
+}
+
+int akx_model_init_full(akx_model *m) {
+  return guard([&] {
+    require(m != nullptr && m->etype >= 0, "set the mesh first");
+    require(!m->initialized, "initFull called twice");
+    require(!m->mats.empty(), "no material");
+    m->set_device();
+    int64_t nslots = 0;
+    for (auto &mp : m->mats)
+      nslots += mp->n * m->nnpe;
+    // IntegratorGauss::checkJacobians
+    {
+      InitArgs A = init_args(m, nullptr);
+      long long init = INT64_MAX;
+      HIPCHECK(hipMemcpy(m->scratch_i64.p, &init, sizeof(init), hipMemcpyHostToDevice));
+      A.first_bad = m->scratch_i64.p;
+      if (m->etype == AKX_TET4)
+        hipLaunchKernelGGL(k_check_jacobians<ET_TET4>, dim3(grid_for(m->nel)),
+                           dim3(TPB), 0, m->stream, A);
+      else
+        hipLaunchKernelGGL(k_check_jacobians<ET_HEX8>, dim3(grid_for(m->nel)),
+                           dim3(TPB), 0, m->stream, A);
+      HIPCHECK(hipGetLastError());
+      long long bad = 0;
+      HIPCHECK(hipMemcpyAsync(&bad, m->scratch_i64.p, sizeof(bad),
+                              hipMemcpyDeviceToHost, m->stream));
+      HIPCHECK(hipStreamSynchronize(m->stream));
+      if (bad != INT64_MAX) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "Negative jacobian computed, possible problem in the "
+                      "element node ordering (Quadrature Point %lld:%lld)",
+                      bad % m->nqp, bad / m->nqp);
+        throw Error(AKX_ERR_JACOBIAN, msg);
+      }
+    }
+    build_gather_table(m);
+    m->fe.alloc(std::max<int64_t>(nslots, 1) * 3 * m->fe_elem_size());
+    // lumped mass
+    {
+      DevBuf<double> me;
+      me.alloc(std::max<int64_t>(nslots, 1));
+      for (auto &mp : m->mats) {
+        if (mp->n == 0)
+          continue;
+        InitArgs A = init_args(m, mp.get());
+        A.me = me.p;
+        if (m->etype == AKX_TET4)
+          hipLaunchKernelGGL(k_mass_elements<ET_TET4>, dim3(grid_for(mp->n)),
+                             dim3(TPB), 0, m->stream, A);
+        else
+          hipLaunchKernelGGL(k_mass_elements<ET_HEX8>, dim3(grid_for(mp->n)),
+                             dim3(TPB), 0, m->stream, A);
+        HIPCHECK(hipGetLastError());
+      }
+      hipLaunchKernelGGL(k_gather_mass, dim3(grid_for(m->nn)), dim3(TPB), 0,
+                         m->stream, m->nn, (const int32_t *)m->rowptr.p,
+                         (const int32_t *)m->slots.p, (const double *)me.p,
+                         m->mass.p);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(m->stream));
+    }
+    for (auto &mp : m->mats)
+      alloc_internals(m, *mp);
+    if (m->precision == AKX_PRECISION_FP32) {
+      m->cu.alloc(3 * m->nn);
+      m->cv.alloc(3 * m->nn);
+      hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(3 * m->nn)), dim3(TPB), 0,
+                         m->stream, m->cu.p, 3 * m->nn, 0.f);
+      hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(3 * m->nn)), dim3(TPB), 0,
+                         m->stream, m->cv.p, 3 * m->nn, 0.f);
+      HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    m->initialized = true;
+  });
+}
+
+int akx_model_get_stable_time_step(akx_model *m, double *dt) {
+  return guard([&] {
+    check_ready(m);
+    require(dt != nullptr, "null output");
+    m->set_device();
+    unsigned long long init = 0x7FEFFFFFFFFFFFFFull; // DBL_MAX bits
+    HIPCHECK(hipMemcpyAsync(m->scratch_u64.p, &init, sizeof(init),
+                            hipMemcpyHostToDevice, m->stream));
+    for (auto &mp : m->mats) {
+      if (mp->n == 0)
+        continue;
+      InitArgs A = init_args(m, mp.get());
+      A.min_bits = m->scratch_u64.p;
+      if (m->etype == AKX_TET4)
+        hipLaunchKernelGGL(k_stable_dt<ET_TET4>, dim3(grid_for(mp->n)),
+                           dim3(TPB), 0, m->stream, A);
+      else
+        hipLaunchKernelGGL(k_stable_dt<ET_HEX8>, dim3(grid_for(mp->n)),
+                           dim3(TPB), 0, m->stream, A);
+      HIPCHECK(hipGetLastError());
+    }
+    unsigned long long bits = 0;
+    HIPCHECK(hipMemcpyAsync(&bits, m->scratch_u64.p, sizeof(bits),
+                            hipMemcpyDeviceToHost, m->stream));
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    double r;
+    std::memcpy(&r, &bits, sizeof(r));
+    *dt = r;
+  });
+}
+
+int akx_model_set_time_step(akx_model *m, double dt) {
+  return guard([&] {
+    require(m != nullptr, "null model");
+    require(dt > 0 && std::isfinite(dt), "time step must be positive");
+    m->dt = dt;
+  });
+}
+
+int akx_model_solve_step(akx_model *m, int64_t nb_steps) {
+  return guard([&] {
+    check_ready(m);
+    require(m->dt > 0, "setTimeStep has not been called");
+    require(nb_steps >= 0, "negative step count");
+    if (nb_steps == 0)
+      return;
+    m->set_device();
+    launch_predictor(m);
+    for (int64_t s = 0; s < nb_steps; ++s) {
+      launch_element_pass(m, false);
+      flip_plastic_state(m);
+      launch_gather(m, true, s + 1 < nb_steps);
+    }
+  });
+}
+
+int akx_model_assemble_internal_forces(akx_model *m) {
+  return guard([&] {
+    check_ready(m);
+    m->set_device();
+    // recompute from the previous plastic state so that calling it twice is
+    // idempotent (the reference recomputes from the saved previous state)
+    for (auto &mp : m->mats)
+      if (mp->law == AKX_LAW_PLASTIC_LIH)
+        mp->cur = 1 - mp->cur;
+    launch_element_pass(m, false);
+    flip_plastic_state(m);
+    launch_gather(m, false, false);
+    HIPCHECK(hipStreamSynchronize(m->stream));
+  });
+}
+
+int akx_model_get_nb_nodes(akx_model *m, int64_t *nb_nodes) {
+  return guard([&] {
+    require(m && nb_nodes, "null argument");
+    *nb_nodes = m->nn;
+  });
+}
+
+int akx_model_get_field(akx_model *m, int field, void *host) {
+  return guard([&] {
+    require(m != nullptr && m->etype >= 0, "set the mesh first");
+    require(host != nullptr, "null host array");
+    m->set_device();
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    double *src = nullptr;
+    switch (field) {
+    case AKX_FIELD_DISPLACEMENT: src = m->u.p; break;
+    case AKX_FIELD_VELOCITY: src = m->v.p; break;
+    case AKX_FIELD_ACCELERATION: src = m->a.p; break;
+    case AKX_FIELD_INTERNAL_FORCE: src = m->fint.p; break;
+    case AKX_FIELD_NODES: src = m->X.p; break;
+    case AKX_FIELD_EXTERNAL_FORCE:
+      if (!m->has_fext) {
+        std::memset(host, 0, sizeof(double) * 3 * m->nn);
+        return;
+      }
+      src = m->fext.p;
+      break;
+    case AKX_FIELD_MASS: {
+      size_t bytes = sizeof(double) * 3 * m->nn;
+      m->staging.alloc(std::max(m->staging.n, bytes));
+      hipLaunchKernelGGL(k_mass_to3, dim3(grid_for(m->nn)), dim3(TPB), 0,
+                         m->stream, (const double *)m->mass.p,
+                         (double *)m->staging.p, m->nn);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipMemcpyAsync(host, m->staging.p, bytes, hipMemcpyDeviceToHost,
+                              m->stream));
+      HIPCHECK(hipStreamSynchronize(m->stream));
+      return;
+    }
+    case AKX_FIELD_BLOCKED_DOFS: {
+      size_t bytes = 3 * m->nn;
+      m->staging.alloc(std::max(m->staging.n, bytes));
+      hipLaunchKernelGGL(k_bool3_from_bits, dim3(grid_for(m->nn)), dim3(TPB), 0,
+                         m->stream, (const uint8_t *)m->blocked.p,
+                         (uint8_t *)m->staging.p, m->nn);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipMemcpyAsync(host, m->staging.p, bytes, hipMemcpyDeviceToHost,
+                              m->stream));
+      HIPCHECK(hipStreamSynchronize(m->stream));
+      return;
+    }
+    default:
+      throw Error(AKX_ERR_INVALID, "unknown field");
+    }
+    download_soa(m, (const double *)src, (double *)host, m->nn, 3);
+  });
+}
+
+int akx_model_set_field(akx_model *m, int field, const void *host) {
+  return guard([&] {
+    require(m != nullptr && m->etype >= 0, "set the mesh first");
+    require(host != nullptr, "null host array");
+    m->set_device();
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    double *dst = nullptr;
+    switch (field) {
+    case AKX_FIELD_DISPLACEMENT: dst = m->u.p; break;
+    case AKX_FIELD_VELOCITY: dst = m->v.p; break;
+    case AKX_FIELD_ACCELERATION: dst = m->a.p; break;
+    case AKX_FIELD_EXTERNAL_FORCE: {
+      const double *h = (const double *)host;
+      bool any = false;
+      for (int64_t i = 0; i < 3 * m->nn && !any; ++i)
+        any = h[i] != 0.;
+      if (!any && !m->has_fext)
+        return;
+      m->fext.alloc(3 * m->nn);
+      m->has_fext = true;
+      dst = m->fext.p;
+      break;
+    }
+    case AKX_FIELD_BLOCKED_DOFS: {
+      size_t bytes = 3 * m->nn;
+      m->staging.alloc(std::max(m->staging.n, bytes));
+      HIPCHECK(hipMemcpyAsync(m->staging.p, host, bytes, hipMemcpyHostToDevice,
+                              m->stream));
+      hipLaunchKernelGGL(k_bits_from_bool3, dim3(grid_for(m->nn)), dim3(TPB), 0,
+                         m->stream, (const uint8_t *)m->staging.p, m->blocked.p,
+                         m->nn);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(m->stream));
+      return;
+    }
+    default:
+      throw Error(AKX_ERR_INVALID, "field is read only or unknown");
+    }
+    upload_soa(m, (const double *)host, dst, m->nn, 3);
+  });
+}
+
+int akx_model_compute_all_stresses(akx_model *m) {
+  return guard([&] {
+    check_ready(m);
+    m->set_device();
+    for (auto &mp : m->mats) {
+      int64_t nq = mp->n * m->nqp;
+      if (mp->law != AKX_LAW_PLASTIC_LIH) {
+        mp->grad.alloc(9 * nq);
+        mp->stress.alloc(9 * nq);
+      } else {
+        mp->cur = 1 - mp->cur; // recompute the current state from the previous
+      }
+    }
+    launch_element_pass(m, true);
+    flip_plastic_state(m);
+    HIPCHECK(hipStreamSynchronize(m->stream));
+  });
+}
+
+int akx_model_get_internal(akx_model *m, int material_id, int internal,
+                           double *host) {
+  return guard([&] {
+    check_ready(m);
+    require(host != nullptr, "null host array");
+    require(material_id >= 0 && material_id < (int)m->mats.size(), "bad material id");
+    m->set_device();
+    Material &mat = *m->mats[material_id];
+    int64_t nq = mat.n * m->nqp;
+    const double *src = nullptr;
+    int comps = 9;
+    bool plastic = mat.law == AKX_LAW_PLASTIC_LIH;
+    switch (internal) {
+    case AKX_INTERNAL_GRAD_U: src = plastic ? mat.hgrad[mat.cur].p : mat.grad.p; break;
+    case AKX_INTERNAL_STRESS: src = plastic ? mat.hsig[mat.cur].p : mat.stress.p; break;
+    case AKX_INTERNAL_INELASTIC_STRAIN:
+      require(plastic, "inelastic_strain exists only for plastic materials");
+      src = mat.heps[mat.cur].p;
+      break;
+    case AKX_INTERNAL_ISO_HARDENING:
+      require(plastic, "iso_hardening exists only for plastic materials");
+      src = mat.hiso[mat.cur].p;
+      comps = 1;
+      break;
+    default:
+      throw Error(AKX_ERR_INVALID, "unknown internal");
+    }
+    require(src != nullptr, "call computeAllStresses first");
+    if (nq == 0)
+      return;
+    download_soa(m, src, host, nq, comps);
+  });
+}
+
+int akx_model_synchronize(akx_model *m) {
+  return guard([&] {
+    require(m != nullptr, "null model");
+    m->set_device();
+    HIPCHECK(hipStreamSynchronize(m->stream));
+  });
+}
+
+int akx_model_enable_kernel_timing(akx_model *m, int enable) {
+  return guard([&] {
+    require(m != nullptr, "null model");
+    m->drain_timing();
+    m->timing = enable != 0;
+    for (int k = 0; k < 3; ++k) {
+      m->t_ms[k] = 0;
+      m->t_n[k] = 0;
+    }
+  });
+}
+
+int akx_model_get_kernel_timing(akx_model *m, int kernel_class, double *total_ms,
+                                int64_t *launches) {
+  return guard([&] {
+    require(m != nullptr, "null model");
+    require(kernel_class >= 0 && kernel_class < 3, "bad kernel class");
+    m->set_device();
+    m->drain_timing();
+    if (total_ms)
+      *total_ms = m->t_ms[kernel_class];
+    if (launches)
+      *launches = m->t_n[kernel_class];
+  });
+}
+
+int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
+  return guard([&] {
+    check_ready(m);
+    require(bytes != nullptr, "null output");
+    const double s_node = 8.0; // FP64 nodal state
+    const double s_f = (double)m->fe_elem_size();
+    const double s_c = m->precision == AKX_PRECISION_FP64 ? 8.0 : 4.0;
+    int64_t nel_all = 0;
+    double extra = 0.;
+    for (auto &mp : m->mats) {
+      nel_all += mp->n;
+      if (mp->per_element)
+        extra += mp->n * 2.0 * 8.0; // lambda, mu per element (stored FP64)
+      if (mp->law == AKX_LAW_PLASTIC_LIH)
+        extra += (double)mp->n * m->nqp * (2 * (9 + 9 + 9 + 1) * 8.0); // read + write history
+      if (!mp->contiguous)
+        extra += mp->n * 4.0;
+    }
+    (void)s_c;
+    const double deg = (double)nel_all * m->nnpe / (double)m->nn;
+    (void)deg;
+    switch (kernel_class) {
+    case 0: // element pass: connectivity, X and u once per node, r_e write
+      *bytes = nel_all * (4.0 * m->nnpe + 3.0 * s_f * m->nnpe) +
+               m->nn * (3.0 * s_node + 3.0 * s_node) + extra;
+      break;
+    case 1: { // gather + update: rowptr, slots, r_e read, f_int write,
+              // mass, blocked, a/v read+write, u read+write (fused predictor)
+      double per_node = 4.0 + 3 * s_node /*f_int*/ + s_node /*M*/ + 1.0 +
+                        2 * 3 * s_node /*a*/ + 2 * 3 * s_node /*v*/ +
+                        2 * 3 * s_node /*u*/ + (m->has_fext ? 3 * s_node : 0.);
+      if (m->precision == AKX_PRECISION_FP32)
+        per_node += 2 * 2 * 3 * 4.0; // compensation terms read + write
+      *bytes = nel_all * m->nnpe * (4.0 + 3.0 * s_f) + m->nn * per_node;
+      break;
+    }
+    case 2:
+      *bytes = m->nn * (1.0 + 3 * s_node + 2 * 2 * 3 * s_node);
+      break;
+    default:
+      throw Error(AKX_ERR_INVALID, "bad kernel class");
+    }
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,258 @@
+"""Python front end of the MI355X explicit path, shaped like the reference's
+Python API (python/py_solid_mechanics_model.cc:57-173, py_model.cc:131-139,
+py_material.cc:45-80): ``SolidMechanicsModel(mesh)``, ``initFull``,
+``getStableTimeStep``, ``setTimeStep``, ``solveStep``, ``getDisplacement`` ...
+
+Nodal arrays are numpy arrays in the reference's [nb_nodes][3] layout.  The
+arrays returned by the getters are host mirrors of device-resident fields:
+edits made to them are uploaded before the next ``solveStep`` (the reference
+lets users write through ``getDisplacement()``), and after a step the next
+getter call refreshes the mirror in place.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .mesh import BoxMesh
+
+_explicit_lumped_mass = "explicit_lumped_mass"
+
+_FIELDS = {
+    "displacement": N.FIELD_DISPLACEMENT,
+    "velocity": N.FIELD_VELOCITY,
+    "acceleration": N.FIELD_ACCELERATION,
+    "internal_force": N.FIELD_INTERNAL_FORCE,
+    "external_force": N.FIELD_EXTERNAL_FORCE,
+    "mass": N.FIELD_MASS,
+    "blocked_dofs": N.FIELD_BLOCKED_DOFS,
+}
+_WRITABLE = {"displacement", "velocity", "acceleration", "external_force",
+             "blocked_dofs"}
+_LAWS = {"elastic": N.LAW_ELASTIC, "elastic_fd": N.LAW_ELASTIC_FD,
+         "neohookean": N.LAW_NEOHOOKEAN,
+         "plastic_linear_isotropic_hardening": N.LAW_PLASTIC_LIH}
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Material:
+    """Handle on one material of the model (Material's accessors)."""
+
+    def __init__(self, model, mid, law, params, nb_elements):
+        self._model, self.id, self.law = model, mid, law
+        self.params = dict(params)
+        self.nb_elements = nb_elements
+
+    def getParam(self, name):
+        return self.params[name]
+
+    get = getParam
+
+    def _internal(self, which, comps=9):
+        m = self._model
+        m._push()
+        if self._needs_compute():
+            N.check(N.lib().akx_model_compute_all_stresses(m._h))
+            m._stresses_fresh = True
+        out = np.empty((self.nb_elements * m.nqp, comps))
+        N.check(N.lib().akx_model_get_internal(m._h, self.id, which, _p(out)))
+        return out
+
+    def _needs_compute(self):
+        m = self._model
+        if self.law == N.LAW_PLASTIC_LIH:
+            return False  # the history buffers hold the current state
+        return not m._stresses_fresh
+
+    def getGradU(self):
+        return self._internal(N.INTERNAL_GRAD_U)
+
+    def getStress(self):
+        """sigma for small strain laws, the second Piola-Kirchhoff stress for
+        finite deformation (as Material::getStress / piola_kirchhoff_2)."""
+        return self._internal(N.INTERNAL_STRESS)
+
+    def getInelasticStrain(self):
+        return self._internal(N.INTERNAL_INELASTIC_STRAIN)
+
+    def getIsoHardening(self):
+        return self._internal(N.INTERNAL_ISO_HARDENING, 1)[:, 0]
+
+
+class SolidMechanicsModel:
+    """One mesh of one element type on one gfx950 device."""
+
+    def __init__(self, mesh: BoxMesh, device: int = 0):
+        if N.device_count() == 0:
+            raise RuntimeError("no gfx950 device visible: the MI355X path has no "
+                               "CPU fallback")
+        h = C.c_void_p()
+        N.check(N.lib().akx_model_create(device, C.byref(h)))
+        self._h = h
+        self.mesh = mesh
+        self.nqp = mesh.nqp
+        nodes = np.ascontiguousarray(mesh.nodes, np.float64)
+        conn = np.ascontiguousarray(mesh.conn, np.int32)
+        N.check(N.lib().akx_model_set_mesh(h, mesh.etype, nodes.shape[0], _p(nodes),
+                                           conn.shape[0], _p(conn)))
+        self.nb_nodes = nodes.shape[0]
+        self.materials = []
+        self._mirrors = {}
+        self._snap = {}
+        self._stale = set()
+        self._initialized = False
+        self._stresses_fresh = False
+        self.time_step = 0.0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                N.lib().akx_model_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # ------------------------------------------------------------ materials
+    def addMaterial(self, law="elastic", rho=1.0, E=1.0, nu=0.3, sigma_y=0.0, h=0.0,
+                    finite_deformation=False, elements=None, E_el=None, nu_el=None,
+                    sigma_y_el=None, h_el=None):
+        """Register a material (the ``material <law> [...]`` section of the
+        reference's input file).  ``elements`` is the element filter in
+        assembly order (default: all elements)."""
+        if law == "elastic" and finite_deformation:
+            law = "elastic_fd"
+        lid = _LAWS[law] if isinstance(law, str) else int(law)
+        params = np.array([rho, E, nu, sigma_y, h], np.float64)
+        mid = C.c_int(-1)
+        filt = None if elements is None else np.ascontiguousarray(elements, np.int32)
+        n = self.mesh.nb_elements if filt is None else filt.size
+        N.check(N.lib().akx_model_add_material(
+            self._h, lid, params.ctypes.data_as(C.POINTER(C.c_double)), 5,
+            _p(filt), 0 if filt is None else filt.size, C.byref(mid)))
+        if any(x is not None for x in (E_el, nu_el, sigma_y_el, h_el)):
+            cv = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
+            arrs = [cv(E_el), cv(nu_el), cv(sigma_y_el), cv(h_el)]
+            for a in arrs:
+                if a is not None and a.size != n:
+                    raise ValueError("per-element parameters must match the filter")
+            N.check(N.lib().akx_model_set_element_parameters(
+                self._h, mid.value, *[_p(a) for a in arrs]))
+        mat = Material(self, mid.value, lid,
+                       dict(rho=rho, E=E, nu=nu, sigma_y=sigma_y, h=h), n)
+        self.materials.append(mat)
+        return mat
+
+    def getMaterial(self, i):
+        return self.materials[i]
+
+    def setPrecision(self, precision):
+        p = {"fp64": N.PRECISION_FP64, "fp32": N.PRECISION_FP32}[precision]
+        N.check(N.lib().akx_model_set_precision(self._h, p))
+
+    # ---------------------------------------------------------------- init
+    def initFull(self, _analysis_method=_explicit_lumped_mass, **material):
+        if _analysis_method != _explicit_lumped_mass:
+            raise NotImplementedError("only _explicit_lumped_mass is provided")
+        if not self.materials:
+            self.addMaterial(**(material or {}))
+        N.check(N.lib().akx_model_init_full(self._h))
+        self._initialized = True
+
+    def getStableTimeStep(self) -> float:
+        self._push()
+        dt = C.c_double()
+        N.check(N.lib().akx_model_get_stable_time_step(self._h, C.byref(dt)))
+        return dt.value
+
+    def setTimeStep(self, dt: float):
+        N.check(N.lib().akx_model_set_time_step(self._h, float(dt)))
+        self.time_step = float(dt)
+
+    def getTimeStep(self) -> float:
+        return self.time_step
+
+    # ---------------------------------------------------------------- steps
+    def solveStep(self, nb_steps: int = 1):
+        """``nb_steps`` consecutive ModelSolver::solveStep() calls."""
+        self._push()
+        N.check(N.lib().akx_model_solve_step(self._h, int(nb_steps)))
+        self._invalidate()
+
+    def assembleInternalForces(self):
+        self._push()
+        N.check(N.lib().akx_model_assemble_internal_forces(self._h))
+        self._invalidate()
+
+    def synchronize(self):
+        N.check(N.lib().akx_model_synchronize(self._h))
+
+    # --------------------------------------------------------------- fields
+    def _invalidate(self):
+        self._stale = set(self._mirrors)
+        self._stresses_fresh = False
+
+    def _push(self):
+        for name, arr in self._mirrors.items():
+            if name in _WRITABLE and not np.array_equal(arr, self._snap[name]):
+                a = np.ascontiguousarray(arr)
+                N.check(N.lib().akx_model_set_field(self._h, _FIELDS[name], _p(a)))
+                self._snap[name] = arr.copy()
+                self._stresses_fresh = False
+
+    def _get(self, name):
+        self._push()
+        dtype = np.uint8 if name == "blocked_dofs" else np.float64
+        arr = self._mirrors.get(name)
+        if arr is None or name in self._stale:
+            tmp = np.empty((self.nb_nodes, 3), dtype)
+            N.check(N.lib().akx_model_get_field(self._h, _FIELDS[name], _p(tmp)))
+            if arr is None:
+                arr = tmp
+                self._mirrors[name] = arr
+            else:
+                arr[...] = tmp
+            self._snap[name] = arr.copy()
+            self._stale.discard(name)
+        return arr
+
+    def getDisplacement(self):
+        return self._get("displacement")
+
+    def getVelocity(self):
+        return self._get("velocity")
+
+    def getAcceleration(self):
+        return self._get("acceleration")
+
+    def getInternalForce(self):
+        return self._get("internal_force")
+
+    def getExternalForce(self):
+        return self._get("external_force")
+
+    def getMass(self):
+        return self._get("mass")
+
+    def getBlockedDOFs(self):
+        return self._get("blocked_dofs")
+
+    # ----------------------------------------------------------- measurement
+    def enableKernelTiming(self, enable=True):
+        N.check(N.lib().akx_model_enable_kernel_timing(self._h, int(enable)))
+
+    def kernelTiming(self, kernel_class):
+        ms, n = C.c_double(), C.c_int64()
+        N.check(N.lib().akx_model_get_kernel_timing(self._h, kernel_class,
+                                                    C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def kernelBytes(self, kernel_class):
+        b = C.c_double()
+        N.check(N.lib().akx_model_get_kernel_bytes(self._h, kernel_class, C.byref(b)))
+        return b.value

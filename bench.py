@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit internal-force step on one MI355X.
+
+Workload (the single-GPU configuration the headline metric is quoted on):
+200 x 100 x 100 structured _hexahedron_8 grid (2,000,000 elements, 8 Gauss
+points each = 16,000,000 element quadrature points), interior nodes jittered
+by U(-0.1, 0.1) h (seed 1), compressible neo-Hookean E = 1, nu = 0.3,
+rho = 1, x = 0 face clamped, initial velocity 1e-3 sin(pi x / L) U(-1, 1)^3
+(seed 1), dt = 0.5 dt_crit, plain FP64 path.
+
+One step = one SolidMechanicsModel::solveStep (predictor, internal force:
+gradient -> neo-Hookean PK2 -> (B_L + B_NL)^T S -> integrate -> gather
+assembly, residual, lumped solve, corrector).  Inputs are resident in HBM
+before the timed region.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-baseline/--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
+SHAPE = (200, 100, 100)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_hex8_neohookean_f64.json")
+
+
+def build_c2(shape=SHAPE):
+    from akantu2_amd import mesh as M
+    mesh = M.box_mesh("_hexahedron_8", *shape, jitter=0.1, seed=1)
+    blocked = M.clamped_face_x0(mesh)
+    v0 = M.initial_velocity(mesh, 1e-3, seed=1)
+    return mesh, blocked, v0
+
+
+def cpu_baseline(shape=(20, 100, 100), steps=10):
+    """The CPU oracle (plain C, one thread, FP64, reference dataflow with
+    precomputed shape derivatives) timed on a bounded sample of the same
+    workload: the first 20 cells in x of the 200 x 100 x 100 grid."""
+    import oracle
+    mesh, blocked, v0 = build_c2(shape)
+    o = oracle.OracleModel(oracle.HEX8, mesh.nodes, mesh.conn, "neohookean")
+    o.init_full()
+    o.blocked[:] = blocked
+    o.v[:] = v0
+    o.set_time_step(0.5 * o.stable_dt())
+    t0 = time.perf_counter()
+    o.solve_steps(steps)
+    el = time.perf_counter() - t0
+    value = mesh.nb_elements * 8 * steps / el
+    return {"value": value, "unit": "elem*qp/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps of a {shape[0]}x{shape[1]}x{shape[2]} Hex8 "
+                      f"slab ({mesh.nb_elements} elements) of the same jittered "
+                      "neo-Hookean workload, oracle/akantu_oracle.c, 1 thread",
+            "seconds": el}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--shape", type=str, default=None,
+                    help="override cells per direction, e.g. 20,10,10")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is designed, not built)"}))
+        sys.exit(2)
+
+    from akantu2_amd import SolidMechanicsModel
+
+    shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else SHAPE
+    t = time.perf_counter()
+    mesh, blocked, v0 = build_c2(shape)
+    model = SolidMechanicsModel(mesh)
+    model.addMaterial("neohookean", rho=1.0, E=1.0, nu=0.3)
+    model.initFull()
+    model.getBlockedDOFs()[:] = blocked
+    model.getVelocity()[:] = v0
+    dt = 0.5 * model.getStableTimeStep()
+    model.setTimeStep(dt)
+    setup_s = time.perf_counter() - t
+    print(f"# setup {setup_s:.1f} s, {mesh.nb_elements} elements, "
+          f"{mesh.nb_nodes} nodes, dt {dt:.6g}", file=sys.stderr, flush=True)
+
+    model.solveStep(args.warmup)
+    model.synchronize()
+    model.enableKernelTiming(True)
+    model.synchronize()
+    t0 = time.perf_counter()
+    model.solveStep(args.steps)
+    model.synchronize()
+    elapsed = time.perf_counter() - t0
+    nqp = mesh.nb_elements * 8
+    value = nqp * args.steps / elapsed
+    ms_step = 1e3 * elapsed / args.steps
+
+    kern = {}
+    names = {0: "element_pass", 1: "gather_update", 2: "predictor"}
+    for k, name in names.items():
+        ms, n = model.kernelTiming(k)
+        if n:
+            kern[name] = {"avg_ms": ms / n, "launches": n, "bytes": model.kernelBytes(k)}
+    el = kern["element_pass"]
+    achieved = el["bytes"] / (el["avg_ms"] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            tr = json.load(fh)
+        if tr.get("shape") == list(shape):
+            traffic = tr.get("element_pass_bytes")
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "kernel": "k_element_forces<Hex8, neo-Hookean, f64>",
+                "algorithmic_bytes_per_launch": el["bytes"],
+                "avg_kernel_ms": el["avg_ms"]}
+    out = {
+        "metric": "elem*qp stress+force updates/s",
+        "value": value,
+        "unit": "elem*qp/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Hex8 {shape[0]}x{shape[1]}x{shape[2]} jittered grid, "
+                               "compressible neo-Hookean, explicit central difference "
+                               "with lumped mass, plain FP64",
+                   "elements": mesh.nb_elements, "nodes": mesh.nb_nodes,
+                   "quadrature_points": nqp, "dt": dt},
+        "roofline": roofline,
+        "kernels": kern,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

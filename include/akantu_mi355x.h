@@ -1,0 +1,163 @@
+/*
+ * akantu_mi355x.h -- C ABI of the MI355X (gfx950) explicit-dynamics
+ * internal-force path.
+ *
+ * One `akx_model` is one SolidMechanicsModel: a mesh of one element type,
+ * one or more materials (each with an element filter), device-resident
+ * nodal state, one HIP stream.  Every function returns AKX_OK (0) or a
+ * negative status; akx_last_error() gives the message of the last failure on
+ * the calling thread.  Nothing throws across this boundary.
+ *
+ * Host arrays passed in or out use the reference's Array<T> layout
+ * (array-of-structures, one tuple per row: nodal fields [nb_nodes][3],
+ * connectivity [nb_elements][nnpe], quadrature-point tensors
+ * [nb_elements*nqp][9] column-major).  The library converts to its own
+ * structure-of-arrays device layout.  The caller owns host memory; the
+ * library owns device memory.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   SolidMechanicsModel(mesh) + initFull(_analysis_method =
+ *     _explicit_lumped_mass)      src/model/solid_mechanics/solid_mechanics_model.cc:52-78,186-232
+ *   ModelSolver::solveStep       src/model/common/model_solver.cc:267-280
+ *   getStableTimeStep            solid_mechanics_model.cc:423-473
+ *   setTimeStep                  src/model/common/model_solver.cc (setTimeStep)
+ *   getDisplacement/getVelocity/getAcceleration/getInternalForce/
+ *   getExternalForce/getBlockedDOFs/getMass
+ *                                solid_mechanics_model.hh:440-468 (the nodal Arrays)
+ *   Material::computeAllStresses + Material::computeStress (plugin)
+ *                                src/model/solid_mechanics/material.cc:128-154, material.hh:78-81
+ *   MaterialFactory / instantiateMaterial<Mat>("name")
+ *                                material.hh:41-42,610-621
+ *   SolidMechanicsModel::assembleInternalForces
+ *                                solid_mechanics_model.cc:333-370
+ */
+#ifndef AKANTU_MI355X_H
+#define AKANTU_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AKX_OK 0
+#define AKX_ERR_INVALID (-1)    /* bad argument or call order */
+#define AKX_ERR_HIP (-2)        /* HIP runtime failure */
+#define AKX_ERR_JACOBIAN (-3)   /* negative jacobian (IntegratorGaussException) */
+#define AKX_ERR_MEMORY (-4)     /* device allocation failed */
+#define AKX_ERR_UNSUPPORTED (-5)
+
+/* ElementType subset (src/common/aka_element_classes_info.hh) */
+#define AKX_TET4 0 /* _tetrahedron_4 */
+#define AKX_HEX8 1 /* _hexahedron_8 */
+
+/* Constitutive laws registered in the material factory under the
+ * reference's names: "elastic" (material_elastic.cc:216-217), "neohookean"
+ * (material_neohookean.cc:161-162), "plastic_linear_isotropic_hardening". */
+#define AKX_LAW_ELASTIC 0     /* finite_deformation = false */
+#define AKX_LAW_ELASTIC_FD 1  /* "elastic" with finite_deformation = true */
+#define AKX_LAW_NEOHOOKEAN 2
+#define AKX_LAW_PLASTIC_LIH 3
+
+/* Arithmetic of the element pass */
+#define AKX_PRECISION_FP64 0 /* plain path: everything in FP64 */
+#define AKX_PRECISION_FP32 1 /* grad u, stress, B^T sigma in FP32; nodal
+                                update in FP64 with compensated summation */
+
+/* Nodal fields, [nb_nodes][3] doubles unless stated */
+#define AKX_FIELD_DISPLACEMENT 0
+#define AKX_FIELD_VELOCITY 1
+#define AKX_FIELD_ACCELERATION 2
+#define AKX_FIELD_INTERNAL_FORCE 3
+#define AKX_FIELD_EXTERNAL_FORCE 4
+#define AKX_FIELD_MASS 5          /* [nb_nodes][3], read only */
+#define AKX_FIELD_BLOCKED_DOFS 6  /* [nb_nodes][3] uint8 (bool) */
+#define AKX_FIELD_NODES 7         /* [nb_nodes][3], read only */
+
+/* Quadrature-point internals, [nb_filter_elements * nqp][9] column-major */
+#define AKX_INTERNAL_GRAD_U 0
+#define AKX_INTERNAL_STRESS 1          /* sigma (small strain) or PK2 */
+#define AKX_INTERNAL_INELASTIC_STRAIN 2 /* plastic only */
+#define AKX_INTERNAL_ISO_HARDENING 3    /* plastic only, [n*nqp][1] */
+
+typedef struct akx_model akx_model;
+
+/* Library ---------------------------------------------------------------- */
+const char *akx_last_error(void);
+int akx_version(void);
+/* Number of visible gfx950 devices (0 on a machine without GPU). */
+int akx_device_count(int *count);
+
+/* Model life cycle ---------------------------------------------------------
+ * Replaces `SolidMechanicsModel model(mesh)`: the mesh is given as arrays. */
+int akx_model_create(int device, akx_model **out);
+int akx_model_destroy(akx_model *m);
+int akx_model_set_mesh(akx_model *m, int element_type, int64_t nb_nodes,
+                       const double *nodes, int64_t nb_elements,
+                       const int32_t *connectivity);
+
+/* Material -----------------------------------------------------------------
+ * Replaces the parsed `material <name> [ rho = .. E = .. nu = .. ]` section
+ * and MaterialFactory::allocate.  params = {rho, E, nu, sigma_y, h} (the last
+ * two only for the plastic law).  `filter` lists the elements of the
+ * material in assembly order (NULL: all elements in ascending order).
+ * Materials are assembled in creation order, like for_each_constitutive_law
+ * (constitutive_laws_handler.hh:116-126). */
+int akx_model_add_material(akx_model *m, int law, const double *params,
+                           int nparams, const int32_t *filter,
+                           int64_t nb_filter, int *material_id);
+/* Per-element E, nu (and sigma_y, h for plasticity) over the material's
+ * filter, for the per-element-parameter material.  Any pointer may be NULL
+ * (keeps the scalar). */
+int akx_model_set_element_parameters(akx_model *m, int material_id,
+                                     const double *E, const double *nu,
+                                     const double *sigma_y, const double *h);
+/* Look a law up by the reference's material name ("elastic", "neohookean",
+ * "plastic_linear_isotropic_hardening"); finite_deformation selects the total
+ * Lagrangian branch of "elastic". */
+int akx_material_law_by_name(const char *name, int finite_deformation,
+                             int *law);
+
+int akx_model_set_precision(akx_model *m, int precision);
+
+/* initFull(_analysis_method = _explicit_lumped_mass): checks the jacobians
+ * (AKX_ERR_JACOBIAN, like IntegratorGauss::checkJacobians), builds the
+ * node->element gather table and assembles the lumped mass. */
+int akx_model_init_full(akx_model *m);
+
+/* Time stepping -------------------------------------------------------------*/
+int akx_model_get_stable_time_step(akx_model *m, double *dt);
+int akx_model_set_time_step(akx_model *m, double dt);
+/* nb_steps consecutive solveStep() calls. */
+int akx_model_solve_step(akx_model *m, int64_t nb_steps);
+/* SolidMechanicsModel::assembleInternalForces on the current displacement
+ * (f_int only, no time integration). */
+int akx_model_assemble_internal_forces(akx_model *m);
+
+/* Fields --------------------------------------------------------------------*/
+int akx_model_get_nb_nodes(akx_model *m, int64_t *nb_nodes);
+int akx_model_get_field(akx_model *m, int field, void *host);
+int akx_model_set_field(akx_model *m, int field, const void *host);
+/* Material::computeAllStresses on the current displacement, then copy one
+ * internal of one material to the host. */
+int akx_model_compute_all_stresses(akx_model *m);
+int akx_model_get_internal(akx_model *m, int material_id, int internal,
+                           double *host);
+
+/* Synchronise the model's stream (all queued steps finished). */
+int akx_model_synchronize(akx_model *m);
+
+/* Kernel timing -------------------------------------------------------------
+ * When enabled, every launch of the element pass and of the nodal gather is
+ * bracketed by HIP events on the stream it runs on.  Kernel classes:
+ * 0 = element pass, 1 = nodal gather + update, 2 = predictor. */
+int akx_model_enable_kernel_timing(akx_model *m, int enable);
+int akx_model_get_kernel_timing(akx_model *m, int kernel_class,
+                                double *total_ms, int64_t *launches);
+/* Bytes each kernel class moves per launch in the unique-bytes model. */
+int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
