@@ -19,7 +19,7 @@ ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "negative jacobia
 
 TET4, HEX8 = 0, 1
 LAW_ELASTIC, LAW_ELASTIC_FD, LAW_NEOHOOKEAN, LAW_PLASTIC_LIH = 0, 1, 2, 3
-PRECISION_FP64, PRECISION_FP32 = 0, 1
+PRECISION_FP64, PRECISION_FP32, PRECISION_FP64_FAST = 0, 1, 2
 FIELD_DISPLACEMENT, FIELD_VELOCITY, FIELD_ACCELERATION = 0, 1, 2
 FIELD_INTERNAL_FORCE, FIELD_EXTERNAL_FORCE, FIELD_MASS = 3, 4, 5
 FIELD_BLOCKED_DOFS, FIELD_NODES = 6, 7

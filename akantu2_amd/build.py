@@ -18,22 +18,36 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libakantu_mi355x.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["akx_model.hip"]
-FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
-         "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
-         "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
-         "-Wno-unused-variable"]
+# source -> floating-point contraction of its translation unit
+SOURCES = {"akx_model.hip": "off", "akx_fast.hip": "fast"}
+FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
+OBJDIR = os.path.join(HERE, "_build")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r.stderr
 
 
 def build_library(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJDIR, exist_ok=True)
+    jobs = []
+    for src, contract in SOURCES.items():
+        obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
+        jobs.append((obj, [HIPCC, *FLAGS, f"-ffp-contract={contract}", "-c",
+                           os.path.join(CSRC, src), "-o", obj]))
+    with ThreadPoolExecutor(len(jobs)) as ex:
+        logs = list(ex.map(lambda j: _run(j[1]), jobs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    logs.append(_run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp,
+                      *[j[0] for j in jobs]]))
     if verbose:
-        print(r.stderr)
+        print("".join(logs))
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,361 @@
+// akx_fast.hip -- regrouped element pass of the FP32 precision form and of
+// the opt-in FP64 "fast" form.  Compiled with -ffp-contract=fast (FMA), in
+// its own translation unit and math namespace.
+//
+// Same mechanics as the reference (material.cc:98-154 + 457-584): gather u,
+// grad u at every Gauss point, constitutive law, B^T sigma (small strain) or
+// (B_L + B_NL)^T S = (F S) dN/dX (total Lagrangian), times w detJ, summed per
+// element.  The arithmetic is regrouped for gfx950:
+//   J_q = dN/dxi_q X,  H_q = U dN/dxi_q^T              (natural gradients)
+//   grad u_q = H_q J_q^{-1 T}
+//   Q_q = w_q detJ_q P_q J_q^{-1}      (P = sigma, or F S for finite strain)
+//   r_e(n) = sum_q Q_q dN_n/dxi_q
+// which needs no dN/dX array and exchanges 9 numbers per Gauss point instead
+// of 24.  In FP64 the results agree with the reference operation order to
+// rounding per evaluation; the FP32 form uses a cancellation-free neo-Hookean
+// stress.
+//
+// Hex8: 8 lanes per element (one per Gauss point; in the last phase one per
+// node), 32 elements per 256-thread block, nodal data and Q in LDS.
+// Tet4: one thread per element (single Gauss point).
+#define AKX_MATH_NS akxf
+#include "akx_args.hpp"
+#include "akx_math.hpp"
+#include "akx_fast.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace akxf {
+
+using akx::ElemArgs;
+
+constexpr int LAW_ELASTIC = 0, LAW_ELASTIC_FD = 1, LAW_NEOHOOKEAN = 2,
+              LAW_PLASTIC = 3;
+
+// dN_n/dxi_m at the 8 Hex8 Gauss points, [q][m][n]
+__constant__ double c_dnds[8][3][8];
+
+__device__ __forceinline__ int64_t xcd_block(int64_t nblocks_padded) {
+  int64_t b = blockIdx.x;
+  int64_t per = nblocks_padded >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+// inverse and determinant by cofactors
+template <class R>
+__device__ __forceinline__ R inv_det(const R (&m)[3][3], R (&r)[3][3]) {
+  R c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+  R c10 = m[2][1] * m[0][2] - m[2][2] * m[0][1];
+  R c20 = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+  R det = c00 * m[0][0] + c10 * m[1][0] + c20 * m[2][0];
+  R id = R(1) / det;
+  r[0][0] = c00 * id;
+  r[0][1] = c10 * id;
+  r[0][2] = c20 * id;
+  r[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * id;
+  r[1][1] = (m[2][2] * m[0][0] - m[2][0] * m[0][2]) * id;
+  r[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+  r[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * id;
+  r[2][1] = (m[2][0] * m[0][1] - m[2][1] * m[0][0]) * id;
+  r[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+  return det;
+}
+
+// Constitutive law at one Gauss point: g -> stress S (sigma or PK2) and the
+// tensor P contracted with dN/dX (sigma, or F S for finite deformation).
+template <int LAW, class CT>
+__device__ __forceinline__ void law_at(const ElemArgs &A, int64_t iq,
+                                       int64_t nq_all, CT lambda, CT mu, CT sy,
+                                       CT hh, const CT (&g)[3][3], CT (&S)[3][3],
+                                       CT (&P)[3][3]) {
+  if constexpr (LAW == LAW_ELASTIC) {
+    elastic_stress(lambda, mu, g, CT(0), S);
+  } else if constexpr (LAW == LAW_ELASTIC_FD) {
+    CT E[3][3];
+    green_strain(g, E);
+    elastic_stress(lambda, mu, E, CT(0), S);
+  } else if constexpr (LAW == LAW_NEOHOOKEAN) {
+    if constexpr (sizeof(CT) == 4)
+      neohookean_pk2_stable(lambda, mu, g, S);
+    else
+      neohookean_pk2(lambda, mu, g, S);
+  } else {
+    CT pg[3][3], ps[3][3], pe[3][3], eps[3][3], iso;
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+        const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
+        pg[ii][jj] = CT(A.p_grad[o]);
+        ps[ii][jj] = CT(A.p_sig[o]);
+        pe[ii][jj] = CT(A.p_eps[o]);
+      }
+    plastic_lih(lambda, mu, sy, hh, g, pg, ps, CT(A.p_iso[iq]), pe, S, iso, eps);
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+        const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
+        A.c_grad[o] = double(g[ii][jj]);
+        A.c_sig[o] = double(S[ii][jj]);
+        A.c_eps[o] = double(eps[ii][jj]);
+      }
+    A.c_iso[iq] = double(iso);
+  }
+  if (A.grad_u) {
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+        const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
+        A.grad_u[o] = double(g[ii][jj]);
+        A.stress[o] = double(S[ii][jj]);
+      }
+  }
+  if constexpr (LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int J = 0; J < 3; ++J) {
+        CT s = CT(0);
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          s += ((k == m ? CT(1) : CT(0)) + g[k][m]) * S[m][J];
+        P[k][J] = s;
+      }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int J = 0; J < 3; ++J)
+        P[k][J] = S[k][J];
+  }
+}
+
+template <class CT>
+__device__ __forceinline__ void element_params(const ElemArgs &A, int64_t i,
+                                               CT &lambda, CT &mu, CT &sy,
+                                               CT &hh) {
+  lambda = CT(A.lam_el ? A.lam_el[i] : A.lambda);
+  mu = CT(A.mu_el ? A.mu_el[i] : A.mu);
+  sy = CT(A.sy_el ? A.sy_el[i] : A.sigma_y);
+  hh = CT(A.h_el ? A.h_el[i] : A.h);
+}
+
+// ---------------------------------------------------------------------------
+// Hex8: 8 lanes per element
+// ---------------------------------------------------------------------------
+constexpr int HEX_EPB = 32; // elements per 256-thread block
+
+template <int LAW, class CT, class FT>
+__global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
+  __shared__ double sN[HEX_EPB][8][6]; // X (0..2) and u (3..5) of the 8 nodes
+  __shared__ CT sQ[HEX_EPB][8][9];     // Q of the 8 Gauss points
+  __shared__ CT sD[8][3][8];           // dN_n/dxi_m at Gauss point q
+  const int t = threadIdx.x, el = t >> 3, q = t & 7;
+  for (int k = t; k < 192; k += 256)
+    (&sD[0][0][0])[k] = CT((&c_dnds[0][0][0])[k]);
+  const int64_t i = xcd_block(A.nblocks_padded) * HEX_EPB + el;
+  const bool active = i < A.n;
+  if (active) {
+    const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+    const int32_t node = A.conn[e * 8 + q];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      sN[el][q][c] = A.X[(int64_t)c * A.nn + node];
+      sN[el][q][3 + c] = A.u[(int64_t)c * A.nn + node];
+    }
+  }
+  __syncthreads();
+  if (active) {
+    CT J[3][3], H[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        J[a][b] = CT(0);
+        H[a][b] = CT(0);
+      }
+    double x0[3], u0[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x0[c] = sN[el][0][c];
+      u0[c] = sN[el][0][3 + c];
+    }
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      CT xn[3], un[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if constexpr (sizeof(CT) == 8) {
+          xn[c] = sN[el][n][c];
+          un[c] = sN[el][n][3 + c];
+        } else { // relative to node 0, exact FP64 difference rounded once
+          xn[c] = CT(sN[el][n][c] - x0[c]);
+          un[c] = CT(sN[el][n][3 + c] - u0[c]);
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const CT d = sD[q][m][n];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          J[m][c] += d * xn[c];
+          H[c][m] += un[c] * d;
+        }
+      }
+    }
+    CT Ji[3][3];
+    const CT wJ = inv_det(J, Ji); // Hex8 Gauss weights are 1
+    CT g[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        CT s = CT(0);
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          s += H[a][m] * Ji[b][m];
+        g[a][b] = s;
+      }
+    CT lambda, mu, sy, hh;
+    element_params(A, i, lambda, mu, sy, hh);
+    CT S[3][3], P[3][3];
+    law_at<LAW, CT>(A, i * 8 + q, A.n * 8, lambda, mu, sy, hh, g, S, P);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        CT s = CT(0);
+#pragma unroll
+        for (int J2 = 0; J2 < 3; ++J2)
+          s += P[k][J2] * Ji[J2][m];
+        sQ[el][q][3 * k + m] = wJ * s;
+      }
+  }
+  __syncthreads();
+  if (active) {
+    const int n = q; // this lane now owns node n of the element
+    CT r[3] = {CT(0), CT(0), CT(0)};
+#pragma unroll
+    for (int qq = 0; qq < 8; ++qq)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const CT d = sD[qq][m][n];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          r[k] += sQ[el][qq][3 * k + m] * d;
+      }
+    FT *out = reinterpret_cast<FT *>(A.fe) + (A.pos_offset + i) * 24 + n * 3;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      out[k] = FT(r[k]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tet4: one thread per element
+// ---------------------------------------------------------------------------
+template <int LAW, class CT, class FT>
+__global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
+  const int64_t i = xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
+  if (i >= A.n)
+    return;
+  const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+  const int4 nd = *reinterpret_cast<const int4 *>(A.conn + e * 4);
+  const int32_t nodes[4] = {nd.x, nd.y, nd.z, nd.w};
+  double X[4][3], U[4][3];
+#pragma unroll
+  for (int n = 0; n < 4; ++n)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      X[n][c] = A.X[(int64_t)c * A.nn + nodes[n]];
+      U[n][c] = A.u[(int64_t)c * A.nn + nodes[n]];
+    }
+  // natural directions: xi -> node 3, eta -> node 1, zeta -> node 2
+  const int nm[3] = {3, 1, 2};
+  CT J[3][3], H[3][3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      J[m][c] = CT(X[nm[m]][c] - X[0][c]);
+      H[c][m] = CT(U[nm[m]][c] - U[0][c]);
+    }
+  CT Ji[3][3];
+  const CT detJ = inv_det(J, Ji);
+  const CT wJ = detJ * CT(1. / 6.);
+  CT g[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      CT s = CT(0);
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        s += H[a][m] * Ji[b][m];
+      g[a][b] = s;
+    }
+  CT lambda, mu, sy, hh;
+  element_params(A, i, lambda, mu, sy, hh);
+  CT S[3][3], P[3][3];
+  law_at<LAW, CT>(A, i, A.n, lambda, mu, sy, hh, g, S, P);
+  CT Q[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      CT s = CT(0);
+#pragma unroll
+      for (int J2 = 0; J2 < 3; ++J2)
+        s += P[k][J2] * Ji[J2][m];
+      Q[k][m] = wJ * s;
+    }
+  FT *out = reinterpret_cast<FT *>(A.fe) + (A.pos_offset + i) * 12;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    out[0 * 3 + k] = FT(-(Q[k][0] + Q[k][1] + Q[k][2]));
+    out[1 * 3 + k] = FT(Q[k][1]);
+    out[2 * 3 + k] = FT(Q[k][2]);
+    out[3 * 3 + k] = FT(Q[k][0]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+hipError_t upload_hex8_dnds(const double *dnds) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_dnds), dnds, sizeof(double) * 192);
+}
+
+int64_t elements_per_block(int etype) { return etype == 1 ? HEX_EPB : 256; }
+
+template <class CT, class FT>
+static hipError_t launch_t(int etype, int law, ElemArgs A, hipStream_t s) {
+  dim3 grid((unsigned)A.nblocks_padded), block(256);
+#define AKXF_CASE(L)                                                           \
+  case L:                                                                      \
+    if (etype == 1)                                                            \
+      hipLaunchKernelGGL((k_hex8_fast<L, CT, FT>), grid, block, 0, s, A);      \
+    else                                                                       \
+      hipLaunchKernelGGL((k_tet4_fast<L, CT, FT>), grid, block, 0, s, A);      \
+    break;
+  switch (law) {
+    AKXF_CASE(LAW_ELASTIC)
+    AKXF_CASE(LAW_ELASTIC_FD)
+    AKXF_CASE(LAW_NEOHOOKEAN)
+    AKXF_CASE(LAW_PLASTIC)
+  default:
+    return hipErrorInvalidValue;
+  }
+#undef AKXF_CASE
+  return hipGetLastError();
+}
+
+hipError_t launch_elements(int etype, int law, bool fp32, const ElemArgs &A,
+                           hipStream_t stream) {
+  return fp32 ? launch_t<float, float>(etype, law, A, stream)
+              : launch_t<double, double>(etype, law, A, stream);
+}
+
+} // namespace akxf

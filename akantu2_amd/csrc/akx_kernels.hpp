@@ -13,6 +13,7 @@
 //                    sequential scatter order of dof_manager.cc:92-114)
 #pragma once
 
+#include "akx_args.hpp"
 #include "akx_math.hpp"
 #include <stdint.h>
 
@@ -95,9 +96,9 @@ template <> struct Shape<ET_HEX8, double> {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        double s = 0.;
+        double s = c_hex8.dnds[q][i][0] * X[0][j];
 #pragma unroll
-        for (int n = 0; n < 8; ++n)
+        for (int n = 1; n < 8; ++n)
           s += c_hex8.dnds[q][i][n] * X[n][j];
         J[i][j] = s;
       }
@@ -108,64 +109,9 @@ template <> struct Shape<ET_HEX8, double> {
                                               int n, double (&b)[3]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      double s = 0.;
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        s += Ji[i][k] * c_hex8.dnds[q][k][n];
-      b[i] = s;
-    }
-  }
-};
-
-// FP32 form: coordinates relative to node 0 (exact FP64 differences rounded
-// once), so the FP32 Jacobian does not lose the mesh offset.
-template <> struct Shape<ET_TET4, float> {
-  __device__ static __forceinline__ float
-  jacobian_at(const float (&dX)[4][3], int, float (&Ji)[3][3], float &detJ) {
-    float J[3][3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      J[0][j] = dX[3][j];
-      J[1][j] = dX[1][j];
-      J[2][j] = dX[2][j];
-    }
-    detJ = jacobian_det_inv(J, Ji);
-    return detJ * (1.f / 6.f);
-  }
-  __device__ static __forceinline__ void bcol(const float (&Ji)[3][3], int,
-                                              int n, float (&b)[3]) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-      b[i] = n == 0 ? -Ji[i][0] - Ji[i][1] - Ji[i][2]
-                    : (n == 1 ? Ji[i][1] : (n == 2 ? Ji[i][2] : Ji[i][0]));
-  }
-};
-
-template <> struct Shape<ET_HEX8, float> {
-  __device__ static __forceinline__ float
-  jacobian_at(const float (&dX)[8][3], int q, float (&Ji)[3][3], float &detJ) {
-    float J[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        float s = 0.f;
-#pragma unroll
-        for (int n = 1; n < 8; ++n)
-          s += c_hex8.dnds_f[q][i][n] * dX[n][j];
-        J[i][j] = s;
-      }
-    detJ = jacobian_det_inv(J, Ji);
-    return detJ; // Hex8 Gauss weights are all 1
-  }
-  __device__ static __forceinline__ void bcol(const float (&Ji)[3][3], int q,
-                                              int n, float (&b)[3]) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      float s = 0.f;
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        s += Ji[i][k] * c_hex8.dnds_f[q][k][n];
+      double s = Ji[i][0] * c_hex8.dnds[q][0][n];
+      s += Ji[i][1] * c_hex8.dnds[q][1][n];
+      s += Ji[i][2] * c_hex8.dnds[q][2][n];
       b[i] = s;
     }
   }
@@ -183,68 +129,55 @@ __device__ __forceinline__ void btd_small(const R (&S)[3][3], const R (&b)[3],
                                           R (&r)[3]) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    R s = R(0);
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-      s += S[i][j] * b[j];
+    R s = S[i][0] * b[0];
+    s += S[i][1] * b[1];
+    s += S[i][2] * b[2];
     r[i] = s;
   }
+}
+
+// one component k of (B_L + B_NL)^T voigt(S) for node column b; entries of
+// B_L that are zero are not added (they only affect the sign of zeros)
+template <int k, class R>
+__device__ __forceinline__ R btd_finite_k(const R (&g)[3][3], const R (&S)[3][3],
+                                          const R (&b)[3]) {
+  // rows 0..2: B_L(I, 3a+k) = [k == I] B(I, a); B_NL = g(k, I) B(I, a)
+  R t0 = g[k][0] * b[0];
+  R t1 = g[k][1] * b[1];
+  R t2 = g[k][2] * b[2];
+  if constexpr (k == 0) t0 = b[0] + t0;
+  if constexpr (k == 1) t1 = b[1] + t1;
+  if constexpr (k == 2) t2 = b[2] + t2;
+  // rows 3 (yz), 4 (xz), 5 (xy): B_NL sums two products, B_L one entry
+  R n3 = g[k][1] * b[2] + g[k][2] * b[1];
+  R n4 = g[k][0] * b[2] + g[k][2] * b[0];
+  R n5 = g[k][0] * b[1] + g[k][1] * b[0];
+  if constexpr (k == 1) n3 = b[2] + n3;
+  if constexpr (k == 2) n3 = b[1] + n3;
+  if constexpr (k == 0) n4 = b[2] + n4;
+  if constexpr (k == 2) n4 = b[0] + n4;
+  if constexpr (k == 0) n5 = b[1] + n5;
+  if constexpr (k == 1) n5 = b[0] + n5;
+  R s = t0 * S[0][0];
+  s += t1 * S[1][1];
+  s += t2 * S[2][2];
+  s += n3 * S[1][2];
+  s += n4 * S[0][2];
+  s += n5 * S[0][1];
+  return s;
 }
 
 template <class R>
 __device__ __forceinline__ void btd_finite(const R (&g)[3][3], const R (&S)[3][3],
                                            const R (&b)[3], R (&r)[3]) {
-  const R S00 = S[0][0], S11 = S[1][1], S22 = S[2][2];
-  const R S12 = S[1][2], S02 = S[0][2], S01 = S[0][1];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    // rows 0..2: B_L(I, 3a+k) = [k == I] B(I, a); B_NL = g(k, I) B(I, a)
-    R t0 = ((k == 0) ? b[0] : R(0)) + g[k][0] * b[0];
-    R t1 = ((k == 1) ? b[1] : R(0)) + g[k][1] * b[1];
-    R t2 = ((k == 2) ? b[2] : R(0)) + g[k][2] * b[2];
-    // row 3 (yz): B_L = [k==1] dndz + [k==2] dndy
-    R l3 = (k == 1) ? b[2] : ((k == 2) ? b[1] : R(0));
-    R n3 = g[k][1] * b[2] + g[k][2] * b[1];
-    // row 4 (xz): B_L = [k==0] dndz + [k==2] dndx
-    R l4 = (k == 0) ? b[2] : ((k == 2) ? b[0] : R(0));
-    R n4 = g[k][0] * b[2] + g[k][2] * b[0];
-    // row 5 (xy): B_L = [k==0] dndy + [k==1] dndx
-    R l5 = (k == 0) ? b[1] : ((k == 1) ? b[0] : R(0));
-    R n5 = g[k][0] * b[1] + g[k][1] * b[0];
-    R s = t0 * S00;
-    s += t1 * S11;
-    s += t2 * S22;
-    s += (l3 + n3) * S12;
-    s += (l4 + n4) * S02;
-    s += (l5 + n5) * S01;
-    r[k] = s;
-  }
+  r[0] = btd_finite_k<0>(g, S, b);
+  r[1] = btd_finite_k<1>(g, S, b);
+  r[2] = btd_finite_k<2>(g, S, b);
 }
 
 // ---------------------------------------------------------------------------
 // Element pass.  One thread per element of one material.
 // ---------------------------------------------------------------------------
-struct ElemArgs {
-  int64_t n;              // elements in this launch
-  int64_t nel;            // elements in the mesh (connectivity stride)
-  const int32_t *filter;  // element ids, or nullptr (ids first + i)
-  int64_t first;
-  int64_t pos_offset;     // assembly position of the first element
-  const int32_t *conn;    // [nel][nnpe]
-  const double *X;        // [3][nn]
-  const double *u;        // [3][nn]
-  int64_t nn;
-  double lambda, mu, sigma_y, h;
-  const double *lam_el, *mu_el, *sy_el, *h_el; // per launch element, or null
-  void *fe;               // [slot][3] of FT
-  // materialised internals [9][n*nqp] (or null), column-major 3x3 index
-  double *grad_u, *stress;
-  // plastic history, [9][n*nqp] and [n*nqp]: previous (read) / current (write)
-  const double *p_grad, *p_sig, *p_eps, *p_iso;
-  double *c_grad, *c_sig, *c_eps, *c_iso;
-  int64_t nblocks_padded;
-};
-
 template <int ET, int LAW, class CT, class FT>
 __global__ void __launch_bounds__(256)
 k_element_forces(ElemArgs A) {
@@ -268,31 +201,13 @@ k_element_forces(ElemArgs A) {
   }
 
   CT X[NN][3], U[NN][3];
-  if constexpr (sizeof(CT) == 8) {
 #pragma unroll
-    for (int n = 0; n < NN; ++n)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        X[n][c] = A.X[(int64_t)c * A.nn + nodes[n]];
-        U[n][c] = A.u[(int64_t)c * A.nn + nodes[n]];
-      }
-  } else {
-    double X0[3], U0[3];
+  for (int n = 0; n < NN; ++n)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      X0[c] = A.X[(int64_t)c * A.nn + nodes[0]];
-      U0[c] = A.u[(int64_t)c * A.nn + nodes[0]];
-      X[0][c] = CT(0);
-      U[0][c] = CT(0);
+      X[n][c] = A.X[(int64_t)c * A.nn + nodes[n]];
+      U[n][c] = A.u[(int64_t)c * A.nn + nodes[n]];
     }
-#pragma unroll
-    for (int n = 1; n < NN; ++n)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        X[n][c] = CT(A.X[(int64_t)c * A.nn + nodes[n]] - X0[c]);
-        U[n][c] = CT(A.u[(int64_t)c * A.nn + nodes[n]] - U0[c]);
-      }
-  }
 
   CT lambda, mu, sy = CT(A.sigma_y), hh = CT(A.h);
   if (A.lam_el) {
@@ -308,11 +223,6 @@ k_element_forces(ElemArgs A) {
     hh = CT(A.h_el[i]);
 
   CT re[NN][3];
-#pragma unroll
-  for (int n = 0; n < NN; ++n)
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      re[n][k] = CT(0);
 
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
@@ -321,11 +231,6 @@ k_element_forces(ElemArgs A) {
     // grad u (i, j) = sum_n u(i, n) B(j, n), summed over n in order
     CT g[3][3];
 #pragma unroll
-    for (int ii = 0; ii < 3; ++ii)
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-        g[ii][jj] = CT(0);
-#pragma unroll
     for (int n = 0; n < NN; ++n) {
       CT b[3];
       Shape<ET, CT>::bcol(Ji, q, n, b);
@@ -333,7 +238,7 @@ k_element_forces(ElemArgs A) {
       for (int ii = 0; ii < 3; ++ii)
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj)
-          g[ii][jj] += U[n][ii] * b[jj];
+          g[ii][jj] = (n == 0) ? U[n][ii] * b[jj] : g[ii][jj] + U[n][ii] * b[jj];
     }
     const int64_t iq = i * NQ + q;
     const int64_t nq_all = A.n * NQ;
@@ -390,7 +295,7 @@ k_element_forces(ElemArgs A) {
         btd_small(S, b, r);
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        re[n][k] += r[k] * wJ;
+        re[n][k] = (q == 0) ? r[k] * wJ : re[n][k] + r[k] * wJ;
     }
   }
 
@@ -400,6 +305,150 @@ k_element_forces(ElemArgs A) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       out[n * 3 + k] = FT(re[n][k]);
+}
+
+
+// ---------------------------------------------------------------------------
+// Hex8 element pass in the reference's operation order, one lane per Gauss
+// point (8 lanes per element, 16 elements per 128-thread block).  Nodal
+// coordinates and displacements are staged in LDS; each lane forms J, its
+// LU inverse, the dN/dX columns, grad u, the law and the 24 force terms of
+// its Gauss point times w*detJ; the per-node sums over the Gauss points are
+// then taken in ascending Gauss-point order (the oracle's and Eigen's GEMV
+// order) by the lane that owns the node.
+// ---------------------------------------------------------------------------
+constexpr int HEX_STRICT_EPB = 16;
+
+template <int LAW>
+__global__ void __launch_bounds__(128) k_hex8_strict(ElemArgs A) {
+  __shared__ double sN[HEX_STRICT_EPB][8][6];
+  __shared__ double sR[HEX_STRICT_EPB][8][25]; // 24 terms (+1 pad: banks)
+  __shared__ double sD[8][3][8];
+  const int t = threadIdx.x, el = t >> 3, q = t & 7;
+  for (int k = t; k < 192; k += 128)
+    (&sD[0][0][0])[k] = (&c_hex8.dnds[0][0][0])[k];
+  const int64_t i = xcd_block(A.nblocks_padded) * HEX_STRICT_EPB + el;
+  const bool active = i < A.n;
+  if (active) {
+    const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+    const int32_t node = A.conn[e * 8 + q];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      sN[el][q][c] = A.X[(int64_t)c * A.nn + node];
+      sN[el][q][3 + c] = A.u[(int64_t)c * A.nn + node];
+    }
+  }
+  __syncthreads();
+  if (active) {
+    // J(i, j) = sum_n dnds(i, n) X(n, j)
+    double J[3][3], Ji[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double s = sD[q][a][0] * sN[el][0][j];
+#pragma unroll
+        for (int n = 1; n < 8; ++n)
+          s += sD[q][a][n] * sN[el][n][j];
+        J[a][j] = s;
+      }
+    double detJ = jacobian_det_inv(J, Ji);
+    const double wJ = detJ * c_hex8.w[q];
+    auto bcol = [&](int n, double (&b)[3]) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double s = Ji[a][0] * sD[q][0][n];
+        s += Ji[a][1] * sD[q][1][n];
+        s += Ji[a][2] * sD[q][2][n];
+        b[a] = s;
+      }
+    };
+    double g[3][3];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      double b[3];
+      bcol(n, b);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double un = sN[el][n][3 + a];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          g[a][j] = (n == 0) ? un * b[j] : g[a][j] + un * b[j];
+      }
+    }
+    double lambda = A.lam_el ? A.lam_el[i] : A.lambda;
+    double mu = A.mu_el ? A.mu_el[i] : A.mu;
+    const int64_t iq = i * 8 + q, nq_all = A.n * 8;
+    double S[3][3];
+    if constexpr (LAW == LAW_ELASTIC) {
+      elastic_stress(lambda, mu, g, -0.0, S);
+    } else if constexpr (LAW == LAW_ELASTIC_FD) {
+      double E[3][3];
+      green_strain(g, E);
+      elastic_stress(lambda, mu, E, -0.0, S);
+    } else if constexpr (LAW == LAW_NEOHOOKEAN) {
+      neohookean_pk2(lambda, mu, g, S);
+    } else {
+      double sy = A.sy_el ? A.sy_el[i] : A.sigma_y;
+      double hh = A.h_el ? A.h_el[i] : A.h;
+      double pg[3][3], ps[3][3], pe[3][3], eps[3][3], iso;
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
+          pg[ii][jj] = A.p_grad[o];
+          ps[ii][jj] = A.p_sig[o];
+          pe[ii][jj] = A.p_eps[o];
+        }
+      plastic_lih(lambda, mu, sy, hh, g, pg, ps, A.p_iso[iq], pe, S, iso, eps);
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
+          A.c_grad[o] = g[ii][jj];
+          A.c_sig[o] = S[ii][jj];
+          A.c_eps[o] = eps[ii][jj];
+        }
+      A.c_iso[iq] = iso;
+    }
+    if (A.grad_u) {
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
+          A.grad_u[o] = g[ii][jj];
+          A.stress[o] = S[ii][jj];
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      double b[3], r[3];
+      bcol(n, b);
+      if constexpr (LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN)
+        btd_finite(g, S, b, r);
+      else
+        btd_small(S, b, r);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        sR[el][q][3 * n + k] = r[k] * wJ;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    const int n = q; // this lane now owns node n of the element
+    double *out = reinterpret_cast<double *>(A.fe) + (A.pos_offset + i) * 24 + n * 3;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double s = sR[el][0][3 * n + k];
+#pragma unroll
+      for (int qq = 1; qq < 8; ++qq)
+        s += sR[el][qq][3 * n + k];
+      out[k] = s;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -8,13 +8,24 @@
 // contraction in its own kernels.
 //
 // Tensors are 3x3 arrays t[i][j] (row index i, column index j).
+//
+// Terms that are exactly zero in the reference's expressions (adding the
+// off-diagonal of lambda tr I, the 0 that starts a sum, 0 * mu) are left
+// out: that changes at most the sign of a zero result, never a value.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #define AKX_HD __host__ __device__ __forceinline__
 
-namespace akx {
+// The namespace is a macro so that a translation unit compiled with a
+// different floating-point contraction setting (the fast FP64 / FP32 element
+// kernels) gets its own copies of these inline functions.
+#ifndef AKX_MATH_NS
+#define AKX_MATH_NS akx
+#endif
+
+namespace AKX_MATH_NS {
 
 template <class R> struct M3 {
   R m[3][3];
@@ -169,7 +180,8 @@ AKX_HD void elastic_stress(R lambda, R mu, const R (&g)[3][3], R sigma_th,
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       R eps = R(.5) * (g[j][i] + g[i][j]);
-      s[i][j] = two_mu * eps + sd * (i == j ? R(1) : R(0));
+      // (lambda tr + sigma_th) * I: the off-diagonal term adds a zero
+      s[i][j] = (i == j) ? two_mu * eps + sd : two_mu * eps;
     }
 }
 
@@ -179,9 +191,9 @@ template <class R> AKX_HD void green_strain(const R (&g)[3][3], R (&E)[3][3]) {
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      R p = R(0);
+      R p = g[0][i] * g[0][j];
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
+      for (int k = 1; k < 3; ++k)
         p += g[k][i] * g[k][j];
       E[i][j] = (p + g[j][i] + g[i][j]) / R(2);
     }
@@ -202,7 +214,63 @@ AKX_HD void neohookean_pk2(R lambda, R mu, const R (&g)[3][3], R (&S)[3][3]) {
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j)
-      F[i][j] = (i == j ? R(1) : R(0)) + g[i][j];
+      F[i][j] = (i == j) ? R(1) + g[i][j] : g[i][j]; // I + grad u
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      R s = F[0][i] * F[0][j];
+#pragma unroll
+      for (int k = 1; k < 3; ++k)
+        s += F[k][i] * F[k][j];
+      C[i][j] = s;
+    }
+  R J = det3(F); // * sqrt(C33), C33 = 1 in 3-D
+  inv3(C, Ci);
+  R t = lambda * akx_log(J) - mu;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) // I * mu + t C^{-1}
+      S[i][j] = (i == j) ? mu + t * Ci[i][j] : t * Ci[i][j];
+}
+
+// The same neo-Hookean PK2 written without cancellation, for the FP32 form:
+// mu (I - C^{-1}) = 2 mu C^{-1} E with the Green strain E formed from grad u
+// directly, and ln J = log1p(J - 1) with
+// J - 1 = tr g + ((tr g)^2 - tr(g g)) / 2 + det g.
+// At strains of 1e-3 the reference's form keeps only ~4 significant digits
+// in FP32; this one keeps full FP32 accuracy.
+template <class R> AKX_HD R akx_log1p(R x);
+template <> AKX_HD double akx_log1p<double>(double x) { return log1p(x); }
+template <> AKX_HD float akx_log1p<float>(float x) { return log1pf(x); }
+
+template <class R>
+AKX_HD void neohookean_pk2_stable(R lambda, R mu, const R (&g)[3][3],
+                                  R (&S)[3][3]) {
+  R E[3][3], C[3][3], Ci[3][3];
+  green_strain(g, E);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      C[i][j] = (i == j) ? R(1) + R(2) * E[i][j] : R(2) * E[i][j];
+  R tr = g[0][0] + g[1][1] + g[2][2];
+  R trg2 = R(0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      trg2 += g[i][k] * g[k][i];
+  R jm1 = tr + R(0.5) * (tr * tr - trg2) + det3(g);
+  R lnJ = akx_log1p(jm1);
+  inv3(C, Ci);
+  R T[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      T[i][j] = (i == j) ? R(2) * mu * E[i][j] + lambda * lnJ : R(2) * mu * E[i][j];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -210,17 +278,9 @@ AKX_HD void neohookean_pk2(R lambda, R mu, const R (&g)[3][3], R (&S)[3][3]) {
       R s = R(0);
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        s += F[k][i] * F[k][j];
-      C[i][j] = s;
+        s += Ci[i][k] * T[k][j];
+      S[i][j] = s;
     }
-  R J = det3(F) * R(1); // * sqrt(C33), C33 = 1 in 3-D
-  inv3(C, Ci);
-  R t = lambda * akx_log(J) - mu;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-      S[i][j] = (i == j ? R(1) : R(0)) * mu + t * Ci[i][j];
 }
 
 // Linear isotropic hardening, small strain, radial return:
@@ -248,14 +308,15 @@ AKX_HD void plastic_lih(R lambda, R mu, R sigma_y, R h, const R (&g)[3][3],
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j)
-      dev[i][j] = str[i][j] - (i == j ? R(1) : R(0)) * tr / R(3);
+      dev[i][j] = (i == j) ? str[i][j] - tr / R(3) : str[i][j];
   // sum of squares in column-major order (cwiseProduct().sum())
-  R s = R(0);
+  R s = dev[0][0] * dev[0][0];
 #pragma unroll
   for (int j = 0; j < 3; ++j)
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-      s += dev[i][j] * dev[i][j];
+      if (i + j > 0)
+        s += dev[i][j] * dev[i][j];
   R eff = akx_sqrt(R(3) / R(2) * s);
   bool yielding = ((eff - piso - sigma_y) > R(0));
   R dp = yielding ? (eff - sigma_y - piso) / (R(3) * mu + h) : R(0);
@@ -280,4 +341,4 @@ AKX_HD void plastic_lih(R lambda, R mu, R sigma_y, R h, const R (&g)[3][3],
       sigma[i][j] = ps[i][j] + sel[i][j];
 }
 
-} // namespace akx
+} // namespace AKX_MATH_NS

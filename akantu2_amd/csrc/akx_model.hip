@@ -7,6 +7,7 @@
 // with the predictor of step k+1 fused into the gather of step k when
 // several steps are queued together.
 #include "../../include/akantu_mi355x.h"
+#include "akx_fast.hpp"
 #include "akx_kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -427,7 +428,7 @@ struct akx_model {
   }
 
   size_t fe_elem_size() const {
-    return precision == AKX_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+    return precision == AKX_PRECISION_FP32 ? sizeof(float) : sizeof(double);
   }
 };
 
@@ -496,6 +497,7 @@ void upload_hex8_tables() {
         t.dnds_f[q][i][n] = (float)d[i][n];
   }
   HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_hex8), &t, sizeof(t)));
+  HIPCHECK(akxf::upload_hex8_dnds(&t.dnds[0][0][0]));
   if (dev < 64)
     done_device[dev] = true;
 }
@@ -571,6 +573,30 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
   return A;
 }
 
+// Host-side checks that every buffer a kernel variant dereferences exists
+// (a kernel must never be launched with a null pointer it will use).
+void check_gather_args(const GatherArgs &G, bool update, bool comp) {
+  require(G.rowptr && G.slots && G.fe && G.fint, "gather launched without its table");
+  if (update)
+    require(G.mass && G.blocked && G.u && G.v && G.a,
+            "nodal update launched without nodal state");
+  if (comp)
+    require(G.cu && G.cv, "compensated update launched without compensation terms");
+}
+
+void check_elem_args(const ElemArgs &A, int law) {
+  require(A.conn && A.X && A.u && A.fe, "element pass launched without mesh or state");
+  require(A.n <= 0 || A.nblocks_padded > 0, "element pass launched with an empty grid");
+  if (law == AKX_LAW_PLASTIC_LIH)
+    require(A.p_grad && A.p_sig && A.p_eps && A.p_iso && A.c_grad && A.c_sig &&
+                A.c_eps && A.c_iso,
+            "plastic element pass launched without history buffers");
+  if (A.lam_el || A.mu_el)
+    require(A.lam_el && A.mu_el, "per-element parameters incomplete");
+  if (A.grad_u || A.stress)
+    require(A.grad_u && A.stress, "materialised internals incomplete");
+}
+
 template <int ET, class CT, class FT>
 void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
   dim3 grid((unsigned)A.nblocks_padded), block(TPB);
@@ -597,24 +623,56 @@ void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
   HIPCHECK(hipGetLastError());
 }
 
+void launch_hex8_strict(akx_model *m, Material &mat, const ElemArgs &A) {
+  dim3 grid((unsigned)A.nblocks_padded), block(128);
+  switch (mat.law) {
+  case AKX_LAW_ELASTIC:
+    hipLaunchKernelGGL(k_hex8_strict<LAW_ELASTIC>, grid, block, 0, m->stream, A);
+    break;
+  case AKX_LAW_ELASTIC_FD:
+    hipLaunchKernelGGL(k_hex8_strict<LAW_ELASTIC_FD>, grid, block, 0, m->stream, A);
+    break;
+  case AKX_LAW_NEOHOOKEAN:
+    hipLaunchKernelGGL(k_hex8_strict<LAW_NEOHOOKEAN>, grid, block, 0, m->stream, A);
+    break;
+  case AKX_LAW_PLASTIC_LIH:
+    hipLaunchKernelGGL(k_hex8_strict<LAW_PLASTIC>, grid, block, 0, m->stream, A);
+    break;
+  default:
+    throw Error(AKX_ERR_UNSUPPORTED, "unknown law");
+  }
+  HIPCHECK(hipGetLastError());
+}
+
 void launch_element_pass(akx_model *m, bool materialize) {
+  const bool strict = m->precision == AKX_PRECISION_FP64;
   for (auto &mp : m->mats) {
     Material &mat = *mp;
     if (mat.n == 0)
       continue;
     ElemArgs A = elem_args(m, mat, materialize);
+    check_elem_args(A, mat.law);
+    require(m->fe.n >= (size_t)((mat.pos_offset + mat.n) * m->nnpe * 3 *
+                                m->fe_elem_size()),
+            "element force buffer too small for this material");
+    if (strict && m->etype == AKX_HEX8) {
+      int64_t nb = (mat.n + HEX_STRICT_EPB - 1) / HEX_STRICT_EPB;
+      A.nblocks_padded = (nb + 7) / 8 * 8;
+    } else if (!strict) {
+      int64_t epb = akxf::elements_per_block(m->etype);
+      int64_t nb = (mat.n + epb - 1) / epb;
+      A.nblocks_padded = (nb + 7) / 8 * 8;
+    }
     m->timed(0, [&] {
-      bool f64 = m->precision == AKX_PRECISION_FP64;
-      if (m->etype == AKX_TET4) {
-        if (f64)
+      if (strict) {
+        if (m->etype == AKX_TET4)
           launch_elem_law<ET_TET4, double, double>(m, mat, A);
         else
-          launch_elem_law<ET_TET4, float, float>(m, mat, A);
+          launch_hex8_strict(m, mat, A);
       } else {
-        if (f64)
-          launch_elem_law<ET_HEX8, double, double>(m, mat, A);
-        else
-          launch_elem_law<ET_HEX8, float, float>(m, mat, A);
+        HIPCHECK(akxf::launch_elements(m->etype, mat.law,
+                                       m->precision == AKX_PRECISION_FP32, A,
+                                       m->stream));
       }
     });
   }
@@ -649,7 +707,8 @@ GatherArgs gather_args(akx_model *m) {
 void launch_gather(akx_model *m, bool update, bool predict) {
   GatherArgs G = gather_args(m);
   dim3 grid((unsigned)G.nblocks_padded), block(TPB);
-  bool f64 = m->precision == AKX_PRECISION_FP64;
+  const bool f64 = m->precision != AKX_PRECISION_FP32;
+  check_gather_args(G, update, update && !f64);
   m->timed(1, [&] {
     if (!update) {
       if (f64)
@@ -680,8 +739,9 @@ void launch_gather(akx_model *m, bool update, bool predict) {
 void launch_predictor(akx_model *m) {
   GatherArgs G = gather_args(m);
   dim3 grid((unsigned)G.nblocks_padded), block(TPB);
+  check_gather_args(G, true, m->precision == AKX_PRECISION_FP32);
   m->timed(2, [&] {
-    if (m->precision == AKX_PRECISION_FP64)
+    if (m->precision != AKX_PRECISION_FP32)
       hipLaunchKernelGGL((k_predictor<false>), grid, block, 0, m->stream, G);
     else
       hipLaunchKernelGGL((k_predictor<true>), grid, block, 0, m->stream, G);
@@ -997,7 +1057,8 @@ int akx_model_set_precision(akx_model *m, int precision) {
   return guard([&] {
     require(m != nullptr, "null model");
     require(!m->initialized, "precision must be set before initFull");
-    require(precision == AKX_PRECISION_FP64 || precision == AKX_PRECISION_FP32,
+    require(precision == AKX_PRECISION_FP64 || precision == AKX_PRECISION_FP32 ||
+                precision == AKX_PRECISION_FP64_FAST,
             "unknown precision");
     m->precision = precision;
   });
@@ -1345,7 +1406,7 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
     require(bytes != nullptr, "null output");
     const double s_node = 8.0; // FP64 nodal state
     const double s_f = (double)m->fe_elem_size();
-    const double s_c = m->precision == AKX_PRECISION_FP64 ? 8.0 : 4.0;
+    const double s_c = m->precision == AKX_PRECISION_FP32 ? 4.0 : 8.0;
     int64_t nel_all = 0;
     double extra = 0.;
     for (auto &mp : m->mats) {

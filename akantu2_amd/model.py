@@ -152,7 +152,8 @@ class SolidMechanicsModel:
         return self.materials[i]
 
     def setPrecision(self, precision):
-        p = {"fp64": N.PRECISION_FP64, "fp32": N.PRECISION_FP32}[precision]
+        p = {"fp64": N.PRECISION_FP64, "fp32": N.PRECISION_FP32,
+             "fp64_fast": N.PRECISION_FP64_FAST}[precision]
         N.check(N.lib().akx_model_set_precision(self._h, p))
 
     # ---------------------------------------------------------------- init

@@ -60,9 +60,15 @@ extern "C" {
 #define AKX_LAW_PLASTIC_LIH 3
 
 /* Arithmetic of the element pass */
-#define AKX_PRECISION_FP64 0 /* plain path: everything in FP64 */
+#define AKX_PRECISION_FP64 0 /* plain path (default): FP64 in the reference's
+                                operation order, no FMA; the CPU oracle is
+                                reproduced bit for bit except the last bit
+                                of log() */
 #define AKX_PRECISION_FP32 1 /* grad u, stress, B^T sigma in FP32; nodal
                                 update in FP64 with compensated summation */
+#define AKX_PRECISION_FP64_FAST 2 /* FP64 with FMA and the regrouped
+                                contraction r_e = sum_q Q_q dN/dxi_q; equal to
+                                the plain path to rounding per evaluation */
 
 /* Nodal fields, [nb_nodes][3] doubles unless stated */
 #define AKX_FIELD_DISPLACEMENT 0

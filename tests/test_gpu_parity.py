@@ -1,7 +1,8 @@
-"""Parity of the gfx950 FP64 plain path with the CPU oracle (same seeded
-inputs).  Bar: 1e-12 relative (max norm) on u, v and f_int after N steps --
-the tolerance the explicit path is held to; in practice the FP64 path matches
-the oracle to the last bit except for log() in the neo-Hookean law."""
+"""Parity of the gfx950 plain FP64 path with the CPU oracle (same seeded
+inputs).  Bar: 1e-12 relative (max norm) on u, v, a and f_int after N steps,
+the tolerance the explicit path is held to.  The plain path keeps the
+reference's operation order and in practice reproduces the oracle bit for
+bit, except for the last bit of log() in the neo-Hookean law."""
 import os
 
 import numpy as np
@@ -26,31 +27,36 @@ def _compare(g, o, tag=""):
     return eu, ev, ef
 
 
-def test_c1_plain_fp64_1000_steps():
+PRECISIONS = ["fp64"]
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_c1_plain_fp64_1000_steps(precision):
     """Config 1: 20x4x4 Kuhn Tet4 bar, E=1 nu=0.3 rho=1, clamped x=0, seeded
     sin-enveloped initial velocity, dt = 0.5 dt_crit, 1000 steps."""
     mesh, blocked, v0 = workload("_tetrahedron_4", (20, 4, 4), seed=0)
     o, dt = make_oracle(mesh, blocked, v0, "elastic")
-    g, dtg = make_gpu(mesh, blocked, v0, "elastic")
+    g, dtg = make_gpu(mesh, blocked, v0, "elastic", precision=precision)
     assert dtg == dt  # same stable time step to the bit
     np.testing.assert_array_equal(g.getMass(), o.mass)
     for k in range(10):
         o.solve_steps(100)
         g.solveStep(100)
         _compare(g, o, f"C1 step {(k + 1) * 100}")
-    # the plain path reproduces the oracle bit for bit
-    assert np.array_equal(g.getDisplacement(), o.u)
-    assert np.array_equal(g.getInternalForce(), o.f_int)
+    if precision == "fp64":  # bit for bit
+        assert np.array_equal(g.getDisplacement(), o.u)
+        assert np.array_equal(g.getInternalForce(), o.f_int)
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("etype,law", [
     ("_hexahedron_8", "neohookean"), ("_hexahedron_8", "elastic"),
     ("_hexahedron_8", "elastic_fd"), ("_tetrahedron_4", "neohookean"),
     ("_tetrahedron_4", "elastic_fd")])
-def test_laws_and_elements_jittered(etype, law):
+def test_laws_and_elements_jittered(etype, law, precision):
     mesh, blocked, v0 = workload(etype, (8, 4, 3), jitter=0.1, seed=3)
     o, dt = make_oracle(mesh, blocked, v0, law, E=1.0, nu=0.3)
-    g, dtg = make_gpu(mesh, blocked, v0, law, E=1.0, nu=0.3)
+    g, dtg = make_gpu(mesh, blocked, v0, law, E=1.0, nu=0.3, precision=precision)
     assert dtg == dt
     for k in range(3):
         o.solve_steps(100)
@@ -58,24 +64,27 @@ def test_laws_and_elements_jittered(etype, law):
         _compare(g, o, f"{etype} {law} step {(k + 1) * 100}")
 
 
-def test_c2_reduced_checked_every_100_steps():
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_c2_reduced_checked_every_100_steps(precision):
     """Config 2 at reduced size: jittered Hex8 neo-Hookean, checked against
-    the oracle at every 100th step."""
+    the oracle at every 100th step over 1000 steps."""
     mesh, blocked, v0 = workload("_hexahedron_8", (20, 10, 10), jitter=0.1, seed=1)
     o, dt = make_oracle(mesh, blocked, v0, "neohookean")
-    g, _ = make_gpu(mesh, blocked, v0, "neohookean", dt=dt)
-    for k in range(5):
+    g, _ = make_gpu(mesh, blocked, v0, "neohookean", dt=dt, precision=precision)
+    for k in range(10):
         o.solve_steps(100)
         g.solveStep(100)
         _compare(g, o, f"C2-reduced step {(k + 1) * 100}")
 
 
-def test_plastic_linear_isotropic_hardening():
-    mesh, blocked, v0 = workload("_tetrahedron_4", (10, 3, 3), jitter=0.05, seed=5,
-                                 amp=3e-2)
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("etype", ["_tetrahedron_4", "_hexahedron_8"])
+def test_plastic_linear_isotropic_hardening(etype, precision):
+    mesh, blocked, v0 = workload(etype, (10, 3, 3), jitter=0.05, seed=5, amp=3e-2)
     kw = dict(E=1.0, nu=0.3, sigma_y=2e-3, h=0.05)
     o, dt = make_oracle(mesh, blocked, v0, "plastic_linear_isotropic_hardening", **kw)
-    g, _ = make_gpu(mesh, blocked, v0, "plastic_linear_isotropic_hardening", dt=dt, **kw)
+    g, _ = make_gpu(mesh, blocked, v0, "plastic_linear_isotropic_hardening", dt=dt,
+                    precision=precision, **kw)
     for k in range(3):
         o.solve_steps(100)
         g.solveStep(100)
@@ -87,24 +96,27 @@ def test_plastic_linear_isotropic_hardening():
     assert relerr(mat.getInelasticStrain(), o.eps_p) <= TOL
 
 
-def test_per_element_parameters():
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_per_element_parameters(precision):
     """Config 3's per-element E (log-uniform [0.5, 2]) and nu (U[0.2, 0.35])."""
     mesh, blocked, v0 = workload("_tetrahedron_4", (10, 4, 4), jitter=0.1, seed=7)
     rng = np.random.default_rng(7)
     E = np.exp(rng.uniform(np.log(0.5), np.log(2.0), mesh.nb_elements))
     nu = rng.uniform(0.2, 0.35, mesh.nb_elements)
     o, dt = make_oracle(mesh, blocked, v0, "elastic", E_el=E, nu_el=nu)
-    g, dtg = make_gpu(mesh, blocked, v0, "elastic", E_el=E, nu_el=nu)
+    g, dtg = make_gpu(mesh, blocked, v0, "elastic", E_el=E, nu_el=nu,
+                      precision=precision)
     assert dtg == dt
     o.solve_steps(200)
     g.solveStep(200)
     _compare(g, o, "per-element params")
 
 
-def test_stresses_and_gradients_match():
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_stresses_and_gradients_match(precision):
     mesh, blocked, v0 = workload("_hexahedron_8", (6, 3, 3), jitter=0.1, seed=2)
     o, dt = make_oracle(mesh, blocked, v0, "neohookean")
-    g, _ = make_gpu(mesh, blocked, v0, "neohookean", dt=dt)
+    g, _ = make_gpu(mesh, blocked, v0, "neohookean", dt=dt, precision=precision)
     o.solve_steps(50)
     g.solveStep(50)
     mat = g.getMaterial(0)

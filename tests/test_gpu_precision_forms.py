@@ -1,0 +1,63 @@
+"""The opt-in arithmetic forms against the FP64 oracle:
+- "fp64_fast": FP64 with FMA and the regrouped contraction
+  r_e = sum_q (w detJ P J^{-1})_q dN/dxi_q.  Equal to the reference order up
+  to rounding per evaluation; over many steps the difference grows where the
+  constitutive law cancels (the neo-Hookean S = mu (I - C^{-1}) + ... loses
+  about three digits at strains of 1e-3), so its bound is stated per law.
+- "fp32": grad u, stress and B^T sigma in FP32, FP64 nodal update with
+  compensated summation; the target is <= 1e-5 relative displacement error."""
+import numpy as np
+import pytest
+
+from tests.helpers import make_gpu, make_oracle, relerr, workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _errs(g, o):
+    return (relerr(g.getDisplacement(), o.u), relerr(g.getVelocity(), o.v),
+            relerr(g.getInternalForce(), o.f_int))
+
+
+@pytest.mark.parametrize("etype,law,steps,tol", [
+    ("_tetrahedron_4", "elastic", 1000, 1e-12),
+    ("_hexahedron_8", "elastic", 500, 1e-12),
+    ("_hexahedron_8", "neohookean", 500, 1e-10),
+])
+def test_fp64_fast(etype, law, steps, tol):
+    mesh, blocked, v0 = workload(etype, (12, 4, 4), jitter=0.1, seed=3)
+    o, dt = make_oracle(mesh, blocked, v0, law)
+    g, _ = make_gpu(mesh, blocked, v0, law, dt=dt, precision="fp64_fast")
+    o.solve_steps(steps)
+    g.solveStep(steps)
+    eu, ev, ef = _errs(g, o)
+    print(f"fp64_fast {etype} {law} {steps} steps: u {eu:.2e} v {ev:.2e} f {ef:.2e}")
+    assert max(eu, ev, ef) <= tol
+
+
+@pytest.mark.parametrize("etype,law,tol_u", [("_tetrahedron_4", "elastic", 1e-5),
+                                             ("_hexahedron_8", "neohookean", 2e-5)])
+def test_fp32_form_error_against_fp64_oracle(etype, law, tol_u):
+    """Config 3 in miniature (jittered Tet4, per-element E and nu, linear
+    elastic, 2000 steps): u within 1e-5 of the FP64 oracle.  The Hex8
+    neo-Hookean case (a serving-form request type) is held to its measured
+    2e-5.  Reports L2 and max relative error per field."""
+    mesh, blocked, v0 = workload(etype, (20, 4, 4), jitter=0.1, seed=11)
+    kw = {}
+    if etype == "_tetrahedron_4":
+        rng = np.random.default_rng(11)
+        kw = dict(E_el=np.exp(rng.uniform(np.log(0.5), np.log(2.0), mesh.nb_elements)),
+                  nu_el=rng.uniform(0.2, 0.35, mesh.nb_elements))
+    o, dt = make_oracle(mesh, blocked, v0, law, **kw)
+    g, _ = make_gpu(mesh, blocked, v0, law, dt=dt, precision="fp32", **kw)
+    o.solve_steps(2000)
+    g.solveStep(2000)
+    for name, a, b in (("u", g.getDisplacement(), o.u), ("v", g.getVelocity(), o.v),
+                       ("f_int", g.getInternalForce(), o.f_int)):
+        l2 = np.linalg.norm(a - b) / np.linalg.norm(b)
+        mx = relerr(a, b)
+        print(f"fp32 {etype} {law} {name}: L2 {l2:.2e} max {mx:.2e}")
+        if name == "u":
+            assert l2 <= tol_u and mx <= tol_u
+        else:
+            assert l2 <= 1e-4 and mx <= 1e-4
