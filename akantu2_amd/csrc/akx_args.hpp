@@ -20,7 +20,9 @@ struct ElemArgs {
   int64_t nn;
   double lambda, mu, sigma_y, h;
   const double *lam_el, *mu_el, *sy_el, *h_el; // per launch element, or null
-  void *fe;               // [slot][3] of FT
+  const int32_t *pos;     // [(pos_offset + i) * nnpe + a] -> entry of the
+                          // node-major elemental-force table
+  void *fe;               // [entry][3] of FT
   // materialised internals [9][n*nqp] (or null), column-major 3x3 index
   double *grad_u, *stress;
   // plastic history, [9][n*nqp] and [n*nqp]: previous (read) / current (write)

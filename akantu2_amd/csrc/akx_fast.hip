@@ -246,7 +246,8 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
         for (int k = 0; k < 3; ++k)
           r[k] += sQ[el][qq][3 * k + m] * d;
       }
-    FT *out = reinterpret_cast<FT *>(A.fe) + (A.pos_offset + i) * 24 + n * 3;
+    FT *out = reinterpret_cast<FT *>(A.fe) +
+              (int64_t)A.pos[(A.pos_offset + i) * 8 + n] * 3;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       out[k] = FT(r[k]);
@@ -311,13 +312,14 @@ __global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
         s += P[k][J2] * Ji[J2][m];
       Q[k][m] = wJ * s;
     }
-  FT *out = reinterpret_cast<FT *>(A.fe) + (A.pos_offset + i) * 12;
+  FT *fe = reinterpret_cast<FT *>(A.fe);
+  const int4 pv = *reinterpret_cast<const int4 *>(A.pos + (A.pos_offset + i) * 4);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    out[0 * 3 + k] = FT(-(Q[k][0] + Q[k][1] + Q[k][2]));
-    out[1 * 3 + k] = FT(Q[k][1]);
-    out[2 * 3 + k] = FT(Q[k][2]);
-    out[3 * 3 + k] = FT(Q[k][0]);
+    fe[(int64_t)pv.x * 3 + k] = FT(-(Q[k][0] + Q[k][1] + Q[k][2]));
+    fe[(int64_t)pv.y * 3 + k] = FT(Q[k][1]);
+    fe[(int64_t)pv.z * 3 + k] = FT(Q[k][2]);
+    fe[(int64_t)pv.w * 3 + k] = FT(Q[k][0]);
   }
 }
 

@@ -6,11 +6,14 @@
 //                                                components are equal)
 //   blocked DOFs     bits 0..2 of a byte       : [nn]
 //   connectivity                               : [nel][nnpe] int32 (16 B loads)
-//   elemental force  r_e                       : [slot][3], slot = p*nnpe + a
-//                    where p is the element's position in the assembly order
-//   gather table     rowptr [nn+1], slots [nel*nnpe] (node -> slot list in
-//                    ascending assembly order, i.e. the reference's
-//                    sequential scatter order of dof_manager.cc:92-114)
+//   elemental force  r_e of element position p, local node a, is stored at
+//                    entry pos[p*nnpe + a] of a node-major sliced-ELL table:
+//                    nodes are cut in slices of 256; entry (node, d) of
+//                    slice s is ell_off[s] + d*256 + (node % 256), where d is
+//                    the rank of the element in the node's list in ascending
+//                    assembly order (the reference's sequential scatter
+//                    order, dof_manager.cc:92-114).  The gather then reads
+//                    slot d of 256 consecutive nodes as one coalesced block.
 #pragma once
 
 #include "akx_args.hpp"
@@ -299,12 +302,18 @@ k_element_forces(ElemArgs A) {
     }
   }
 
-  FT *out = reinterpret_cast<FT *>(A.fe) + (A.pos_offset + i) * NN * 3;
+  FT *fe = reinterpret_cast<FT *>(A.fe);
+  const int4 *p4 = reinterpret_cast<const int4 *>(A.pos + (A.pos_offset + i) * NN);
 #pragma unroll
-  for (int n = 0; n < NN; ++n)
+  for (int kk = 0; kk < NN / 4; ++kk) {
+    const int4 pv = p4[kk];
+    const int32_t pn[4] = {pv.x, pv.y, pv.z, pv.w};
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      out[n * 3 + k] = FT(re[n][k]);
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        fe[(int64_t)pn[j] * 3 + k] = FT(re[4 * kk + j][k]);
+  }
 }
 
 
@@ -439,7 +448,8 @@ __global__ void __launch_bounds__(128) k_hex8_strict(ElemArgs A) {
   __syncthreads();
   if (active) {
     const int n = q; // this lane now owns node n of the element
-    double *out = reinterpret_cast<double *>(A.fe) + (A.pos_offset + i) * 24 + n * 3;
+    double *out = reinterpret_cast<double *>(A.fe) +
+                  (int64_t)A.pos[(A.pos_offset + i) * 8 + n] * 3;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       double s = sR[el][0][3 * n + k];
@@ -454,7 +464,8 @@ __global__ void __launch_bounds__(128) k_hex8_strict(ElemArgs A) {
 // ---------------------------------------------------------------------------
 // Nodal gather + residual + lumped solve + corrector (+ the next step's
 // predictor).  One thread per node.
-//   f_int  = sum over the node's slots, in assembly order, of -1 * r_e
+//   f_int  = sum over the node's entries d = 0..deg-1 (ascending assembly
+//            order) of -1 * r_e
 //   r      = (0 + f_ext) + f_int + M * (0 + -1 * a)
 //   x      = 1 * (r / M)                  (unblocked DOFs only)
 //   a     += 1 * x ; v += (beta dt) * x    (u += 0 * x is the identity)
@@ -465,9 +476,9 @@ __global__ void __launch_bounds__(128) k_hex8_strict(ElemArgs A) {
 // ---------------------------------------------------------------------------
 struct GatherArgs {
   int64_t nn;
-  const int32_t *rowptr; // [nn+1]
-  const int32_t *slots;  // [nel*nnpe]
-  const void *fe;        // [slot][3]
+  const int64_t *ell_off; // [nn/256 + 1] first entry of each 256-node slice
+  const uint8_t *deg;      // [nn] elements around each node
+  const void *fe;          // [entry][3]
   double *fint;          // [3][nn]
   const double *fext;    // [3][nn] or null (zero external force)
   const double *mass;    // [nn]
@@ -492,10 +503,9 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
     return;
   const FT *fe = reinterpret_cast<const FT *>(A.fe);
   double f0 = 0., f1 = 0., f2 = 0.;
-  const int32_t s0 = A.rowptr[n], s1 = A.rowptr[n + 1];
-  for (int32_t s = s0; s < s1; ++s) {
-    const int64_t slot = A.slots[s];
-    const FT *p = fe + slot * 3;
+  const int deg = A.deg[n];
+  const FT *p = fe + (A.ell_off[n >> 8] + (n & 255)) * 3;
+  for (int d = 0; d < deg; ++d, p += 256 * 3) {
     f0 = f0 + -1. * double(p[0]);
     f1 = f1 + -1. * double(p[1]);
     f2 = f2 + -1. * double(p[2]);

@@ -157,7 +157,8 @@ struct InitArgs {
   double rho;
   double lambda, mu;
   const double *lam_el, *mu_el;
-  double *me; // [slot] mass contributions
+  const int32_t *pos; // [slot] entry of the node-major table
+  double *me;         // [entry] mass contributions
   unsigned long long *min_bits;
   long long *first_bad;
 };
@@ -220,7 +221,7 @@ template <int ET> __global__ void k_mass_elements(InitArgs A) {
   }
 #pragma unroll
   for (int n = 0; n < NN; ++n)
-    A.me[(A.pos_offset + i) * NN + n] = m[n];
+    A.me[A.pos[(A.pos_offset + i) * NN + n]] = m[n];
 }
 
 // Inradius: Tet4 2 * Math::tetrahedron_inradius (aka_math_tmpl.hh:71-124),
@@ -311,16 +312,17 @@ template <int ET> __global__ void __launch_bounds__(256) k_stable_dt(InitArgs A)
   }
 }
 
-// Lumped mass gather: M = 0 + sum 1 * m_e over the node's slots in order
-__global__ void k_gather_mass(int64_t nn, const int32_t *rowptr,
-                              const int32_t *slots, const double *me,
+// Lumped mass gather: M = 0 + sum 1 * m_e over the node's entries in order
+__global__ void k_gather_mass(int64_t nn, const int64_t *ell_off,
+                              const uint8_t *deg, const double *me,
                               double *mass) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn)
     return;
   double m = 0.;
-  for (int32_t s = rowptr[n]; s < rowptr[n + 1]; ++s)
-    m += 1. * me[slots[s]];
+  const double *p = me + ell_off[n >> 8] + (n & 255);
+  for (int d = 0; d < deg[n]; ++d)
+    m += 1. * p[d * 256];
   mass[n] = m;
 }
 
@@ -362,8 +364,11 @@ struct akx_model {
   DevBuf<double> X, u, v, a, fint, fext, mass;
   DevBuf<uint8_t> blocked;
   DevBuf<float> cu, cv;
-  DevBuf<int32_t> conn, rowptr, slots;
-  DevBuf<unsigned char> fe; // [slots][3] of double or float
+  DevBuf<int32_t> conn, pos;
+  DevBuf<int64_t> ell_off; // [nn/256 + 1]
+  DevBuf<uint8_t> deg;     // [nn]
+  int64_t ell_total = 0;   // entries of the node-major table
+  DevBuf<unsigned char> fe; // [entries][3] of double or float
   DevBuf<unsigned char> staging;
   DevBuf<unsigned long long> scratch_u64;
   DevBuf<long long> scratch_i64;
@@ -553,6 +558,7 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
   A.mu_el = mat.per_element ? mat.mu_el.p : nullptr;
   A.sy_el = mat.sy_el.p;
   A.h_el = mat.h_el.p;
+  A.pos = m->pos.p;
   A.fe = m->fe.p;
   if (materialize && mat.law != AKX_LAW_PLASTIC_LIH) {
     A.grad_u = mat.grad.p;
@@ -576,7 +582,7 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
 // Host-side checks that every buffer a kernel variant dereferences exists
 // (a kernel must never be launched with a null pointer it will use).
 void check_gather_args(const GatherArgs &G, bool update, bool comp) {
-  require(G.rowptr && G.slots && G.fe && G.fint, "gather launched without its table");
+  require(G.ell_off && G.deg && G.fe && G.fint, "gather launched without its table");
   if (update)
     require(G.mass && G.blocked && G.u && G.v && G.a,
             "nodal update launched without nodal state");
@@ -585,7 +591,8 @@ void check_gather_args(const GatherArgs &G, bool update, bool comp) {
 }
 
 void check_elem_args(const ElemArgs &A, int law) {
-  require(A.conn && A.X && A.u && A.fe, "element pass launched without mesh or state");
+  require(A.conn && A.X && A.u && A.fe && A.pos,
+          "element pass launched without mesh or state");
   require(A.n <= 0 || A.nblocks_padded > 0, "element pass launched with an empty grid");
   if (law == AKX_LAW_PLASTIC_LIH)
     require(A.p_grad && A.p_sig && A.p_eps && A.p_iso && A.c_grad && A.c_sig &&
@@ -652,9 +659,9 @@ void launch_element_pass(akx_model *m, bool materialize) {
       continue;
     ElemArgs A = elem_args(m, mat, materialize);
     check_elem_args(A, mat.law);
-    require(m->fe.n >= (size_t)((mat.pos_offset + mat.n) * m->nnpe * 3 *
-                                m->fe_elem_size()),
-            "element force buffer too small for this material");
+    require(m->fe.n >= (size_t)(m->ell_total * 3 * m->fe_elem_size()) &&
+                m->pos.n >= (size_t)((mat.pos_offset + mat.n) * m->nnpe),
+            "element force table too small for this material");
     if (strict && m->etype == AKX_HEX8) {
       int64_t nb = (mat.n + HEX_STRICT_EPB - 1) / HEX_STRICT_EPB;
       A.nblocks_padded = (nb + 7) / 8 * 8;
@@ -687,8 +694,8 @@ void flip_plastic_state(akx_model *m) {
 GatherArgs gather_args(akx_model *m) {
   GatherArgs G{};
   G.nn = m->nn;
-  G.rowptr = m->rowptr.p;
-  G.slots = m->slots.p;
+  G.ell_off = m->ell_off.p;
+  G.deg = m->deg.p;
   G.fe = m->fe.p;
   G.fint = m->fint.p;
   G.fext = m->has_fext ? m->fext.p : nullptr;
@@ -775,43 +782,60 @@ void check_ready(akx_model *m) {
   require(m->initialized, "initFull has not been called");
 }
 
-// Gather table: for each node the slots (position * nnpe + local node) in
-// ascending assembly order -- materials in creation order, each material's
-// filter in order -- which is the order of the reference's sequential
-// scatter (dof_manager.cc:92-114 called per material by
-// material.cc:98-120).  MeshUtils::buildNode2Elements (mesh_utils.cc:37-82)
-// builds the same node -> element CSR.
+// Gather table.  Every (element position p, local node a) gets an entry of a
+// node-major sliced-ELL table: nodes are cut in slices of 256, slice s has
+// width w_s = its largest node degree and starts at ell_off[s]; the entry of
+// the d-th element of node n is ell_off[s] + d*256 + n%256.  Elements are
+// ranked per node in ascending assembly order -- materials in creation
+// order, each material's filter in order -- which is the order of the
+// reference's sequential scatter (dof_manager.cc:92-114, called per material
+// by material.cc:98-120).  MeshUtils::buildNode2Elements
+// (mesh_utils.cc:37-82) builds the same node -> element lists.
 void build_gather_table(akx_model *m) {
   int64_t nslots = 0;
   for (auto &mp : m->mats)
     nslots += mp->n * m->nnpe;
   require(nslots < (int64_t)INT32_MAX, "too many element slots for int32 indices");
-  std::vector<int32_t> rowptr(m->nn + 1, 0), slots(nslots);
+  std::vector<int32_t> deg(m->nn, 0);
   for (auto &mp : m->mats)
     for (int64_t i = 0; i < mp->n; ++i) {
       int64_t e = mp->contiguous ? mp->first + i : mp->filter_host[i];
       for (int a = 0; a < m->nnpe; ++a)
-        rowptr[m->conn_host[e * m->nnpe + a] + 1]++;
+        deg[m->conn_host[e * m->nnpe + a]]++;
     }
-  for (int64_t n = 0; n < m->nn; ++n)
-    rowptr[n + 1] += rowptr[n];
-  std::vector<int32_t> fill(rowptr.begin(), rowptr.end() - 1);
+  const int64_t nslices = (m->nn + 255) / 256;
+  std::vector<int64_t> off(nslices + 1, 0);
+  for (int64_t sl = 0; sl < nslices; ++sl) {
+    int w = 0;
+    for (int64_t n = sl * 256; n < std::min<int64_t>(m->nn, sl * 256 + 256); ++n)
+      w = std::max(w, deg[n]);
+    require(w <= 255, "a node belongs to more than 255 elements");
+    off[sl + 1] = off[sl] + (int64_t)w * 256;
+  }
+  m->ell_total = off[nslices];
+  require(m->ell_total < (int64_t)INT32_MAX, "element force table too large for int32");
+  std::vector<int32_t> fill(m->nn, 0), pos(std::max<int64_t>(nslots, 1));
   for (auto &mp : m->mats)
     for (int64_t i = 0; i < mp->n; ++i) {
       int64_t e = mp->contiguous ? mp->first + i : mp->filter_host[i];
-      int64_t pos = mp->pos_offset + i;
+      int64_t p = mp->pos_offset + i;
       for (int a = 0; a < m->nnpe; ++a) {
         int32_t node = m->conn_host[e * m->nnpe + a];
-        slots[fill[node]++] = (int32_t)(pos * m->nnpe + a);
+        int32_t d = fill[node]++;
+        pos[p * m->nnpe + a] = (int32_t)(off[node >> 8] + (int64_t)d * 256 + (node & 255));
       }
     }
-  m->rowptr.alloc(m->nn + 1);
-  m->slots.alloc(std::max<int64_t>(nslots, 1));
-  HIPCHECK(hipMemcpy(m->rowptr.p, rowptr.data(), rowptr.size() * sizeof(int32_t),
+  std::vector<uint8_t> deg8(m->nn);
+  for (int64_t n = 0; n < m->nn; ++n)
+    deg8[n] = (uint8_t)deg[n];
+  m->pos.alloc(pos.size());
+  m->ell_off.alloc(off.size());
+  m->deg.alloc(m->nn);
+  HIPCHECK(hipMemcpy(m->pos.p, pos.data(), pos.size() * sizeof(int32_t),
                      hipMemcpyHostToDevice));
-  if (nslots)
-    HIPCHECK(hipMemcpy(m->slots.p, slots.data(), nslots * sizeof(int32_t),
-                       hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(m->ell_off.p, off.data(), off.size() * sizeof(int64_t),
+                     hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(m->deg.p, deg8.data(), m->nn, hipMemcpyHostToDevice));
 }
 
 void alloc_internals(akx_model *m, Material &mat) {
@@ -1100,16 +1124,17 @@ int akx_model_init_full(akx_model *m) {
       }
     }
     build_gather_table(m);
-    m->fe.alloc(std::max<int64_t>(nslots, 1) * 3 * m->fe_elem_size());
+    m->fe.alloc(std::max<int64_t>(m->ell_total, 1) * 3 * m->fe_elem_size());
     // lumped mass
     {
       DevBuf<double> me;
-      me.alloc(std::max<int64_t>(nslots, 1));
+      me.alloc(std::max<int64_t>(m->ell_total, 1));
       for (auto &mp : m->mats) {
         if (mp->n == 0)
           continue;
         InitArgs A = init_args(m, mp.get());
         A.me = me.p;
+        A.pos = m->pos.p;
         if (m->etype == AKX_TET4)
           hipLaunchKernelGGL(k_mass_elements<ET_TET4>, dim3(grid_for(mp->n)),
                              dim3(TPB), 0, m->stream, A);
@@ -1119,8 +1144,8 @@ int akx_model_init_full(akx_model *m) {
         HIPCHECK(hipGetLastError());
       }
       hipLaunchKernelGGL(k_gather_mass, dim3(grid_for(m->nn)), dim3(TPB), 0,
-                         m->stream, m->nn, (const int32_t *)m->rowptr.p,
-                         (const int32_t *)m->slots.p, (const double *)me.p,
+                         m->stream, m->nn, (const int64_t *)m->ell_off.p,
+                         (const uint8_t *)m->deg.p, (const double *)me.p,
                          m->mass.p);
       HIPCHECK(hipGetLastError());
       HIPCHECK(hipStreamSynchronize(m->stream));
@@ -1422,18 +1447,19 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
     const double deg = (double)nel_all * m->nnpe / (double)m->nn;
     (void)deg;
     switch (kernel_class) {
-    case 0: // element pass: connectivity, X and u once per node, r_e write
-      *bytes = nel_all * (4.0 * m->nnpe + 3.0 * s_f * m->nnpe) +
+    case 0: // element pass: connectivity + table entries, X and u once per
+            // node, r_e write
+      *bytes = nel_all * (8.0 * m->nnpe + 3.0 * s_f * m->nnpe) +
                m->nn * (3.0 * s_node + 3.0 * s_node) + extra;
       break;
-    case 1: { // gather + update: rowptr, slots, r_e read, f_int write,
+    case 1: { // gather + update: degree, r_e read, f_int write,
               // mass, blocked, a/v read+write, u read+write (fused predictor)
       double per_node = 4.0 + 3 * s_node /*f_int*/ + s_node /*M*/ + 1.0 +
                         2 * 3 * s_node /*a*/ + 2 * 3 * s_node /*v*/ +
                         2 * 3 * s_node /*u*/ + (m->has_fext ? 3 * s_node : 0.);
       if (m->precision == AKX_PRECISION_FP32)
         per_node += 2 * 2 * 3 * 4.0; // compensation terms read + write
-      *bytes = nel_all * m->nnpe * (4.0 + 3.0 * s_f) + m->nn * per_node;
+      *bytes = nel_all * m->nnpe * (3.0 * s_f) + m->nn * (per_node - 3.0);
       break;
     }
     case 2:
