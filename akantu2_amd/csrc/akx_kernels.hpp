@@ -319,22 +319,22 @@ k_element_forces(ElemArgs A) {
 
 // ---------------------------------------------------------------------------
 // Hex8 element pass in the reference's operation order, one lane per Gauss
-// point (8 lanes per element, 16 elements per 128-thread block).  Nodal
+// point (8 lanes per element, 8 elements per 64-lane block).  Nodal
 // coordinates and displacements are staged in LDS; each lane forms J, its
 // LU inverse, the dN/dX columns, grad u, the law and the 24 force terms of
 // its Gauss point times w*detJ; the per-node sums over the Gauss points are
 // then taken in ascending Gauss-point order (the oracle's and Eigen's GEMV
-// order) by the lane that owns the node.
+// order), four nodes at a time, by the lanes that own the nodes.
 // ---------------------------------------------------------------------------
-constexpr int HEX_STRICT_EPB = 16;
+constexpr int HEX_STRICT_EPB = 8;
 
 template <int LAW>
-__global__ void __launch_bounds__(128) k_hex8_strict(ElemArgs A) {
+__global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
   __shared__ double sN[HEX_STRICT_EPB][8][6];
-  __shared__ double sR[HEX_STRICT_EPB][8][25]; // 24 terms (+1 pad: banks)
+  __shared__ double sR[HEX_STRICT_EPB][8][13]; // 12 terms (+1 pad: banks)
   __shared__ double sD[8][3][8];
   const int t = threadIdx.x, el = t >> 3, q = t & 7;
-  for (int k = t; k < 192; k += 128)
+  for (int k = t; k < 192; k += 64)
     (&sD[0][0][0])[k] = (&c_hex8.dnds[0][0][0])[k];
   const int64_t i = xcd_block(A.nblocks_padded) * HEX_STRICT_EPB + el;
   const bool active = i < A.n;
@@ -348,9 +348,10 @@ __global__ void __launch_bounds__(128) k_hex8_strict(ElemArgs A) {
     }
   }
   __syncthreads();
+  double Ji[3][3], g[3][3], S[3][3], wJ = 0.;
   if (active) {
     // J(i, j) = sum_n dnds(i, n) X(n, j)
-    double J[3][3], Ji[3][3];
+    double J[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -362,8 +363,10 @@ __global__ void __launch_bounds__(128) k_hex8_strict(ElemArgs A) {
         J[a][j] = s;
       }
     double detJ = jacobian_det_inv(J, Ji);
-    const double wJ = detJ * c_hex8.w[q];
-    auto bcol = [&](int n, double (&b)[3]) {
+    wJ = detJ * c_hex8.w[q];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      double b[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         double s = Ji[a][0] * sD[q][0][n];
@@ -371,12 +374,6 @@ __global__ void __launch_bounds__(128) k_hex8_strict(ElemArgs A) {
         s += Ji[a][2] * sD[q][2][n];
         b[a] = s;
       }
-    };
-    double g[3][3];
-#pragma unroll
-    for (int n = 0; n < 8; ++n) {
-      double b[3];
-      bcol(n, b);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         const double un = sN[el][n][3 + a];
@@ -388,7 +385,6 @@ __global__ void __launch_bounds__(128) k_hex8_strict(ElemArgs A) {
     double lambda = A.lam_el ? A.lam_el[i] : A.lambda;
     double mu = A.mu_el ? A.mu_el[i] : A.mu;
     const int64_t iq = i * 8 + q, nq_all = A.n * 8;
-    double S[3][3];
     if constexpr (LAW == LAW_ELASTIC) {
       elastic_stress(lambda, mu, g, -0.0, S);
     } else if constexpr (LAW == LAW_ELASTIC_FD) {
@@ -432,32 +428,47 @@ __global__ void __launch_bounds__(128) k_hex8_strict(ElemArgs A) {
           A.stress[o] = S[ii][jj];
         }
     }
-#pragma unroll
-    for (int n = 0; n < 8; ++n) {
-      double b[3], r[3];
-      bcol(n, b);
-      if constexpr (LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN)
-        btd_finite(g, S, b, r);
-      else
-        btd_small(S, b, r);
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        sR[el][q][3 * n + k] = r[k] * wJ;
-    }
   }
-  __syncthreads();
-  if (active) {
-    const int n = q; // this lane now owns node n of the element
-    double *out = reinterpret_cast<double *>(A.fe) +
-                  (int64_t)A.pos[(A.pos_offset + i) * 8 + n] * 3;
+  // two halves of four nodes: the force terms of this Gauss point go to LDS,
+  // then the lanes q < 4 sum node 4h + q over the Gauss points in order
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double s = sR[el][0][3 * n + k];
+  for (int hh = 0; hh < 2; ++hh) {
+    if (active) {
 #pragma unroll
-      for (int qq = 1; qq < 8; ++qq)
-        s += sR[el][qq][3 * n + k];
-      out[k] = s;
+      for (int nn = 0; nn < 4; ++nn) {
+        const int n = 4 * hh + nn;
+        double b[3], r[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          double s = Ji[a][0] * sD[q][0][n];
+          s += Ji[a][1] * sD[q][1][n];
+          s += Ji[a][2] * sD[q][2][n];
+          b[a] = s;
+        }
+        if constexpr (LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN)
+          btd_finite(g, S, b, r);
+        else
+          btd_small(S, b, r);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          sR[el][q][3 * nn + k] = r[k] * wJ;
+      }
     }
+    __syncthreads();
+    if (active && q < 4) {
+      const int n = 4 * hh + q;
+      double *out = reinterpret_cast<double *>(A.fe) +
+                    (int64_t)A.pos[(A.pos_offset + i) * 8 + n] * 3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double s = sR[el][0][3 * q + k];
+#pragma unroll
+        for (int qq = 1; qq < 8; ++qq)
+          s += sR[el][qq][3 * q + k];
+        out[k] = s;
+      }
+    }
+    __syncthreads();
   }
 }
 

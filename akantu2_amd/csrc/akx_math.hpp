@@ -73,10 +73,11 @@ AKX_HD void jacobian_lu(const R (&A)[3][3], R (&lu)[3][3], int (&perm)[3],
     }
     perm[k] = p;
     if (big != R(0)) {
+      if (p != k) { // most elements never pivot: skip the selects
 #pragma unroll
-      for (int i = k + 1; i < 3; ++i) {
-        swap_rows_if(lu, k, i, p == i);
-        nb_transp += (p == i) ? 1 : 0;
+        for (int i = k + 1; i < 3; ++i)
+          swap_rows_if(lu, k, i, p == i);
+        nb_transp += 1;
       }
 #pragma unroll
       for (int i = k + 1; i < 3; ++i)
@@ -100,9 +101,11 @@ AKX_HD R jacobian_det_inv(const R (&J)[3][3], R (&Ji)[3][3]) {
   R x[3][3] = {{R(1), R(0), R(0)}, {R(0), R(1), R(0)}, {R(0), R(0), R(1)}};
 #pragma unroll
   for (int k = 0; k < 3; ++k)
+    if (perm[k] != k) {
 #pragma unroll
-    for (int i = k + 1; i < 3; ++i)
-      swap_rows_if(x, k, i, perm[k] == i);
+      for (int i = k + 1; i < 3; ++i)
+        swap_rows_if(x, k, i, perm[k] == i);
+    }
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
 #pragma unroll

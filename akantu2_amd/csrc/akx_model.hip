@@ -631,7 +631,7 @@ void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
 }
 
 void launch_hex8_strict(akx_model *m, Material &mat, const ElemArgs &A) {
-  dim3 grid((unsigned)A.nblocks_padded), block(128);
+  dim3 grid((unsigned)A.nblocks_padded), block(64);
   switch (mat.law) {
   case AKX_LAW_ELASTIC:
     hipLaunchKernelGGL(k_hex8_strict<LAW_ELASTIC>, grid, block, 0, m->stream, A);
