@@ -71,6 +71,7 @@ def _declare(lib):
         "akx_model_enable_kernel_timing": (C.c_int, [_vp, C.c_int]),
         "akx_model_get_kernel_timing": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(_i64)]),
         "akx_model_get_kernel_bytes": (C.c_int, [_vp, C.c_int, _dp]),
+        "akx_calibration_copy": (C.c_int, [C.c_int, _i64, C.c_int, _dp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -98,6 +99,14 @@ def check(status):
         if status == -3:
             raise JacobianError(status, msg)
         raise AkantuError(status, msg)
+
+
+def calibration_copy(nbytes: int, repeats: int = 10, device: int = 0) -> float:
+    """Average ms of a streaming device copy of nbytes (read nbytes, write
+    nbytes)."""
+    ms = C.c_double()
+    check(lib().akx_calibration_copy(device, int(nbytes), int(repeats), C.byref(ms)))
+    return ms.value
 
 
 def device_count() -> int:

@@ -52,5 +52,18 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+CPP_DRIVER_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "drive_explicit.cc")
+CPP_DRIVER = os.path.join(os.path.dirname(HERE), "tests", "cpp", "drive_explicit")
+
+
+def build_cpp_driver() -> str:
+    """The C++ API test driver (tests/cpp/drive_explicit.cc), linked against
+    the in-tree library with an rpath so it runs from any checkout."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    _run(["g++", "-O2", "-std=c++17", f"-I{inc}", "-o", CPP_DRIVER, CPP_DRIVER_SRC,
+          f"-L{HERE}", "-lakantu_mi355x", "-Wl,-rpath,$ORIGIN/../../akantu2_amd"])
+    return CPP_DRIVER
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))

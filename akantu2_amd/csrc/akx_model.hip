@@ -124,6 +124,16 @@ __global__ void k_mass_to3(const double *m, double *m3, int64_t n) {
   m3[3 * i + 2] = m[i];
 }
 
+// streaming copy used to calibrate the HBM byte counters and to report the
+// measured copy bandwidth next to the 8 TB/s peak
+__global__ void __launch_bounds__(256) k_copy_calibration(const double4 *src,
+                                                          double4 *dst,
+                                                          int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    dst[i] = src[i];
+}
+
 __global__ void k_fill_f32(float *p, int64_t n, float v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n)
@@ -1388,6 +1398,38 @@ int akx_model_get_internal(akx_model *m, int material_id, int internal,
     if (nq == 0)
       return;
     download_soa(m, src, host, nq, comps);
+  });
+}
+
+int akx_calibration_copy(int device, int64_t bytes, int repeats, double *ms) {
+  return guard([&] {
+    require(bytes >= 32 && repeats > 0 && ms, "bad calibration arguments");
+    HIPCHECK(hipSetDevice(device));
+    const int64_t n = bytes / 32;
+    DevBuf<double4> a, b;
+    a.alloc(n);
+    b.alloc(n);
+    hipStream_t st;
+    HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    require((n + 255) / 256 < (int64_t)UINT32_MAX, "calibration copy too large");
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_copy_calibration, dim3(grid), dim3(256), 0, st,
+                       (const double4 *)a.p, b.p, n); // warm-up
+    HIPCHECK(hipEventRecord(e0, st));
+    for (int r = 0; r < repeats; ++r)
+      hipLaunchKernelGGL(k_copy_calibration, dim3(grid), dim3(256), 0, st,
+                         (const double4 *)a.p, b.p, n);
+    HIPCHECK(hipEventRecord(e1, st));
+    HIPCHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+    *ms = (double)t / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
   });
 }
 

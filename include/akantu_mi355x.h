@@ -162,6 +162,10 @@ int akx_model_get_kernel_timing(akx_model *m, int kernel_class,
                                 double *total_ms, int64_t *launches);
 /* Bytes each kernel class moves per launch in the unique-bytes model. */
 int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes);
+/* Streaming device copy of `bytes` (read + write), repeated; average time
+ * per copy in ms.  No counterpart in the reference: it calibrates the HBM
+ * byte counters and reports the measured copy bandwidth. */
+int akx_calibration_copy(int device, int64_t bytes, int repeats, double *ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,230 @@
+// akantu_mi355x.hpp -- C++17 host side above the C ABI (akantu_mi355x.h),
+// shaped like the reference's SolidMechanicsModel / Material API so that
+// driver code written against Akantu reads the same:
+//
+//   akantu_mi355x::SolidMechanicsModel model(mesh);
+//   model.addMaterial("neohookean", {1.0, 1.0, 0.3});
+//   model.initFull(akantu_mi355x::_explicit_lumped_mass);
+//   model.setTimeStep(0.5 * model.getStableTimeStep());
+//   for (int s = 0; s < 1000; ++s) model.solveStep();
+//   const auto & u = model.getDisplacement();   // [nb_nodes][3]
+//
+// Arrays are host mirrors of the device fields in the reference's layout
+// (Array<Real>: one tuple of nb_component values per row).  Mutable getters
+// mark the field dirty; the next solveStep uploads it (the reference lets
+// users write through getDisplacement()).  Errors throw akantu_mi355x::Exception
+// (the reference throws debug::Exception; a negative jacobian throws
+// JacobianException like IntegratorGaussException).
+#pragma once
+
+#include "akantu_mi355x.h"
+
+#include <array>
+#include <cstdint>
+#include <deque>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace akantu_mi355x {
+
+struct Exception : std::runtime_error {
+  int code;
+  Exception(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+struct JacobianException : Exception {
+  using Exception::Exception;
+};
+
+inline void check(int status) {
+  if (status == AKX_OK)
+    return;
+  std::string msg = akx_last_error();
+  if (status == AKX_ERR_JACOBIAN)
+    throw JacobianException(status, msg);
+  throw Exception(status, msg);
+}
+
+enum AnalysisMethod { _explicit_lumped_mass };
+enum ElementType { _tetrahedron_4 = AKX_TET4, _hexahedron_8 = AKX_HEX8 };
+
+// Minimal Array<T>: contiguous rows of nb_component values (aka_array.hh).
+template <class T> class Array {
+public:
+  Array() = default;
+  Array(std::int64_t size, int nb_component, T value = T())
+      : nb_comp(nb_component), data_(size * nb_component, value) {}
+  std::int64_t size() const { return (std::int64_t)data_.size() / nb_comp; }
+  int getNbComponent() const { return nb_comp; }
+  T &operator()(std::int64_t i, int c = 0) { return data_[i * nb_comp + c]; }
+  const T &operator()(std::int64_t i, int c = 0) const { return data_[i * nb_comp + c]; }
+  T *data() { return data_.data(); }
+  const T *data() const { return data_.data(); }
+
+private:
+  int nb_comp = 1;
+  std::vector<T> data_;
+};
+
+// Mesh given as arrays (Mesh::getNodes / Mesh::getConnectivity).
+struct Mesh {
+  ElementType type;
+  Array<double> nodes;          // [nb_nodes][3]
+  Array<std::int32_t> connectivity; // [nb_elements][nnpe]
+};
+
+struct MaterialParameters {
+  double rho, E, nu, sigma_y = 0., h = 0.;
+};
+
+class SolidMechanicsModel;
+
+// Handle on one material (Material's accessors).
+class Material {
+public:
+  Material(SolidMechanicsModel &m, int id, std::int64_t n) : model(&m), id(id), n(n) {}
+  int getID() const { return id; }
+  // Material::getGradU / getStress (or PK2 for finite deformation)
+  Array<double> getGradU() const;
+  Array<double> getStress() const;
+
+private:
+  SolidMechanicsModel *model;
+  int id;
+  std::int64_t n;
+};
+
+class SolidMechanicsModel {
+public:
+  explicit SolidMechanicsModel(const Mesh &mesh, int device = 0)
+      : nb_nodes(mesh.nodes.size()),
+        nqp(mesh.type == _hexahedron_8 ? 8 : 1),
+        nb_elements(mesh.connectivity.size()) {
+    check(akx_model_create(device, &h));
+    check(akx_model_set_mesh(h, mesh.type, nb_nodes, mesh.nodes.data(), nb_elements,
+                             mesh.connectivity.data()));
+    for (auto &f : fields)
+      f = Array<double>(nb_nodes, 3);
+    blocked = Array<std::uint8_t>(nb_nodes, 3);
+  }
+  ~SolidMechanicsModel() { akx_model_destroy(h); }
+  SolidMechanicsModel(const SolidMechanicsModel &) = delete;
+  SolidMechanicsModel &operator=(const SolidMechanicsModel &) = delete;
+
+  // material <name> [ rho = .. E = .. nu = .. ] ; finite_deformation selects
+  // the total-Lagrangian branch of "elastic"
+  Material &addMaterial(const std::string &name, const MaterialParameters &p,
+                        bool finite_deformation = false,
+                        const std::vector<std::int32_t> &filter = {}) {
+    int law = 0, id = -1;
+    check(akx_material_law_by_name(name.c_str(), finite_deformation, &law));
+    double params[5] = {p.rho, p.E, p.nu, p.sigma_y, p.h};
+    check(akx_model_add_material(h, law, params, 5,
+                                 filter.empty() ? nullptr : filter.data(),
+                                 (std::int64_t)filter.size(), &id));
+    materials.emplace_back(*this, id,
+                           filter.empty() ? nb_elements : (std::int64_t)filter.size());
+    return materials.back();
+  }
+  Material &getMaterial(int i) { return materials.at(i); }
+
+  void initFull(AnalysisMethod = _explicit_lumped_mass) {
+    check(akx_model_init_full(h));
+    pull_all();
+  }
+  double getStableTimeStep() {
+    push();
+    double dt = 0.;
+    check(akx_model_get_stable_time_step(h, &dt));
+    return dt;
+  }
+  void setTimeStep(double dt) { check(akx_model_set_time_step(h, dt)); }
+  // ModelSolver::solveStep (nb_steps consecutive calls)
+  void solveStep(std::int64_t nb_steps = 1) {
+    push();
+    check(akx_model_solve_step(h, nb_steps));
+    stale = true;
+  }
+  void assembleInternalForces() {
+    push();
+    check(akx_model_assemble_internal_forces(h));
+    stale = true;
+  }
+
+  // nodal Arrays (solid_mechanics_model.hh:440-468)
+  Array<double> &getDisplacement() { return field(AKX_FIELD_DISPLACEMENT, true); }
+  Array<double> &getVelocity() { return field(AKX_FIELD_VELOCITY, true); }
+  Array<double> &getAcceleration() { return field(AKX_FIELD_ACCELERATION, true); }
+  Array<double> &getExternalForce() { return field(AKX_FIELD_EXTERNAL_FORCE, true); }
+  const Array<double> &getInternalForce() { return field(AKX_FIELD_INTERNAL_FORCE, false); }
+  const Array<double> &getMass() { return field(AKX_FIELD_MASS, false); }
+  Array<std::uint8_t> &getBlockedDOFs() {
+    refresh();
+    dirty_blocked = true;
+    return blocked;
+  }
+
+  akx_model *handle() { return h; }
+  int nbQuadraturePoints() const { return nqp; }
+
+private:
+  friend class Material;
+  static constexpr int NF = 6;
+  Array<double> &field(int f, bool writable) {
+    refresh();
+    if (writable)
+      dirty[f] = true;
+    return fields[f];
+  }
+  void refresh() {
+    if (stale)
+      pull_all();
+  }
+  void pull_all() {
+    for (int f = 0; f < NF; ++f)
+      check(akx_model_get_field(h, f, fields[f].data()));
+    check(akx_model_get_field(h, AKX_FIELD_BLOCKED_DOFS, blocked.data()));
+    stale = false;
+  }
+  void push() {
+    for (int f : {AKX_FIELD_DISPLACEMENT, AKX_FIELD_VELOCITY, AKX_FIELD_ACCELERATION,
+                  AKX_FIELD_EXTERNAL_FORCE})
+      if (dirty[f]) {
+        check(akx_model_set_field(h, f, fields[f].data()));
+        dirty[f] = false;
+      }
+    if (dirty_blocked) {
+      check(akx_model_set_field(h, AKX_FIELD_BLOCKED_DOFS, blocked.data()));
+      dirty_blocked = false;
+    }
+  }
+
+  akx_model *h = nullptr;
+  std::int64_t nb_nodes;
+  int nqp;
+  std::int64_t nb_elements;
+  std::array<Array<double>, NF> fields;
+  std::array<bool, NF> dirty{};
+  Array<std::uint8_t> blocked;
+  bool dirty_blocked = false;
+  bool stale = true;
+  std::deque<Material> materials; // stable references
+};
+
+inline Array<double> Material::getGradU() const {
+  model->push();
+  check(akx_model_compute_all_stresses(model->h));
+  Array<double> out(n * model->nqp, 9);
+  check(akx_model_get_internal(model->h, id, AKX_INTERNAL_GRAD_U, out.data()));
+  return out;
+}
+
+inline Array<double> Material::getStress() const {
+  model->push();
+  check(akx_model_compute_all_stresses(model->h));
+  Array<double> out(n * model->nqp, 9);
+  check(akx_model_get_internal(model->h, id, AKX_INTERNAL_STRESS, out.data()));
+  return out;
+}
+
+} // namespace akantu_mi355x

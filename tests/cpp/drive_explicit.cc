@@ -1,0 +1,91 @@
+// Drives the explicit path through the C++ host API (include/akantu_mi355x.hpp)
+// the way an Akantu driver does (examples/c++/solid_mechanics_model/explicit/
+// explicit_dynamic.cc): build the model, initFull, stable time step, solveStep
+// loop, read the nodal arrays back.  Inputs and outputs are raw binary files
+// written and read by tests/test_gpu_cpp_api.py.
+//
+//   drive_explicit <dir> <etype 0|1> <law name> <finite_deformation 0|1> <steps>
+//
+// <dir>/nodes.f64 [nn][3], conn.i32 [nel][nnpe], blocked.u8 [nn][3],
+// v0.f64 [nn][3], params.f64 {rho, E, nu, sigma_y, h, dt_factor}
+// -> <dir>/u.f64, v.f64, fint.f64, stress.f64, dt.f64
+#include "akantu_mi355x.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+using namespace akantu_mi355x;
+
+template <class T> static std::vector<T> load(const std::string &path) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f)
+    throw std::runtime_error("cannot open " + path);
+  std::streamsize n = f.tellg();
+  f.seekg(0);
+  std::vector<T> v(n / sizeof(T));
+  f.read(reinterpret_cast<char *>(v.data()), n);
+  return v;
+}
+
+template <class T> static void save(const std::string &path, const T *p, size_t n) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char *>(p), n * sizeof(T));
+}
+
+int main(int argc, char **argv) {
+  if (argc != 6) {
+    std::cerr << "usage: drive_explicit <dir> <etype> <law> <finite_deformation> <steps>\n";
+    return 2;
+  }
+  const std::string dir = argv[1];
+  const int etype = std::atoi(argv[2]);
+  const std::string law = argv[3];
+  const bool fd = std::atoi(argv[4]) != 0;
+  const long steps = std::atol(argv[5]);
+  try {
+    const int nnpe = etype == 1 ? 8 : 4;
+    auto X = load<double>(dir + "/nodes.f64");
+    auto C = load<std::int32_t>(dir + "/conn.i32");
+    auto B = load<std::uint8_t>(dir + "/blocked.u8");
+    auto V = load<double>(dir + "/v0.f64");
+    auto P = load<double>(dir + "/params.f64");
+    const std::int64_t nn = X.size() / 3, nel = C.size() / nnpe;
+
+    Mesh mesh{etype == 1 ? _hexahedron_8 : _tetrahedron_4, Array<double>(nn, 3),
+              Array<std::int32_t>(nel, nnpe)};
+    std::copy(X.begin(), X.end(), mesh.nodes.data());
+    std::copy(C.begin(), C.end(), mesh.connectivity.data());
+
+    SolidMechanicsModel model(mesh);
+    model.addMaterial(law, {P[0], P[1], P[2], P[3], P[4]}, fd);
+    model.initFull(_explicit_lumped_mass);
+    auto &blocked = model.getBlockedDOFs();
+    auto &velocity = model.getVelocity();
+    for (std::int64_t i = 0; i < nn; ++i)
+      for (int c = 0; c < 3; ++c) {
+        blocked(i, c) = B[3 * i + c];
+        velocity(i, c) = V[3 * i + c];
+      }
+    const double dt = P[5] * model.getStableTimeStep();
+    model.setTimeStep(dt);
+    for (long s = 0; s < steps; ++s)
+      model.solveStep();
+    save(dir + "/u.f64", model.getDisplacement().data(), 3 * nn);
+    save(dir + "/v.f64", model.getVelocity().data(), 3 * nn);
+    save(dir + "/fint.f64", model.getInternalForce().data(), 3 * nn);
+    auto stress = model.getMaterial(0).getStress();
+    save(dir + "/stress.f64", stress.data(), (size_t)stress.size() * 9);
+    save(dir + "/dt.f64", &dt, 1);
+  } catch (const JacobianException &e) {
+    std::cerr << "jacobian: " << e.what() << "\n";
+    return 3;
+  } catch (const std::exception &e) {
+    std::cerr << "error: " << e.what() << "\n";
+    return 1;
+  }
+  return 0;
+}

@@ -65,3 +65,14 @@ def test_product_does_not_import_oracle():
                           recursive=True):
         src = open(path).read()
         assert not re.search(r"^\s*(import|from)\s+oracle", src, re.M), path
+
+
+def test_cpp_host_api_compiles():
+    """include/akantu_mi355x.hpp (the C++ API above the C ABI) compiles, and
+    the C++ driver used by the GPU test is built."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "drive_explicit.cc")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall",
+                        "-I" + os.path.join(ROOT, "include"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
