@@ -72,6 +72,32 @@ def _declare(lib):
         "akx_model_get_kernel_timing": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(_i64)]),
         "akx_model_get_kernel_bytes": (C.c_int, [_vp, C.c_int, _dp]),
         "akx_calibration_copy": (C.c_int, [C.c_int, _i64, C.c_int, _dp]),
+        "akx_batch_create": (C.c_int, [C.c_int, C.c_int, _i64, _i64, _i64, C.c_int,
+                                       C.POINTER(_vp)]),
+        "akx_batch_destroy": (C.c_int, [_vp]),
+        "akx_batch_footprint": (C.c_int, [C.c_int, C.c_int, C.c_int, _i64, _i64,
+                                          C.POINTER(_i64)]),
+        "akx_batch_add_instance": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _i64, _vp, _i64,
+                                             _vp, _vp, _vp, C.c_double,
+                                             C.POINTER(C.c_int)]),
+        "akx_batch_remove_instance": (C.c_int, [_vp, C.c_int]),
+        "akx_batch_get_time_step": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+        "akx_batch_set_time_step": (C.c_int, [_vp, C.c_int, C.c_double]),
+        "akx_batch_get_steps": (C.c_int, [_vp, C.c_int, C.POINTER(_i64)]),
+        "akx_batch_solve_step": (C.c_int, [_vp, _i64]),
+        "akx_batch_synchronize": (C.c_int, [_vp]),
+        "akx_batch_get_field": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+        "akx_batch_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "akx_batch_get_launch": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "akx_batch_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]),
+        "akx_batch_get_stats": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64),
+                                          C.POINTER(_i64), C.POINTER(_i64)]),
+        "akx_batch_capacity_left": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64),
+                                              C.POINTER(_i64), C.POINTER(_i64)]),
+        "akx_batch_enable_kernel_timing": (C.c_int, [_vp, C.c_int]),
+        "akx_batch_get_kernel_timing": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(_i64)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

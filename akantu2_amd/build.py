@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libakantu_mi355x.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # source -> floating-point contraction of its translation unit
-SOURCES = {"akx_model.hip": "off", "akx_fast.hip": "fast"}
+SOURCES = {"akx_model.hip": "off", "akx_fast.hip": "fast", "akx_batch.hip": "off"}
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]

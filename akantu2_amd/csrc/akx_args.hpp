@@ -1,6 +1,7 @@
 // akx_args.hpp -- plain argument structs shared by the element kernels of
 // the strict (reference operation order) and fast translation units.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace akx {
@@ -30,5 +31,24 @@ struct ElemArgs {
   double *c_grad, *c_sig, *c_eps, *c_iso;
   int64_t nblocks_padded;
 };
+
+// ---------------------------------------------------------------------------
+// Where the Gauss-point state of one launch lives: previous (read) and
+// current (written) plastic history, [9][nq_all] column-major 3x3 and
+// [nq_all] for the hardening, stored in HT; and the optional grad u /
+// stress output (FP64, same indexing).
+// ---------------------------------------------------------------------------
+template <class HT> struct QpViewT {
+  const HT *p_grad, *p_sig, *p_eps, *p_iso;
+  HT *c_grad, *c_sig, *c_eps, *c_iso;
+  double *grad_u, *stress;
+  int64_t nq_all;
+};
+using QpView = QpViewT<double>;
+
+__host__ __device__ inline QpView qp_view(const ElemArgs &A, int nqp) {
+  return QpView{A.p_grad, A.p_sig, A.p_eps, A.p_iso, A.c_grad, A.c_sig,
+                A.c_eps,  A.c_iso, A.grad_u, A.stress, A.n * nqp};
+}
 
 } // namespace akx

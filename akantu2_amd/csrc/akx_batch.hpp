@@ -1,0 +1,82 @@
+// akx_batch.hpp -- device-side tables and launch arguments of the batched
+// multi-instance engine (akx_batch.hip, strict FP64 kernels) shared with the
+// FP32-form batched element kernels (akx_fast.hip).
+//
+// Work lists:
+//   tiles  : runs of at most `tile` consecutive elements of ONE instance, so
+//            material parameters and the history parity are uniform over a
+//            tile; one list per (element type, law) group.
+//   slices : 256-node slices of the live instances (an instance owns whole
+//            slices: its node range starts on a multiple of 256).
+// A persistent launch gives each block a contiguous chunk of the list;
+// blocks are dealt round-robin to the 8 XCDs, so the chunk of hardware block
+// b is logical chunk (b % 8) * (nblocks / 8) + b / 8 and neighbouring chunks
+// (neighbouring elements, shared nodes) stay in one XCD's L2.
+#pragma once
+#include "akx_args.hpp"
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace akx {
+
+struct BInst { // one per instance id
+  double lambda, mu, sigma_y, h, rho, dt;
+  int32_t parity_base; // steps done before the current solve call, mod 2
+  int32_t law;
+};
+
+struct BTile {
+  int64_t first; // first element (index in the element arena of its type)
+  int32_t count;
+  int32_t inst;
+};
+
+struct BSlice {
+  int64_t node0; // multiple of 256
+  int32_t count; // nodes of the instance in this slice
+  int32_t inst;
+};
+
+struct BatchElemArgs {
+  const BTile *tiles;
+  int64_t ntiles;
+  const BInst *inst;
+  const int32_t *conn; // [elem][nnpe] arena node ids
+  const int32_t *pos;  // [elem][nnpe] table entries
+  const double *X, *u; // [3][ncap]
+  int64_t ncap;
+  void *fe;            // [entry][3] elemental forces (FP64 or FP32)
+  // plastic history, two buffers (ping-pong by step parity) of
+  // grad u, stress, plastic strain ([9][hcap]) and hardening ([hcap]);
+  // Gauss point q of element e is index hoff + e * nqp + q
+  void *hist[2][4];
+  int64_t hcap, hoff;
+  const double *hex_tab; // Hex8: dN/dxi [8][3][8], N [8][8], w [8]
+  int32_t parity;        // step index within the solve call, mod 2
+  int64_t nblocks;       // grid size, a multiple of 8
+};
+
+constexpr int HEX_TAB_N = 192, HEX_TAB_W = 256, HEX_TAB_SIZE = 264;
+
+// [w0, w1): the part of a work list of nwork items owned by this block
+__device__ __forceinline__ void batch_chunk(int64_t nwork, int64_t nblocks, int64_t &w0,
+                                            int64_t &w1) {
+  const int64_t b = blockIdx.x, per = nblocks >> 3;
+  const int64_t lb = (b & 7) * per + (b >> 3);
+  const int64_t chunk = (nwork + nblocks - 1) / nblocks;
+  w0 = lb * chunk < nwork ? lb * chunk : nwork;
+  w1 = w0 + chunk < nwork ? w0 + chunk : nwork;
+}
+
+// the Gauss-point view of one step: read buffer `par`, write buffer par ^ 1
+template <class HT>
+__device__ __forceinline__ QpViewT<HT> batch_view(const BatchElemArgs &A, int par) {
+  return QpViewT<HT>{(const HT *)A.hist[par][0], (const HT *)A.hist[par][1],
+                     (const HT *)A.hist[par][2], (const HT *)A.hist[par][3],
+                     (HT *)A.hist[par ^ 1][0],   (HT *)A.hist[par ^ 1][1],
+                     (HT *)A.hist[par ^ 1][2],   (HT *)A.hist[par ^ 1][3],
+                     nullptr,                    nullptr,
+                     A.hcap};
+}
+
+} // namespace akx

@@ -18,6 +18,8 @@
 // Hex8: 8 lanes per element (one per Gauss point; in the last phase one per
 // node), 32 elements per 256-thread block, nodal data and Q in LDS.
 // Tet4: one thread per element (single Gauss point).
+// The same element code serves the batched FP32 form (persistent kernels
+// over the tile list of many instances, per-instance parameters).
 #define AKX_MATH_NS akxf
 #include "akx_args.hpp"
 #include "akx_math.hpp"
@@ -27,7 +29,14 @@
 
 namespace akxf {
 
+using akx::BatchElemArgs;
+using akx::BInst;
+using akx::BTile;
 using akx::ElemArgs;
+using akx::QpViewT;
+using akx::batch_chunk;
+using akx::batch_view;
+using akx::qp_view;
 
 constexpr int LAW_ELASTIC = 0, LAW_ELASTIC_FD = 1, LAW_NEOHOOKEAN = 2,
               LAW_PLASTIC = 3;
@@ -63,11 +72,10 @@ __device__ __forceinline__ R inv_det(const R (&m)[3][3], R (&r)[3][3]) {
 
 // Constitutive law at one Gauss point: g -> stress S (sigma or PK2) and the
 // tensor P contracted with dN/dX (sigma, or F S for finite deformation).
-template <int LAW, class CT>
-__device__ __forceinline__ void law_at(const ElemArgs &A, int64_t iq,
-                                       int64_t nq_all, CT lambda, CT mu, CT sy,
-                                       CT hh, const CT (&g)[3][3], CT (&S)[3][3],
-                                       CT (&P)[3][3]) {
+template <int LAW, class CT, class HT>
+__device__ __forceinline__ void law_at(const QpViewT<HT> &V, int64_t iq, CT lambda,
+                                       CT mu, CT sy, CT hh, const CT (&g)[3][3],
+                                       CT (&S)[3][3], CT (&P)[3][3]) {
   if constexpr (LAW == LAW_ELASTIC) {
     elastic_stress(lambda, mu, g, CT(0), S);
   } else if constexpr (LAW == LAW_ELASTIC_FD) {
@@ -85,31 +93,31 @@ __device__ __forceinline__ void law_at(const ElemArgs &A, int64_t iq,
     for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
       for (int ii = 0; ii < 3; ++ii) {
-        const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
-        pg[ii][jj] = CT(A.p_grad[o]);
-        ps[ii][jj] = CT(A.p_sig[o]);
-        pe[ii][jj] = CT(A.p_eps[o]);
+        const int64_t o = (int64_t)(ii + 3 * jj) * V.nq_all + iq;
+        pg[ii][jj] = CT(V.p_grad[o]);
+        ps[ii][jj] = CT(V.p_sig[o]);
+        pe[ii][jj] = CT(V.p_eps[o]);
       }
-    plastic_lih(lambda, mu, sy, hh, g, pg, ps, CT(A.p_iso[iq]), pe, S, iso, eps);
+    plastic_lih(lambda, mu, sy, hh, g, pg, ps, CT(V.p_iso[iq]), pe, S, iso, eps);
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
       for (int ii = 0; ii < 3; ++ii) {
-        const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
-        A.c_grad[o] = double(g[ii][jj]);
-        A.c_sig[o] = double(S[ii][jj]);
-        A.c_eps[o] = double(eps[ii][jj]);
+        const int64_t o = (int64_t)(ii + 3 * jj) * V.nq_all + iq;
+        V.c_grad[o] = HT(g[ii][jj]);
+        V.c_sig[o] = HT(S[ii][jj]);
+        V.c_eps[o] = HT(eps[ii][jj]);
       }
-    A.c_iso[iq] = double(iso);
+    V.c_iso[iq] = HT(iso);
   }
-  if (A.grad_u) {
+  if (V.grad_u) {
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
       for (int ii = 0; ii < 3; ++ii) {
-        const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
-        A.grad_u[o] = double(g[ii][jj]);
-        A.stress[o] = double(S[ii][jj]);
+        const int64_t o = (int64_t)(ii + 3 * jj) * V.nq_all + iq;
+        V.grad_u[o] = double(g[ii][jj]);
+        V.stress[o] = double(S[ii][jj]);
       }
   }
   if constexpr (LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN) {
@@ -143,27 +151,26 @@ __device__ __forceinline__ void element_params(const ElemArgs &A, int64_t i,
 }
 
 // ---------------------------------------------------------------------------
-// Hex8: 8 lanes per element
+// Hex8: 8 lanes per element.  hex8_group handles EPB elements; every lane of
+// the block calls it (it holds barriers), lanes without an element with
+// active = false.  sD must hold dN_n/dxi_m ([q][m][n]) before the call.
 // ---------------------------------------------------------------------------
 constexpr int HEX_EPB = 32; // elements per 256-thread block
 
-template <int LAW, class CT, class FT>
-__global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
-  __shared__ double sN[HEX_EPB][8][6]; // X (0..2) and u (3..5) of the 8 nodes
-  __shared__ CT sQ[HEX_EPB][8][9];     // Q of the 8 Gauss points
-  __shared__ CT sD[8][3][8];           // dN_n/dxi_m at Gauss point q
-  const int t = threadIdx.x, el = t >> 3, q = t & 7;
-  for (int k = t; k < 192; k += 256)
-    (&sD[0][0][0])[k] = CT((&c_dnds[0][0][0])[k]);
-  const int64_t i = xcd_block(A.nblocks_padded) * HEX_EPB + el;
-  const bool active = i < A.n;
+template <int LAW, class CT, class FT, class HT, int EPB>
+__device__ __forceinline__ void hex8_group(double (&sN)[EPB][8][6], CT (&sQ)[EPB][8][9],
+                                           const CT (&sD)[8][3][8], int el, int q,
+                                           bool active, const int32_t *conn_e,
+                                           const double *Xs, const double *us,
+                                           int64_t stride, CT lambda, CT mu, CT sy,
+                                           CT hh, const QpViewT<HT> &V, int64_t iq,
+                                           const int32_t *pos_e, FT *fe) {
   if (active) {
-    const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
-    const int32_t node = A.conn[e * 8 + q];
+    const int32_t node = conn_e[q];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      sN[el][q][c] = A.X[(int64_t)c * A.nn + node];
-      sN[el][q][3 + c] = A.u[(int64_t)c * A.nn + node];
+      sN[el][q][c] = Xs[(int64_t)c * stride + node];
+      sN[el][q][3 + c] = us[(int64_t)c * stride + node];
     }
   }
   __syncthreads();
@@ -218,10 +225,8 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
           s += H[a][m] * Ji[b][m];
         g[a][b] = s;
       }
-    CT lambda, mu, sy, hh;
-    element_params(A, i, lambda, mu, sy, hh);
     CT S[3][3], P[3][3];
-    law_at<LAW, CT>(A, i * 8 + q, A.n * 8, lambda, mu, sy, hh, g, S, P);
+    law_at<LAW, CT, HT>(V, iq, lambda, mu, sy, hh, g, S, P);
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -246,32 +251,52 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
         for (int k = 0; k < 3; ++k)
           r[k] += sQ[el][qq][3 * k + m] * d;
       }
-    FT *out = reinterpret_cast<FT *>(A.fe) +
-              (int64_t)A.pos[(A.pos_offset + i) * 8 + n] * 3;
+    FT *out = fe + (int64_t)pos_e[n] * 3;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       out[k] = FT(r[k]);
   }
 }
 
+template <int LAW, class CT, class FT>
+__global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
+  __shared__ double sN[HEX_EPB][8][6]; // X (0..2) and u (3..5) of the 8 nodes
+  __shared__ CT sQ[HEX_EPB][8][9];     // Q of the 8 Gauss points
+  __shared__ CT sD[8][3][8];           // dN_n/dxi_m at Gauss point q
+  const int t = threadIdx.x, el = t >> 3, q = t & 7;
+  for (int k = t; k < 192; k += 256)
+    (&sD[0][0][0])[k] = CT((&c_dnds[0][0][0])[k]);
+  const int64_t i = xcd_block(A.nblocks_padded) * HEX_EPB + el;
+  const bool active = i < A.n;
+  int64_t e = 0;
+  CT lambda = CT(0), mu = CT(0), sy = CT(0), hh = CT(0);
+  if (active) {
+    e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+    element_params(A, i, lambda, mu, sy, hh);
+  }
+  hex8_group<LAW, CT, FT, double, HEX_EPB>(
+      sN, sQ, sD, el, q, active, A.conn + e * 8, A.X, A.u, A.nn, lambda, mu, sy, hh,
+      qp_view(A, 8), i * 8 + q, A.pos + (A.pos_offset + (active ? i : 0)) * 8,
+      reinterpret_cast<FT *>(A.fe));
+}
+
 // ---------------------------------------------------------------------------
 // Tet4: one thread per element
 // ---------------------------------------------------------------------------
-template <int LAW, class CT, class FT>
-__global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
-  const int64_t i = xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
-  if (i >= A.n)
-    return;
-  const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
-  const int4 nd = *reinterpret_cast<const int4 *>(A.conn + e * 4);
+template <int LAW, class CT, class FT, class HT>
+__device__ __forceinline__ void tet4_one(const int32_t *conn_e, const double *Xs,
+                                         const double *us, int64_t stride, CT lambda,
+                                         CT mu, CT sy, CT hh, const QpViewT<HT> &V,
+                                         int64_t iq, const int32_t *pos_e, FT *fe) {
+  const int4 nd = *reinterpret_cast<const int4 *>(conn_e);
   const int32_t nodes[4] = {nd.x, nd.y, nd.z, nd.w};
   double X[4][3], U[4][3];
 #pragma unroll
   for (int n = 0; n < 4; ++n)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      X[n][c] = A.X[(int64_t)c * A.nn + nodes[n]];
-      U[n][c] = A.u[(int64_t)c * A.nn + nodes[n]];
+      X[n][c] = Xs[(int64_t)c * stride + nodes[n]];
+      U[n][c] = us[(int64_t)c * stride + nodes[n]];
     }
   // natural directions: xi -> node 3, eta -> node 1, zeta -> node 2
   const int nm[3] = {3, 1, 2};
@@ -297,10 +322,8 @@ __global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
         s += H[a][m] * Ji[b][m];
       g[a][b] = s;
     }
-  CT lambda, mu, sy, hh;
-  element_params(A, i, lambda, mu, sy, hh);
   CT S[3][3], P[3][3];
-  law_at<LAW, CT>(A, i, A.n, lambda, mu, sy, hh, g, S, P);
+  law_at<LAW, CT, HT>(V, iq, lambda, mu, sy, hh, g, S, P);
   CT Q[3][3];
 #pragma unroll
   for (int k = 0; k < 3; ++k)
@@ -312,14 +335,74 @@ __global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
         s += P[k][J2] * Ji[J2][m];
       Q[k][m] = wJ * s;
     }
-  FT *fe = reinterpret_cast<FT *>(A.fe);
-  const int4 pv = *reinterpret_cast<const int4 *>(A.pos + (A.pos_offset + i) * 4);
+  const int4 pv = *reinterpret_cast<const int4 *>(pos_e);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     fe[(int64_t)pv.x * 3 + k] = FT(-(Q[k][0] + Q[k][1] + Q[k][2]));
     fe[(int64_t)pv.y * 3 + k] = FT(Q[k][1]);
     fe[(int64_t)pv.z * 3 + k] = FT(Q[k][2]);
     fe[(int64_t)pv.w * 3 + k] = FT(Q[k][0]);
+  }
+}
+
+template <int LAW, class CT, class FT>
+__global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
+  const int64_t i = xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
+  if (i >= A.n)
+    return;
+  const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+  CT lambda, mu, sy, hh;
+  element_params(A, i, lambda, mu, sy, hh);
+  tet4_one<LAW, CT, FT, double>(A.conn + e * 4, A.X, A.u, A.nn, lambda, mu, sy, hh,
+                                qp_view(A, 1), i, A.pos + (A.pos_offset + i) * 4,
+                                reinterpret_cast<FT *>(A.fe));
+}
+
+// ---------------------------------------------------------------------------
+// Batched FP32 form: persistent over the tiles of one (element type, law)
+// group; per-instance parameters, FP32 history (ping-pong by step parity).
+// ---------------------------------------------------------------------------
+template <int LAW>
+__global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
+  int64_t t0, t1;
+  batch_chunk(A.ntiles, A.nblocks, t0, t1);
+  for (int64_t tt = t0; tt < t1; ++tt) {
+    const BTile T = A.tiles[tt];
+    const BInst P = A.inst[T.inst];
+    const QpViewT<float> V = batch_view<float>(A, (P.parity_base ^ A.parity) & 1);
+    for (int j = threadIdx.x; j < T.count; j += blockDim.x) {
+      const int64_t e = T.first + j;
+      tet4_one<LAW, float, float, float>(A.conn + e * 4, A.X, A.u, A.ncap,
+                                         float(P.lambda), float(P.mu), float(P.sigma_y),
+                                         float(P.h), V, A.hoff + e, A.pos + e * 4,
+                                         reinterpret_cast<float *>(A.fe));
+    }
+  }
+}
+
+template <int LAW, int EPB>
+__global__ void __launch_bounds__(EPB * 8) k_batch_hex8_f32(BatchElemArgs A) {
+  __shared__ double sN[EPB][8][6];
+  __shared__ float sQ[EPB][8][9];
+  __shared__ float sD[8][3][8];
+  const int t = threadIdx.x, el = t >> 3, q = t & 7;
+  for (int k = t; k < 192; k += EPB * 8)
+    (&sD[0][0][0])[k] = float(A.hex_tab[k]);
+  __syncthreads();
+  int64_t t0, t1;
+  batch_chunk(A.ntiles, A.nblocks, t0, t1);
+  for (int64_t tt = t0; tt < t1; ++tt) {
+    const BTile T = A.tiles[tt];
+    const BInst P = A.inst[T.inst];
+    const QpViewT<float> V = batch_view<float>(A, (P.parity_base ^ A.parity) & 1);
+    for (int base = 0; base < T.count; base += EPB) {
+      const bool active = base + el < T.count;
+      const int64_t e = T.first + (active ? base + el : 0);
+      hex8_group<LAW, float, float, float, EPB>(
+          sN, sQ, sD, el, q, active, A.conn + e * 8, A.X, A.u, A.ncap, float(P.lambda),
+          float(P.mu), float(P.sigma_y), float(P.h), V, A.hoff + e * 8 + q,
+          A.pos + e * 8, reinterpret_cast<float *>(A.fe));
+    }
   }
 }
 
@@ -358,6 +441,35 @@ hipError_t launch_elements(int etype, int law, bool fp32, const ElemArgs &A,
                            hipStream_t stream) {
   return fp32 ? launch_t<float, float>(etype, law, A, stream)
               : launch_t<double, double>(etype, law, A, stream);
+}
+
+template <int LAW>
+static void launch_batch_law(int etype, int block, const BatchElemArgs &A,
+                             hipStream_t s) {
+  dim3 grid((unsigned)A.nblocks);
+  if (etype == 0) {
+    hipLaunchKernelGGL(k_batch_tet4_f32<LAW>, grid, dim3(block), 0, s, A);
+  } else if (block == 64) {
+    hipLaunchKernelGGL((k_batch_hex8_f32<LAW, 8>), grid, dim3(64), 0, s, A);
+  } else if (block == 128) {
+    hipLaunchKernelGGL((k_batch_hex8_f32<LAW, 16>), grid, dim3(128), 0, s, A);
+  } else {
+    hipLaunchKernelGGL((k_batch_hex8_f32<LAW, 32>), grid, dim3(256), 0, s, A);
+  }
+}
+
+hipError_t launch_batch_elements_f32(int etype, int law, int block,
+                                     const BatchElemArgs &A, hipStream_t stream) {
+  if (block != 64 && block != 128 && block != 256)
+    return hipErrorInvalidValue;
+  switch (law) {
+  case LAW_ELASTIC: launch_batch_law<LAW_ELASTIC>(etype, block, A, stream); break;
+  case LAW_ELASTIC_FD: launch_batch_law<LAW_ELASTIC_FD>(etype, block, A, stream); break;
+  case LAW_NEOHOOKEAN: launch_batch_law<LAW_NEOHOOKEAN>(etype, block, A, stream); break;
+  case LAW_PLASTIC: launch_batch_law<LAW_PLASTIC>(etype, block, A, stream); break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 } // namespace akxf

@@ -20,6 +20,8 @@
 #include "akx_math.hpp"
 #include <stdint.h>
 
+#include <cmath>
+
 namespace akx {
 
 constexpr int ET_TET4 = 0;
@@ -48,6 +50,58 @@ struct Hex8Tables {
   float dnds_f[8][3][8];
 };
 extern __constant__ Hex8Tables c_hex8;
+
+// Hex8 natural derivatives, shapes and weights at the tensor-product Gauss
+// points +-1/sqrt(3), xi fastest (element_class_hexahedron_8_inline_impl.hh
+// :111-188, gauss_integration.cc:42-43); host code, shared by the
+// single-model and batched engines so both upload the same bits.
+inline void fill_hex8_tables(Hex8Tables &t) {
+  double pos[2] = {-1. / std::sqrt(3.), 1. / std::sqrt(3.)};
+  for (int q = 0; q < 8; ++q) {
+    double c[3] = {pos[q & 1], pos[(q >> 1) & 1], pos[(q >> 2) & 1]};
+    double(*d)[8] = t.dnds[q];
+    d[0][0] = -.125 * (1 - c[1]) * (1 - c[2]);
+    d[0][1] = .125 * (1 - c[1]) * (1 - c[2]);
+    d[0][2] = .125 * (1 + c[1]) * (1 - c[2]);
+    d[0][3] = -.125 * (1 + c[1]) * (1 - c[2]);
+    d[0][4] = -.125 * (1 - c[1]) * (1 + c[2]);
+    d[0][5] = .125 * (1 - c[1]) * (1 + c[2]);
+    d[0][6] = .125 * (1 + c[1]) * (1 + c[2]);
+    d[0][7] = -.125 * (1 + c[1]) * (1 + c[2]);
+    d[1][0] = -.125 * (1 - c[0]) * (1 - c[2]);
+    d[1][1] = -.125 * (1 + c[0]) * (1 - c[2]);
+    d[1][2] = .125 * (1 + c[0]) * (1 - c[2]);
+    d[1][3] = .125 * (1 - c[0]) * (1 - c[2]);
+    d[1][4] = -.125 * (1 - c[0]) * (1 + c[2]);
+    d[1][5] = -.125 * (1 + c[0]) * (1 + c[2]);
+    d[1][6] = .125 * (1 + c[0]) * (1 + c[2]);
+    d[1][7] = .125 * (1 - c[0]) * (1 + c[2]);
+    d[2][0] = -.125 * (1 - c[0]) * (1 - c[1]);
+    d[2][1] = -.125 * (1 + c[0]) * (1 - c[1]);
+    d[2][2] = -.125 * (1 + c[0]) * (1 + c[1]);
+    d[2][3] = -.125 * (1 - c[0]) * (1 + c[1]);
+    d[2][4] = .125 * (1 - c[0]) * (1 - c[1]);
+    d[2][5] = .125 * (1 + c[0]) * (1 - c[1]);
+    d[2][6] = .125 * (1 + c[0]) * (1 + c[1]);
+    d[2][7] = .125 * (1 - c[0]) * (1 + c[1]);
+    t.N[q][0] = .125 * (1 - c[0]) * (1 - c[1]) * (1 - c[2]);
+    t.N[q][1] = .125 * (1 + c[0]) * (1 - c[1]) * (1 - c[2]);
+    t.N[q][2] = .125 * (1 + c[0]) * (1 + c[1]) * (1 - c[2]);
+    t.N[q][3] = .125 * (1 - c[0]) * (1 + c[1]) * (1 - c[2]);
+    t.N[q][4] = .125 * (1 - c[0]) * (1 - c[1]) * (1 + c[2]);
+    t.N[q][5] = .125 * (1 + c[0]) * (1 - c[1]) * (1 + c[2]);
+    t.N[q][6] = .125 * (1 + c[0]) * (1 + c[1]) * (1 + c[2]);
+    t.N[q][7] = .125 * (1 - c[0]) * (1 + c[1]) * (1 + c[2]);
+    double wq = 1.;
+    wq *= 1.;
+    wq *= 1.;
+    wq *= 1.;
+    t.w[q] = wq;
+    for (int i = 0; i < 3; ++i)
+      for (int n = 0; n < 8; ++n)
+        t.dnds_f[q][i][n] = (float)d[i][n];
+  }
+}
 
 // blockIdx -> logical block so that each of the 8 XCDs walks a contiguous
 // range of elements/nodes (blocks are dealt round-robin to the XCDs; the
@@ -121,6 +175,66 @@ template <> struct Shape<ET_HEX8, double> {
 };
 
 // ---------------------------------------------------------------------------
+// Inradius: Tet4 2 * Math::tetrahedron_inradius (aka_math_tmpl.hh:71-124),
+// Hex8 min of the 12 edges (element_class_hexahedron_8_inline_impl.hh:194-212)
+__device__ __forceinline__ double dist3(const double (&a)[3], const double (&b)[3]) {
+  double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ __forceinline__ double tet_volume(const double (&c1)[3],
+                                             const double (&c2)[3],
+                                             const double (&c3)[3],
+                                             const double (&c4)[3]) {
+  double xx[3][3];
+  auto setcol = [&](int k, const double (&c)[3]) {
+    xx[0][k] = c[0];
+    xx[1][k] = c[1];
+    xx[2][k] = c[2];
+  };
+  setcol(0, c2);
+  setcol(1, c3);
+  setcol(2, c4);
+  double vol = det3(xx);
+  setcol(0, c1);
+  vol -= det3(xx);
+  setcol(1, c2);
+  vol += det3(xx);
+  setcol(2, c3);
+  vol -= det3(xx);
+  vol /= 6;
+  return vol;
+}
+
+template <int ET>
+__device__ __forceinline__ double inradius(const double (&X)[Elem<ET>::nnpe][3]) {
+  if constexpr (ET == ET_TET4) {
+    double l12 = dist3(X[0], X[1]), l13 = dist3(X[0], X[2]), l14 = dist3(X[0], X[3]);
+    double l23 = dist3(X[1], X[2]), l24 = dist3(X[1], X[3]), l34 = dist3(X[2], X[3]);
+    double s1 = (l12 + l23 + l13) * 0.5;
+    s1 = sqrt(s1 * (s1 - l12) * (s1 - l23) * (s1 - l13));
+    double s2 = (l12 + l24 + l14) * 0.5;
+    s2 = sqrt(s2 * (s2 - l12) * (s2 - l24) * (s2 - l14));
+    double s3 = (l23 + l34 + l24) * 0.5;
+    s3 = sqrt(s3 * (s3 - l23) * (s3 - l34) * (s3 - l24));
+    double s4 = (l13 + l34 + l14) * 0.5;
+    s4 = sqrt(s4 * (s4 - l13) * (s4 - l34) * (s4 - l14));
+    double volume = tet_volume(X[0], X[1], X[2], X[3]);
+    return 2. * (3 * volume / (s1 + s2 + s3 + s4));
+  } else {
+    const int ed[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
+                           {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}};
+    double p = dist3(X[0], X[1]);
+#pragma unroll
+    for (int k = 1; k < 12; ++k) {
+      double l = dist3(X[ed[k][0]], X[ed[k][1]]);
+      p = (l < p) ? l : p;
+    }
+    return p;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Force of one quadrature point, r_q[n][k] (before the w*detJ factor).
 // Small strain: B^T sigma (shape_lagrange_inline_impl.hh:373-410).
 // Finite deformation: (B_L + B_NL(grad u))^T voigt(S) (material.cc:525-584,
@@ -178,21 +292,72 @@ __device__ __forceinline__ void btd_finite(const R (&g)[3][3], const R (&S)[3][3
   r[2] = btd_finite_k<2>(g, S, b);
 }
 
-// ---------------------------------------------------------------------------
-// Element pass.  One thread per element of one material.
-// ---------------------------------------------------------------------------
-template <int ET, int LAW, class CT, class FT>
-__global__ void __launch_bounds__(256)
-k_element_forces(ElemArgs A) {
-  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
-  const int64_t i = xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
-  if (i >= A.n)
-    return;
-  const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+// Constitutive law of one Gauss point in the reference's operation order
+// (material_elastic_inline_impl.hh:31-73, material_neohookean_inline_impl.hh
+// :39-50, material_linear_isotropic_hardening_inline_impl.hh:31-86), with the
+// history and output traffic of Material::computeAllStresses.
+template <int LAW, class CT, class HT>
+__device__ __forceinline__ void strict_law(const QpViewT<HT> &V, int64_t iq, CT lambda,
+                                           CT mu, CT sy, CT hh, const CT (&g)[3][3],
+                                           CT (&S)[3][3]) {
+  if constexpr (LAW == LAW_ELASTIC) {
+    elastic_stress(lambda, mu, g, CT(-0.0), S);
+  } else if constexpr (LAW == LAW_ELASTIC_FD) {
+    CT E[3][3];
+    green_strain(g, E);
+    elastic_stress(lambda, mu, E, CT(-0.0), S);
+  } else if constexpr (LAW == LAW_NEOHOOKEAN) {
+    neohookean_pk2(lambda, mu, g, S);
+  } else {
+    CT pg[3][3], ps[3][3], pe[3][3], eps[3][3], iso;
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+        const int64_t o = (int64_t)(ii + 3 * jj) * V.nq_all + iq;
+        pg[ii][jj] = CT(V.p_grad[o]);
+        ps[ii][jj] = CT(V.p_sig[o]);
+        pe[ii][jj] = CT(V.p_eps[o]);
+      }
+    plastic_lih(lambda, mu, sy, hh, g, pg, ps, CT(V.p_iso[iq]), pe, S, iso, eps);
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+        const int64_t o = (int64_t)(ii + 3 * jj) * V.nq_all + iq;
+        V.c_grad[o] = HT(g[ii][jj]);
+        V.c_sig[o] = HT(S[ii][jj]);
+        V.c_eps[o] = HT(eps[ii][jj]);
+      }
+    V.c_iso[iq] = HT(iso);
+  }
+  if (V.grad_u) {
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+        const int64_t o = (int64_t)(ii + 3 * jj) * V.nq_all + iq;
+        V.grad_u[o] = double(g[ii][jj]);
+        V.stress[o] = double(S[ii][jj]);
+      }
+  }
+}
 
+// ---------------------------------------------------------------------------
+// Element pass, one thread per element: every Gauss point of element e
+// (nodes conn_e, table entries pos_e, Gauss points iq0 ..), per-node sums
+// over the Gauss points in ascending order.  Shared by the single-model
+// launch (one material) and the batched persistent launch.
+// ---------------------------------------------------------------------------
+template <int ET, int LAW, class CT, class FT, class HT>
+__device__ __forceinline__ void element_forces_one(
+    const int32_t *conn_e, const double *Xs, const double *us, int64_t stride,
+    CT lambda, CT mu, CT sy, CT hh, const QpViewT<HT> &V, int64_t iq0,
+    const int32_t *pos_e, void *fe_out) {
+  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
   int32_t nodes[NN];
   {
-    const int4 *c4 = reinterpret_cast<const int4 *>(A.conn + e * NN);
+    const int4 *c4 = reinterpret_cast<const int4 *>(conn_e);
 #pragma unroll
     for (int k = 0; k < NN / 4; ++k) {
       int4 v = c4[k];
@@ -208,22 +373,9 @@ k_element_forces(ElemArgs A) {
   for (int n = 0; n < NN; ++n)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      X[n][c] = A.X[(int64_t)c * A.nn + nodes[n]];
-      U[n][c] = A.u[(int64_t)c * A.nn + nodes[n]];
+      X[n][c] = Xs[(int64_t)c * stride + nodes[n]];
+      U[n][c] = us[(int64_t)c * stride + nodes[n]];
     }
-
-  CT lambda, mu, sy = CT(A.sigma_y), hh = CT(A.h);
-  if (A.lam_el) {
-    lambda = CT(A.lam_el[i]);
-    mu = CT(A.mu_el[i]);
-  } else {
-    lambda = CT(A.lambda);
-    mu = CT(A.mu);
-  }
-  if (A.sy_el)
-    sy = CT(A.sy_el[i]);
-  if (A.h_el)
-    hh = CT(A.h_el[i]);
 
   CT re[NN][3];
 
@@ -243,51 +395,8 @@ k_element_forces(ElemArgs A) {
         for (int jj = 0; jj < 3; ++jj)
           g[ii][jj] = (n == 0) ? U[n][ii] * b[jj] : g[ii][jj] + U[n][ii] * b[jj];
     }
-    const int64_t iq = i * NQ + q;
-    const int64_t nq_all = A.n * NQ;
     CT S[3][3];
-    if constexpr (LAW == LAW_ELASTIC) {
-      elastic_stress(lambda, mu, g, CT(-0.0), S);
-    } else if constexpr (LAW == LAW_ELASTIC_FD) {
-      CT E[3][3];
-      green_strain(g, E);
-      elastic_stress(lambda, mu, E, CT(-0.0), S);
-    } else if constexpr (LAW == LAW_NEOHOOKEAN) {
-      neohookean_pk2(lambda, mu, g, S);
-    } else {
-      CT pg[3][3], ps[3][3], pe[3][3], eps[3][3], iso;
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-        for (int ii = 0; ii < 3; ++ii) {
-          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
-          pg[ii][jj] = CT(A.p_grad[o]);
-          ps[ii][jj] = CT(A.p_sig[o]);
-          pe[ii][jj] = CT(A.p_eps[o]);
-        }
-      plastic_lih(lambda, mu, sy, hh, g, pg, ps, CT(A.p_iso[iq]), pe, S, iso,
-                  eps);
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-        for (int ii = 0; ii < 3; ++ii) {
-          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
-          A.c_grad[o] = double(g[ii][jj]);
-          A.c_sig[o] = double(S[ii][jj]);
-          A.c_eps[o] = double(eps[ii][jj]);
-        }
-      A.c_iso[iq] = double(iso);
-    }
-    if (A.grad_u) {
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-        for (int ii = 0; ii < 3; ++ii) {
-          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
-          A.grad_u[o] = double(g[ii][jj]);
-          A.stress[o] = double(S[ii][jj]);
-        }
-    }
+    strict_law<LAW, CT, HT>(V, iq0 + q, lambda, mu, sy, hh, g, S);
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
       CT r[3], b[3];
@@ -302,8 +411,8 @@ k_element_forces(ElemArgs A) {
     }
   }
 
-  FT *fe = reinterpret_cast<FT *>(A.fe);
-  const int4 *p4 = reinterpret_cast<const int4 *>(A.pos + (A.pos_offset + i) * NN);
+  FT *fe = reinterpret_cast<FT *>(fe_out);
+  const int4 *p4 = reinterpret_cast<const int4 *>(pos_e);
 #pragma unroll
   for (int kk = 0; kk < NN / 4; ++kk) {
     const int4 pv = p4[kk];
@@ -316,35 +425,59 @@ k_element_forces(ElemArgs A) {
   }
 }
 
+template <int ET, int LAW, class CT, class FT>
+__global__ void __launch_bounds__(256)
+k_element_forces(ElemArgs A) {
+  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
+  const int64_t i = xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
+  if (i >= A.n)
+    return;
+  const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+  CT lambda, mu, sy = CT(A.sigma_y), hh = CT(A.h);
+  if (A.lam_el) {
+    lambda = CT(A.lam_el[i]);
+    mu = CT(A.mu_el[i]);
+  } else {
+    lambda = CT(A.lambda);
+    mu = CT(A.mu);
+  }
+  if (A.sy_el)
+    sy = CT(A.sy_el[i]);
+  if (A.h_el)
+    hh = CT(A.h_el[i]);
+  element_forces_one<ET, LAW, CT, FT, double>(A.conn + e * NN, A.X, A.u, A.nn, lambda, mu, sy,
+                                      hh, qp_view(A, NQ), i * NQ,
+                                      A.pos + (A.pos_offset + i) * NN, A.fe);
+}
+
 
 // ---------------------------------------------------------------------------
 // Hex8 element pass in the reference's operation order, one lane per Gauss
-// point (8 lanes per element, 8 elements per 64-lane block).  Nodal
-// coordinates and displacements are staged in LDS; each lane forms J, its
-// LU inverse, the dN/dX columns, grad u, the law and the 24 force terms of
-// its Gauss point times w*detJ; the per-node sums over the Gauss points are
-// then taken in ascending Gauss-point order (the oracle's and Eigen's GEMV
-// order), four nodes at a time, by the lanes that own the nodes.
+// point (8 lanes per element, EPB elements per block).  Nodal coordinates
+// and displacements are staged in LDS; each lane forms J, its LU inverse,
+// the dN/dX columns, grad u, the law and the 24 force terms of its Gauss
+// point times w*detJ; the per-node sums over the Gauss points are then taken
+// in ascending Gauss-point order (the oracle's and Eigen's GEMV order), four
+// nodes at a time, by the lanes that own the nodes.
+//
+// hex8_strict_group handles one group of EPB elements; every lane of the
+// block calls it (it holds barriers), inactive lanes with active = false.
+// sD holds dN_n/dxi_m at the Gauss points, [q][m][n]; wq is the Gauss weight.
 // ---------------------------------------------------------------------------
 constexpr int HEX_STRICT_EPB = 8;
 
-template <int LAW>
-__global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
-  __shared__ double sN[HEX_STRICT_EPB][8][6];
-  __shared__ double sR[HEX_STRICT_EPB][8][13]; // 12 terms (+1 pad: banks)
-  __shared__ double sD[8][3][8];
-  const int t = threadIdx.x, el = t >> 3, q = t & 7;
-  for (int k = t; k < 192; k += 64)
-    (&sD[0][0][0])[k] = (&c_hex8.dnds[0][0][0])[k];
-  const int64_t i = xcd_block(A.nblocks_padded) * HEX_STRICT_EPB + el;
-  const bool active = i < A.n;
+template <int LAW, int EPB, class HT>
+__device__ __forceinline__ void hex8_strict_group(
+    double (&sN)[EPB][8][6], double (&sR)[EPB][8][13], const double (&sD)[8][3][8],
+    double wq, int el, int q, bool active, const int32_t *conn_e, const double *Xs,
+    const double *us, int64_t stride, double lambda, double mu, double sy, double hh,
+    const QpViewT<HT> &V, int64_t iq, const int32_t *pos_e, double *fe) {
   if (active) {
-    const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
-    const int32_t node = A.conn[e * 8 + q];
+    const int32_t node = conn_e[q];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      sN[el][q][c] = A.X[(int64_t)c * A.nn + node];
-      sN[el][q][3 + c] = A.u[(int64_t)c * A.nn + node];
+      sN[el][q][c] = Xs[(int64_t)c * stride + node];
+      sN[el][q][3 + c] = us[(int64_t)c * stride + node];
     }
   }
   __syncthreads();
@@ -363,7 +496,7 @@ __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
         J[a][j] = s;
       }
     double detJ = jacobian_det_inv(J, Ji);
-    wJ = detJ * c_hex8.w[q];
+    wJ = detJ * wq;
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
       double b[3];
@@ -382,61 +515,16 @@ __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
           g[a][j] = (n == 0) ? un * b[j] : g[a][j] + un * b[j];
       }
     }
-    double lambda = A.lam_el ? A.lam_el[i] : A.lambda;
-    double mu = A.mu_el ? A.mu_el[i] : A.mu;
-    const int64_t iq = i * 8 + q, nq_all = A.n * 8;
-    if constexpr (LAW == LAW_ELASTIC) {
-      elastic_stress(lambda, mu, g, -0.0, S);
-    } else if constexpr (LAW == LAW_ELASTIC_FD) {
-      double E[3][3];
-      green_strain(g, E);
-      elastic_stress(lambda, mu, E, -0.0, S);
-    } else if constexpr (LAW == LAW_NEOHOOKEAN) {
-      neohookean_pk2(lambda, mu, g, S);
-    } else {
-      double sy = A.sy_el ? A.sy_el[i] : A.sigma_y;
-      double hh = A.h_el ? A.h_el[i] : A.h;
-      double pg[3][3], ps[3][3], pe[3][3], eps[3][3], iso;
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-        for (int ii = 0; ii < 3; ++ii) {
-          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
-          pg[ii][jj] = A.p_grad[o];
-          ps[ii][jj] = A.p_sig[o];
-          pe[ii][jj] = A.p_eps[o];
-        }
-      plastic_lih(lambda, mu, sy, hh, g, pg, ps, A.p_iso[iq], pe, S, iso, eps);
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-        for (int ii = 0; ii < 3; ++ii) {
-          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
-          A.c_grad[o] = g[ii][jj];
-          A.c_sig[o] = S[ii][jj];
-          A.c_eps[o] = eps[ii][jj];
-        }
-      A.c_iso[iq] = iso;
-    }
-    if (A.grad_u) {
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-        for (int ii = 0; ii < 3; ++ii) {
-          const int64_t o = (int64_t)(ii + 3 * jj) * nq_all + iq;
-          A.grad_u[o] = g[ii][jj];
-          A.stress[o] = S[ii][jj];
-        }
-    }
+    strict_law<LAW, double, HT>(V, iq, lambda, mu, sy, hh, g, S);
   }
   // two halves of four nodes: the force terms of this Gauss point go to LDS,
   // then the lanes q < 4 sum node 4h + q over the Gauss points in order
 #pragma unroll
-  for (int hh = 0; hh < 2; ++hh) {
+  for (int hh2 = 0; hh2 < 2; ++hh2) {
     if (active) {
 #pragma unroll
       for (int nn = 0; nn < 4; ++nn) {
-        const int n = 4 * hh + nn;
+        const int n = 4 * hh2 + nn;
         double b[3], r[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -456,9 +544,8 @@ __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
     }
     __syncthreads();
     if (active && q < 4) {
-      const int n = 4 * hh + q;
-      double *out = reinterpret_cast<double *>(A.fe) +
-                    (int64_t)A.pos[(A.pos_offset + i) * 8 + n] * 3;
+      const int n = 4 * hh2 + q;
+      double *out = fe + (int64_t)pos_e[n] * 3;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         double s = sR[el][0][3 * q + k];
@@ -470,6 +557,35 @@ __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
     }
     __syncthreads();
   }
+}
+
+template <int LAW>
+__global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
+  __shared__ double sN[HEX_STRICT_EPB][8][6];
+  __shared__ double sR[HEX_STRICT_EPB][8][13]; // 12 terms (+1 pad: banks)
+  __shared__ double sD[8][3][8];
+  const int t = threadIdx.x, el = t >> 3, q = t & 7;
+  for (int k = t; k < 192; k += 64)
+    (&sD[0][0][0])[k] = (&c_hex8.dnds[0][0][0])[k];
+  const int64_t i = xcd_block(A.nblocks_padded) * HEX_STRICT_EPB + el;
+  const bool active = i < A.n;
+  int64_t e = 0;
+  double lambda = A.lambda, mu = A.mu, sy = A.sigma_y, hh = A.h;
+  if (active) {
+    e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+    if (A.lam_el) {
+      lambda = A.lam_el[i];
+      mu = A.mu_el[i];
+    }
+    if (A.sy_el)
+      sy = A.sy_el[i];
+    if (A.h_el)
+      hh = A.h_el[i];
+  }
+  hex8_strict_group<LAW, HEX_STRICT_EPB, double>(
+      sN, sR, sD, c_hex8.w[q], el, q, active, A.conn + e * 8, A.X, A.u, A.nn, lambda,
+      mu, sy, hh, qp_view(A, 8), i * 8 + q, A.pos + (A.pos_offset + (active ? i : 0)) * 8,
+      reinterpret_cast<double *>(A.fe));
 }
 
 // ---------------------------------------------------------------------------

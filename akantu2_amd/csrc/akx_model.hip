@@ -8,6 +8,7 @@
 // several steps are queued together.
 #include "../../include/akantu_mi355x.h"
 #include "akx_fast.hpp"
+#include "akx_host.hpp"
 #include "akx_kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -25,56 +26,7 @@ namespace akx {
 
 __constant__ Hex8Tables c_hex8;
 
-// ---------------------------------------------------------------------------
-// Errors: C++ exceptions inside, status codes across the ABI.
-// ---------------------------------------------------------------------------
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-static thread_local std::string g_last_error;
-
-#define HIPCHECK(x)                                                            \
-  do {                                                                         \
-    hipError_t err_ = (x);                                                     \
-    if (err_ != hipSuccess)                                                    \
-      throw Error(err_ == hipErrorOutOfMemory ? AKX_ERR_MEMORY : AKX_ERR_HIP,  \
-                  std::string(#x) + ": " + hipGetErrorString(err_));           \
-  } while (0)
-
-static void require(bool c, const std::string &msg) {
-  if (!c)
-    throw Error(AKX_ERR_INVALID, msg);
-}
-
-// ---------------------------------------------------------------------------
-// Device buffer
-// ---------------------------------------------------------------------------
-template <class T> struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf &) = delete;
-  DevBuf &operator=(const DevBuf &) = delete;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p)
-      (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  void alloc(size_t count) {
-    if (count == n && p)
-      return;
-    release();
-    if (count == 0)
-      return;
-    HIPCHECK(hipMalloc(&p, count * sizeof(T)));
-    n = count;
-  }
-  size_t bytes() const { return n * sizeof(T); }
-};
+thread_local std::string g_last_error;
 
 // ---------------------------------------------------------------------------
 // Layout kernels (array-of-structures at the API, structure-of-arrays on the
@@ -232,65 +184,6 @@ template <int ET> __global__ void k_mass_elements(InitArgs A) {
 #pragma unroll
   for (int n = 0; n < NN; ++n)
     A.me[A.pos[(A.pos_offset + i) * NN + n]] = m[n];
-}
-
-// Inradius: Tet4 2 * Math::tetrahedron_inradius (aka_math_tmpl.hh:71-124),
-// Hex8 min of the 12 edges (element_class_hexahedron_8_inline_impl.hh:194-212)
-__device__ __forceinline__ double dist3(const double (&a)[3], const double (&b)[3]) {
-  double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
-  return sqrt(dx * dx + dy * dy + dz * dz);
-}
-
-__device__ __forceinline__ double tet_volume(const double (&c1)[3],
-                                             const double (&c2)[3],
-                                             const double (&c3)[3],
-                                             const double (&c4)[3]) {
-  double xx[3][3];
-  auto setcol = [&](int k, const double (&c)[3]) {
-    xx[0][k] = c[0];
-    xx[1][k] = c[1];
-    xx[2][k] = c[2];
-  };
-  setcol(0, c2);
-  setcol(1, c3);
-  setcol(2, c4);
-  double vol = det3(xx);
-  setcol(0, c1);
-  vol -= det3(xx);
-  setcol(1, c2);
-  vol += det3(xx);
-  setcol(2, c3);
-  vol -= det3(xx);
-  vol /= 6;
-  return vol;
-}
-
-template <int ET>
-__device__ __forceinline__ double inradius(const double (&X)[Elem<ET>::nnpe][3]) {
-  if constexpr (ET == ET_TET4) {
-    double l12 = dist3(X[0], X[1]), l13 = dist3(X[0], X[2]), l14 = dist3(X[0], X[3]);
-    double l23 = dist3(X[1], X[2]), l24 = dist3(X[1], X[3]), l34 = dist3(X[2], X[3]);
-    double s1 = (l12 + l23 + l13) * 0.5;
-    s1 = sqrt(s1 * (s1 - l12) * (s1 - l23) * (s1 - l13));
-    double s2 = (l12 + l24 + l14) * 0.5;
-    s2 = sqrt(s2 * (s2 - l12) * (s2 - l24) * (s2 - l14));
-    double s3 = (l23 + l34 + l24) * 0.5;
-    s3 = sqrt(s3 * (s3 - l23) * (s3 - l34) * (s3 - l24));
-    double s4 = (l13 + l34 + l14) * 0.5;
-    s4 = sqrt(s4 * (s4 - l13) * (s4 - l34) * (s4 - l14));
-    double volume = tet_volume(X[0], X[1], X[2], X[3]);
-    return 2. * (3 * volume / (s1 + s2 + s3 + s4));
-  } else {
-    const int ed[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
-                           {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}};
-    double p = dist3(X[0], X[1]);
-#pragma unroll
-    for (int k = 1; k < 12; ++k) {
-      double l = dist3(X[ed[k][0]], X[ed[k][1]]);
-      p = (l < p) ? l : p;
-    }
-    return p;
-  }
 }
 
 // min_e h_e / c_e over one material (solid_mechanics_model.cc:423-473)
@@ -458,59 +351,13 @@ inline int64_t padded_blocks(int64_t n) {
 }
 
 void upload_hex8_tables() {
-  // element_class_hexahedron_8_inline_impl.hh:111-188 at the tensor-product
-  // Gauss points (xi fastest), gauss_integration.cc:42-43
   static bool done_device[64] = {false};
   int dev = 0;
   HIPCHECK(hipGetDevice(&dev));
   if (dev < 64 && done_device[dev])
     return;
   Hex8Tables t;
-  double pos[2] = {-1. / std::sqrt(3.), 1. / std::sqrt(3.)};
-  for (int q = 0; q < 8; ++q) {
-    double c[3] = {pos[q & 1], pos[(q >> 1) & 1], pos[(q >> 2) & 1]};
-    double(*d)[8] = t.dnds[q];
-    d[0][0] = -.125 * (1 - c[1]) * (1 - c[2]);
-    d[0][1] = .125 * (1 - c[1]) * (1 - c[2]);
-    d[0][2] = .125 * (1 + c[1]) * (1 - c[2]);
-    d[0][3] = -.125 * (1 + c[1]) * (1 - c[2]);
-    d[0][4] = -.125 * (1 - c[1]) * (1 + c[2]);
-    d[0][5] = .125 * (1 - c[1]) * (1 + c[2]);
-    d[0][6] = .125 * (1 + c[1]) * (1 + c[2]);
-    d[0][7] = -.125 * (1 + c[1]) * (1 + c[2]);
-    d[1][0] = -.125 * (1 - c[0]) * (1 - c[2]);
-    d[1][1] = -.125 * (1 + c[0]) * (1 - c[2]);
-    d[1][2] = .125 * (1 + c[0]) * (1 - c[2]);
-    d[1][3] = .125 * (1 - c[0]) * (1 - c[2]);
-    d[1][4] = -.125 * (1 - c[0]) * (1 + c[2]);
-    d[1][5] = -.125 * (1 + c[0]) * (1 + c[2]);
-    d[1][6] = .125 * (1 + c[0]) * (1 + c[2]);
-    d[1][7] = .125 * (1 - c[0]) * (1 + c[2]);
-    d[2][0] = -.125 * (1 - c[0]) * (1 - c[1]);
-    d[2][1] = -.125 * (1 + c[0]) * (1 - c[1]);
-    d[2][2] = -.125 * (1 + c[0]) * (1 + c[1]);
-    d[2][3] = -.125 * (1 - c[0]) * (1 + c[1]);
-    d[2][4] = .125 * (1 - c[0]) * (1 - c[1]);
-    d[2][5] = .125 * (1 + c[0]) * (1 - c[1]);
-    d[2][6] = .125 * (1 + c[0]) * (1 + c[1]);
-    d[2][7] = .125 * (1 - c[0]) * (1 + c[1]);
-    t.N[q][0] = .125 * (1 - c[0]) * (1 - c[1]) * (1 - c[2]);
-    t.N[q][1] = .125 * (1 + c[0]) * (1 - c[1]) * (1 - c[2]);
-    t.N[q][2] = .125 * (1 + c[0]) * (1 + c[1]) * (1 - c[2]);
-    t.N[q][3] = .125 * (1 - c[0]) * (1 + c[1]) * (1 - c[2]);
-    t.N[q][4] = .125 * (1 - c[0]) * (1 - c[1]) * (1 + c[2]);
-    t.N[q][5] = .125 * (1 + c[0]) * (1 - c[1]) * (1 + c[2]);
-    t.N[q][6] = .125 * (1 + c[0]) * (1 + c[1]) * (1 + c[2]);
-    t.N[q][7] = .125 * (1 - c[0]) * (1 + c[1]) * (1 + c[2]);
-    double wq = 1.;
-    wq *= 1.;
-    wq *= 1.;
-    wq *= 1.;
-    t.w[q] = wq;
-    for (int i = 0; i < 3; ++i)
-      for (int n = 0; n < 8; ++n)
-        t.dnds_f[q][i][n] = (float)d[i][n];
-  }
+  fill_hex8_tables(t);
   HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_hex8), &t, sizeof(t)));
   HIPCHECK(akxf::upload_hex8_dnds(&t.dnds[0][0][0]));
   if (dev < 64)
@@ -862,25 +709,6 @@ void alloc_internals(akx_model *m, Material &mat) {
       zero_async(m, mat.hiso[s].p, nq);
     }
     mat.cur = 0;
-  }
-}
-
-template <class F> int guard(F &&f) {
-  try {
-    f();
-    return AKX_OK;
-  } catch (const Error &e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    g_last_error = "host allocation failed";
-    return AKX_ERR_MEMORY;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return AKX_ERR_INVALID;
-  } catch (...) {
-    g_last_error = "unknown error";
-    return AKX_ERR_INVALID;
   }
 }
 

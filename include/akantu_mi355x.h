@@ -167,6 +167,80 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes);
  * byte counters and reports the measured copy bandwidth. */
 int akx_calibration_copy(int device, int64_t bytes, int repeats, double *ms);
 
+/* Batches ------------------------------------------------------------------
+ * An `akx_batch` holds many independent instances (each one what the
+ * reference runs as its own SolidMechanicsModel with one material, explicit
+ * lumped mass and zero external force) in device arenas sized at creation,
+ * and advances all live instances together: per step one persistent element
+ * launch per (element type, law) group present and one persistent nodal
+ * gather + update launch, captured as a hipGraph of two steps.  The
+ * reference has no batch object; per instance the calls stand for
+ *   SolidMechanicsModel(mesh) + addMaterial + initFull   -> add_instance
+ *   getStableTimeStep / setTimeStep                      -> get/set_time_step
+ *   solveStep (nb_steps times, every live instance)      -> solve_step
+ *   getDisplacement / getVelocity / getAcceleration /
+ *   getInternalForce                                     -> get_field
+ * with the same arithmetic as akx_model (FP64: the reference's operation
+ * order; FP32: the precision form).  An instance's results do not depend on
+ * which other instances share the batch (per-instance arithmetic, per-node
+ * gather in fixed order). */
+typedef struct akx_batch akx_batch;
+
+/* precision: AKX_PRECISION_FP64 or AKX_PRECISION_FP32.  Capacities: nodes
+ * (each instance takes its node count rounded up to 256), Tet4 and Hex8
+ * elements; with_plasticity reserves the plastic history (ping-pong). */
+int akx_batch_create(int device, int precision, int64_t node_capacity,
+                     int64_t tet_capacity, int64_t hex_capacity,
+                     int with_plasticity, akx_batch **out);
+int akx_batch_destroy(akx_batch *b);
+/* Device bytes one instance takes (admission by HBM footprint). */
+int akx_batch_footprint(int precision, int element_type, int law,
+                        int64_t nb_nodes, int64_t nb_elements, int64_t *bytes);
+/* Add an instance: mesh arrays as in akx_model_set_mesh, params = {rho, E,
+ * nu, sigma_y, h}, blocked_dofs [nb_nodes][3] uint8 (NULL: none), initial
+ * velocity [nb_nodes][3] (NULL: zero), dt = dt_factor * stable time step
+ * (computed here).  AKX_ERR_MEMORY when an arena has no room (nothing is
+ * kept), AKX_ERR_JACOBIAN for an inverted element.  Instance ids of removed
+ * instances are reused.  Calls between solve_step calls only. */
+int akx_batch_add_instance(akx_batch *b, int element_type, int law,
+                           const double *params, int64_t nb_nodes,
+                           const double *nodes, int64_t nb_elements,
+                           const int32_t *connectivity,
+                           const uint8_t *blocked_dofs, const double *velocity,
+                           double dt_factor, int *instance_id);
+int akx_batch_remove_instance(akx_batch *b, int instance_id);
+int akx_batch_get_time_step(akx_batch *b, int instance_id, double *dt,
+                            double *dt_crit);
+int akx_batch_set_time_step(akx_batch *b, int instance_id, double dt);
+/* Steps the instance has done since it was added. */
+int akx_batch_get_steps(akx_batch *b, int instance_id, int64_t *steps);
+/* nb_steps steps of every live instance (queued on the batch's stream). */
+int akx_batch_solve_step(akx_batch *b, int64_t nb_steps);
+int akx_batch_synchronize(akx_batch *b);
+/* Nodal field of one instance, [nb_nodes][3] doubles: displacement,
+ * velocity, acceleration, internal force or nodes. */
+int akx_batch_get_field(akx_batch *b, int instance_id, int field, double *host);
+/* Launch shape: threads per block (64, 128, 256; Hex8 uses 8 lanes per
+ * element), elements per tile, persistent blocks per CU, hipGraph on/off. */
+int akx_batch_set_launch(akx_batch *b, int block, int tile, int blocks_per_cu,
+                         int use_graph);
+int akx_batch_get_launch(akx_batch *b, int *block, int *tile,
+                         int *blocks_per_cu, int *use_graph);
+/* Time the element pass of the live instances over the launch-shape
+ * candidates (repeats launches each) and keep the fastest. */
+int akx_batch_autotune(akx_batch *b, int repeats, int *block, int *tile,
+                       int *blocks_per_cu, double *best_ms);
+int akx_batch_get_stats(akx_batch *b, int64_t *live, int64_t *elements,
+                        int64_t *quadrature_points, int64_t *nodes);
+/* Largest free range of each arena (nodes, Tet4, Hex8, force-table entries). */
+int akx_batch_capacity_left(akx_batch *b, int64_t *nodes, int64_t *tets,
+                            int64_t *hexes, int64_t *entries);
+/* Per-launch HIP-event timing (disables the graph while on); classes
+ * 0 = element pass (all groups of one step), 1 = gather + update. */
+int akx_batch_enable_kernel_timing(akx_batch *b, int enable);
+int akx_batch_get_kernel_timing(akx_batch *b, int kernel_class,
+                                double *total_ms, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif
