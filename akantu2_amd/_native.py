@@ -98,6 +98,7 @@ def _declare(lib):
                                               C.POINTER(_i64), C.POINTER(_i64)]),
         "akx_batch_enable_kernel_timing": (C.c_int, [_vp, C.c_int]),
         "akx_batch_get_kernel_timing": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(_i64)]),
+        "akx_batch_get_kernel_bytes": (C.c_int, [_vp, C.c_int, _dp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -168,7 +168,14 @@ class Batch:
         N.check(N.lib().akx_batch_enable_kernel_timing(self._h, int(enable)))
 
     def kernelTiming(self, kernel_class):
+        """(total ms, launches) of kernel class 0 (element pass, all groups of
+        one step) or 1 (gather + update) since timing was enabled."""
         ms, n = C.c_double(), C.c_int64()
         N.check(N.lib().akx_batch_get_kernel_timing(self._h, int(kernel_class),
                                                     C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def kernelBytes(self, kernel_class):
+        b = C.c_double()
+        N.check(N.lib().akx_batch_get_kernel_bytes(self._h, int(kernel_class), C.byref(b)))
+        return b.value

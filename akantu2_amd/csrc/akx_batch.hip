@@ -9,7 +9,9 @@
 //            an instance owns a node range aligned to 256 (whole ELL slices)
 //   elements AoS connectivity and table positions per element type
 //   table    node-major sliced-ELL elemental forces (FP64 or FP32)
-//   history  plastic ping-pong [2][9][qp_cap] (FP64, or FP32 in the FP32 form)
+//   history  plastic ping-pong: FP64 form grad u, stress, plastic strain
+//            [2][9][qp_cap] and hardening; FP32 form the compact state,
+//            plastic strain [2][6][qp_cap] and hardening, FP32
 // Free lists (first fit) hand out and take back ranges.
 //
 // Per step: element pass (one persistent launch per (element type, law)
@@ -472,6 +474,13 @@ struct akx_batch {
   size_t fe_size() const { return fp32() ? 4 : 8; }
   size_t hist_size() const { return fp32() ? 4 : 8; }
   int64_t hist_stride() const { return tcap + 8 * hexcap; }
+  // components of history slot k (grad u, stress, plastic strain, hardening):
+  // the FP32 form keeps only the plastic strain (Voigt) and the hardening
+  int hist_comps(int k) const {
+    if (fp32())
+      return k == 2 ? 6 : (k == 3 ? 1 : 0);
+    return k == 3 ? 1 : 9;
+  }
 };
 
 namespace {
@@ -593,7 +602,8 @@ void element_pass(akx_batch *b, int parity) {
     if (law == AKX_LAW_PLASTIC_LIH)
       for (int s = 0; s < 2; ++s)
         for (int k = 0; k < 4; ++k)
-          require(A.hist[s][k] != nullptr, "plastic history not allocated");
+          require(b->hist_comps(k) == 0 || A.hist[s][k] != nullptr,
+                  "plastic history not allocated");
     if (b->fp32()) {
       HIPCHECK(akxf::launch_batch_elements_f32(etype, law, b->block, A, b->stream));
     } else {
@@ -752,8 +762,8 @@ int64_t footprint_bytes(int precision, int etype, int law, int64_t nn, int64_t n
   double bytes = (double)nn_pad * (5 * 3 * 8 + 8 + 1 + 1 + (f32 ? 2 * 3 * 4 : 0)) +
                  (double)nn_pad / 256 * 8;
   bytes += (double)nel * nnpe * (4 + 4 + 1.25 * 3 * sf);
-  if (law == AKX_LAW_PLASTIC_LIH)
-    bytes += (double)nel * nqp * 2 * 28 * sf;
+  if (law == AKX_LAW_PLASTIC_LIH) // two buffers of the history
+    bytes += (double)nel * nqp * 2 * (f32 ? 7 * 4 : 28 * 8);
   return (int64_t)bytes;
 }
 
@@ -833,7 +843,8 @@ int akx_batch_create(int device, int precision, int64_t node_capacity,
     if (b->plastic)
       for (int s = 0; s < 2; ++s)
         for (int k = 0; k < 4; ++k)
-          b->hist[s][k].alloc((k == 3 ? 1 : 9) * b->hist_stride() * b->hist_size());
+          if (b->hist_comps(k) > 0)
+            b->hist[s][k].alloc(b->hist_comps(k) * b->hist_stride() * b->hist_size());
     const std::vector<double> tab = hex8_table();
     b->hex_tab.alloc(tab.size());
     HIPCHECK(hipMemcpy(b->hex_tab.p, tab.data(), tab.size() * sizeof(double),
@@ -883,6 +894,11 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
     require(dt_factor > 0 && std::isfinite(dt_factor), "dt_factor must be positive");
     require(params[0] > 0 && params[1] > 0 && params[2] > -1 && params[2] < .5,
             "need rho > 0, E > 0, -1 < nu < 0.5");
+    for (int64_t i = 0; i < 3 * nb_nodes; ++i)
+      require(std::isfinite(nodes[i]), "non-finite node coordinate");
+    if (velocity)
+      for (int64_t i = 0; i < 3 * nb_nodes; ++i)
+        require(std::isfinite(velocity[i]), "non-finite initial velocity");
     HIPCHECK(hipSetDevice(b->device));
     const int nnpe = element_type == AKX_TET4 ? 4 : 8;
     const int nqp = element_type == AKX_TET4 ? 1 : 8;
@@ -1003,7 +1019,7 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
         const int64_t nq = I.nel * nqp;
         for (int s = 0; s < 2; ++s)
           for (int k = 0; k < 4; ++k)
-            for (int c = 0; c < (k == 3 ? 1 : 9); ++c) {
+            for (int c = 0; c < b->hist_comps(k); ++c) {
               if (b->fp32())
                 hipLaunchKernelGGL((k_batch_fill<float>), dim3(gridn(nq)), dim3(TPB), 0,
                                    b->stream, (float *)b->hist[s][k].p + c * stride + q0,
@@ -1302,6 +1318,36 @@ int akx_batch_capacity_left(akx_batch *b, int64_t *nodes, int64_t *tets, int64_t
       *hexes = b->fl_hex.largest();
     if (entries)
       *entries = b->fl_ell.largest();
+  });
+}
+
+// Unique bytes one launch of each kernel class moves over the live instances:
+// element pass = connectivity + table positions + the elemental forces
+// written + X and u of every node once (+ plastic history read and written);
+// gather = the forces read (one entry per element corner) + per node degree,
+// blocked bits, mass, f_int written and u, v, a (+ compensation) read and
+// written.
+int akx_batch_get_kernel_bytes(akx_batch *b, int kernel_class, double *bytes) {
+  return guard([&] {
+    require(b != nullptr && bytes != nullptr && kernel_class >= 0 && kernel_class < 2,
+            "bad arguments");
+    const double fs = (double)b->fe_size(), hs = (double)b->hist_size();
+    double total = 0;
+    for (auto &I : b->inst) {
+      if (!I.live)
+        continue;
+      const double nnpe = I.etype == AKX_TET4 ? 4 : 8, nqp = I.etype == AKX_TET4 ? 1 : 8;
+      const double corners = (double)I.nel * nnpe, nn = (double)I.nn;
+      if (kernel_class == 0) {
+        total += corners * (4 + 4 + 3 * fs) + nn * 2 * 3 * 8;
+        if (I.law == AKX_LAW_PLASTIC_LIH) // history read and written
+          total += (double)I.nel * nqp * 2 * (b->fp32() ? 7 : 28) * hs;
+      } else {
+        total += corners * 3 * fs + nn * (1 + 1 + 8 + 3 * 8 + 3 * 3 * 2 * 8 +
+                                          (b->fp32() ? 2 * 3 * 2 * 4 : 0));
+      }
+    }
+    *bytes = total;
   });
 }
 

@@ -46,9 +46,11 @@ struct BatchElemArgs {
   const double *X, *u; // [3][ncap]
   int64_t ncap;
   void *fe;            // [entry][3] elemental forces (FP64 or FP32)
-  // plastic history, two buffers (ping-pong by step parity) of
+  // plastic history, two buffers (ping-pong by step parity).  FP64 form:
   // grad u, stress, plastic strain ([9][hcap]) and hardening ([hcap]);
-  // Gauss point q of element e is index hoff + e * nqp + q
+  // FP32 form: plastic strain in Voigt order ([6][hcap], in the third
+  // slot) and hardening ([hcap]).  Gauss point q of element e is index
+  // hoff + e * nqp + q.
   void *hist[2][4];
   int64_t hcap, hoff;
   const double *hex_tab; // Hex8: dN/dxi [8][3][8], N [8][8], w [8]
@@ -77,6 +79,20 @@ __device__ __forceinline__ QpViewT<HT> batch_view(const BatchElemArgs &A, int pa
                      (HT *)A.hist[par ^ 1][2],   (HT *)A.hist[par ^ 1][3],
                      nullptr,                    nullptr,
                      A.hcap};
+}
+
+// the compact FP32 plastic state of one step (read `par`, write par ^ 1)
+struct QpCompact {
+  const float *p_eps, *p_iso;
+  float *c_eps, *c_iso;
+  double *grad_u, *stress; // not materialised: always null
+  int64_t nq_all;
+};
+
+__device__ __forceinline__ QpCompact batch_compact_view(const BatchElemArgs &A, int par) {
+  return QpCompact{(const float *)A.hist[par][2], (const float *)A.hist[par][3],
+                   (float *)A.hist[par ^ 1][2], (float *)A.hist[par ^ 1][3], nullptr,
+                   nullptr, A.hcap};
 }
 
 } // namespace akx

@@ -33,9 +33,10 @@ using akx::BatchElemArgs;
 using akx::BInst;
 using akx::BTile;
 using akx::ElemArgs;
+using akx::QpCompact;
 using akx::QpViewT;
 using akx::batch_chunk;
-using akx::batch_view;
+using akx::batch_compact_view;
 using akx::qp_view;
 
 constexpr int LAW_ELASTIC = 0, LAW_ELASTIC_FD = 1, LAW_NEOHOOKEAN = 2,
@@ -70,10 +71,56 @@ __device__ __forceinline__ R inv_det(const R (&m)[3][3], R (&r)[3][3]) {
   return det;
 }
 
+// Linear isotropic hardening with the full incremental history of the
+// reference (grad u, stress, plastic strain, hardening).
+template <class CT, class HT>
+__device__ __forceinline__ void plastic_at(const QpViewT<HT> &V, int64_t iq, CT lambda,
+                                           CT mu, CT sy, CT hh, const CT (&g)[3][3],
+                                           CT (&S)[3][3]) {
+  CT pg[3][3], ps[3][3], pe[3][3], eps[3][3], iso;
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+      const int64_t o = (int64_t)(ii + 3 * jj) * V.nq_all + iq;
+      pg[ii][jj] = CT(V.p_grad[o]);
+      ps[ii][jj] = CT(V.p_sig[o]);
+      pe[ii][jj] = CT(V.p_eps[o]);
+    }
+  plastic_lih(lambda, mu, sy, hh, g, pg, ps, CT(V.p_iso[iq]), pe, S, iso, eps);
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+      const int64_t o = (int64_t)(ii + 3 * jj) * V.nq_all + iq;
+      V.c_grad[o] = HT(g[ii][jj]);
+      V.c_sig[o] = HT(S[ii][jj]);
+      V.c_eps[o] = HT(eps[ii][jj]);
+    }
+  V.c_iso[iq] = HT(iso);
+}
+
+// The total-strain form with the compact state (plastic strain in Voigt
+// order and hardening, FP32): the batched FP32 form.
+template <class CT>
+__device__ __forceinline__ void plastic_at(const QpCompact &V, int64_t iq, CT lambda,
+                                           CT mu, CT sy, CT hh, const CT (&g)[3][3],
+                                           CT (&S)[3][3]) {
+  CT pe[6], pn[6], iso;
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    pe[k] = CT(V.p_eps[(int64_t)k * V.nq_all + iq]);
+  plastic_lih_total(lambda, mu, sy, hh, g, pe, CT(V.p_iso[iq]), S, pn, iso);
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    V.c_eps[(int64_t)k * V.nq_all + iq] = float(pn[k]);
+  V.c_iso[iq] = float(iso);
+}
+
 // Constitutive law at one Gauss point: g -> stress S (sigma or PK2) and the
 // tensor P contracted with dN/dX (sigma, or F S for finite deformation).
-template <int LAW, class CT, class HT>
-__device__ __forceinline__ void law_at(const QpViewT<HT> &V, int64_t iq, CT lambda,
+template <int LAW, class CT, class VT>
+__device__ __forceinline__ void law_at(const VT &V, int64_t iq, CT lambda,
                                        CT mu, CT sy, CT hh, const CT (&g)[3][3],
                                        CT (&S)[3][3], CT (&P)[3][3]) {
   if constexpr (LAW == LAW_ELASTIC) {
@@ -88,27 +135,7 @@ __device__ __forceinline__ void law_at(const QpViewT<HT> &V, int64_t iq, CT lamb
     else
       neohookean_pk2(lambda, mu, g, S);
   } else {
-    CT pg[3][3], ps[3][3], pe[3][3], eps[3][3], iso;
-#pragma unroll
-    for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-      for (int ii = 0; ii < 3; ++ii) {
-        const int64_t o = (int64_t)(ii + 3 * jj) * V.nq_all + iq;
-        pg[ii][jj] = CT(V.p_grad[o]);
-        ps[ii][jj] = CT(V.p_sig[o]);
-        pe[ii][jj] = CT(V.p_eps[o]);
-      }
-    plastic_lih(lambda, mu, sy, hh, g, pg, ps, CT(V.p_iso[iq]), pe, S, iso, eps);
-#pragma unroll
-    for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-      for (int ii = 0; ii < 3; ++ii) {
-        const int64_t o = (int64_t)(ii + 3 * jj) * V.nq_all + iq;
-        V.c_grad[o] = HT(g[ii][jj]);
-        V.c_sig[o] = HT(S[ii][jj]);
-        V.c_eps[o] = HT(eps[ii][jj]);
-      }
-    V.c_iso[iq] = HT(iso);
+    plastic_at(V, iq, lambda, mu, sy, hh, g, S);
   }
   if (V.grad_u) {
 #pragma unroll
@@ -157,13 +184,13 @@ __device__ __forceinline__ void element_params(const ElemArgs &A, int64_t i,
 // ---------------------------------------------------------------------------
 constexpr int HEX_EPB = 32; // elements per 256-thread block
 
-template <int LAW, class CT, class FT, class HT, int EPB>
+template <int LAW, class CT, class FT, class VT, int EPB>
 __device__ __forceinline__ void hex8_group(double (&sN)[EPB][8][6], CT (&sQ)[EPB][8][9],
                                            const CT (&sD)[8][3][8], int el, int q,
                                            bool active, const int32_t *conn_e,
                                            const double *Xs, const double *us,
                                            int64_t stride, CT lambda, CT mu, CT sy,
-                                           CT hh, const QpViewT<HT> &V, int64_t iq,
+                                           CT hh, const VT &V, int64_t iq,
                                            const int32_t *pos_e, FT *fe) {
   if (active) {
     const int32_t node = conn_e[q];
@@ -226,7 +253,7 @@ __device__ __forceinline__ void hex8_group(double (&sN)[EPB][8][6], CT (&sQ)[EPB
         g[a][b] = s;
       }
     CT S[3][3], P[3][3];
-    law_at<LAW, CT, HT>(V, iq, lambda, mu, sy, hh, g, S, P);
+    law_at<LAW, CT>(V, iq, lambda, mu, sy, hh, g, S, P);
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -274,7 +301,7 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
     e = A.filter ? (int64_t)A.filter[i] : A.first + i;
     element_params(A, i, lambda, mu, sy, hh);
   }
-  hex8_group<LAW, CT, FT, double, HEX_EPB>(
+  hex8_group<LAW, CT, FT, QpViewT<double>, HEX_EPB>(
       sN, sQ, sD, el, q, active, A.conn + e * 8, A.X, A.u, A.nn, lambda, mu, sy, hh,
       qp_view(A, 8), i * 8 + q, A.pos + (A.pos_offset + (active ? i : 0)) * 8,
       reinterpret_cast<FT *>(A.fe));
@@ -283,10 +310,10 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
 // ---------------------------------------------------------------------------
 // Tet4: one thread per element
 // ---------------------------------------------------------------------------
-template <int LAW, class CT, class FT, class HT>
+template <int LAW, class CT, class FT, class VT>
 __device__ __forceinline__ void tet4_one(const int32_t *conn_e, const double *Xs,
                                          const double *us, int64_t stride, CT lambda,
-                                         CT mu, CT sy, CT hh, const QpViewT<HT> &V,
+                                         CT mu, CT sy, CT hh, const VT &V,
                                          int64_t iq, const int32_t *pos_e, FT *fe) {
   const int4 nd = *reinterpret_cast<const int4 *>(conn_e);
   const int32_t nodes[4] = {nd.x, nd.y, nd.z, nd.w};
@@ -323,7 +350,7 @@ __device__ __forceinline__ void tet4_one(const int32_t *conn_e, const double *Xs
       g[a][b] = s;
     }
   CT S[3][3], P[3][3];
-  law_at<LAW, CT, HT>(V, iq, lambda, mu, sy, hh, g, S, P);
+  law_at<LAW, CT>(V, iq, lambda, mu, sy, hh, g, S, P);
   CT Q[3][3];
 #pragma unroll
   for (int k = 0; k < 3; ++k)
@@ -353,14 +380,15 @@ __global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
   const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
   CT lambda, mu, sy, hh;
   element_params(A, i, lambda, mu, sy, hh);
-  tet4_one<LAW, CT, FT, double>(A.conn + e * 4, A.X, A.u, A.nn, lambda, mu, sy, hh,
+  tet4_one<LAW, CT, FT, QpViewT<double>>(A.conn + e * 4, A.X, A.u, A.nn, lambda, mu, sy, hh,
                                 qp_view(A, 1), i, A.pos + (A.pos_offset + i) * 4,
                                 reinterpret_cast<FT *>(A.fe));
 }
 
 // ---------------------------------------------------------------------------
 // Batched FP32 form: persistent over the tiles of one (element type, law)
-// group; per-instance parameters, FP32 history (ping-pong by step parity).
+// group; per-instance parameters, compact FP32 plastic state (ping-pong by
+// step parity).
 // ---------------------------------------------------------------------------
 template <int LAW>
 __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
@@ -369,10 +397,10 @@ __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
   for (int64_t tt = t0; tt < t1; ++tt) {
     const BTile T = A.tiles[tt];
     const BInst P = A.inst[T.inst];
-    const QpViewT<float> V = batch_view<float>(A, (P.parity_base ^ A.parity) & 1);
+    const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
     for (int j = threadIdx.x; j < T.count; j += blockDim.x) {
       const int64_t e = T.first + j;
-      tet4_one<LAW, float, float, float>(A.conn + e * 4, A.X, A.u, A.ncap,
+      tet4_one<LAW, float, float, QpCompact>(A.conn + e * 4, A.X, A.u, A.ncap,
                                          float(P.lambda), float(P.mu), float(P.sigma_y),
                                          float(P.h), V, A.hoff + e, A.pos + e * 4,
                                          reinterpret_cast<float *>(A.fe));
@@ -394,11 +422,11 @@ __global__ void __launch_bounds__(EPB * 8) k_batch_hex8_f32(BatchElemArgs A) {
   for (int64_t tt = t0; tt < t1; ++tt) {
     const BTile T = A.tiles[tt];
     const BInst P = A.inst[T.inst];
-    const QpViewT<float> V = batch_view<float>(A, (P.parity_base ^ A.parity) & 1);
+    const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
     for (int base = 0; base < T.count; base += EPB) {
       const bool active = base + el < T.count;
       const int64_t e = T.first + (active ? base + el : 0);
-      hex8_group<LAW, float, float, float, EPB>(
+      hex8_group<LAW, float, float, QpCompact, EPB>(
           sN, sQ, sD, el, q, active, A.conn + e * 8, A.X, A.u, A.ncap, float(P.lambda),
           float(P.mu), float(P.sigma_y), float(P.h), V, A.hoff + e * 8 + q,
           A.pos + e * 8, reinterpret_cast<float *>(A.fe));

@@ -344,4 +344,59 @@ AKX_HD void plastic_lih(R lambda, R mu, R sigma_y, R h, const R (&g)[3][3],
       sigma[i][j] = ps[i][j] + sel[i][j];
 }
 
+// The same radial return written on the total strain, for the FP32 form.
+// The reference's update is incremental (sigma = sigma_prev + C : (d eps -
+// d eps_p)); from a zero initial state it telescopes to
+// sigma = C : (eps - eps_p), so the state reduces to the plastic strain eps_p
+// (symmetric, Voigt order xx yy zz yz xz xy) and the hardening: the trial
+// stress is C : (eps - eps_p_prev), and since d eps_p is deviatoric,
+// sigma = trial - 2 mu d eps_p.  7 numbers per Gauss point instead of 28,
+// and no accumulation of the stress in FP32.
+template <class R>
+AKX_HD void plastic_lih_total(R lambda, R mu, R sigma_y, R h, const R (&g)[3][3],
+                              const R (&pe)[6], R piso, R (&sigma)[3][3],
+                              R (&pe_new)[6], R &iso) {
+  const int vi[6] = {0, 1, 2, 1, 0, 0}, vj[6] = {0, 1, 2, 2, 2, 1};
+  R e[3][3];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int i = vi[k], j = vj[k];
+    const R x = R(.5) * (g[i][j] + g[j][i]) - pe[k];
+    e[i][j] = x;
+    e[j][i] = x;
+  }
+  R tr = e[0][0] + e[1][1] + e[2][2];
+  R two_mu = R(2) * mu;
+  R str[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      str[i][j] = (i == j) ? two_mu * e[i][j] + lambda * tr : two_mu * e[i][j];
+  R trs = str[0][0] + str[1][1] + str[2][2];
+  R dev[3][3], s = R(0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      dev[i][j] = (i == j) ? str[i][j] - trs / R(3) : str[i][j];
+      s += dev[i][j] * dev[i][j];
+    }
+  R eff = akx_sqrt(R(3) / R(2) * s);
+  bool yielding = ((eff - piso - sigma_y) > R(0));
+  R dp = yielding ? (eff - sigma_y - piso) / (R(3) * mu + h) : R(0);
+  iso = piso + h * dp;
+  R tol = sizeof(R) == 8 ? R(1e2 * 2.220446049250313e-16) : R(1e2 * 1.1920929e-07f);
+  bool flow = akx_abs(eff) > akx_sqrt(s) * tol;
+  R c = flow ? R(3) / R(2) * dp / eff : R(0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      sigma[i][j] = str[i][j] - two_mu * (c * dev[i][j]);
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    pe_new[k] = pe[k] + c * dev[vi[k]][vj[k]];
+}
+
 } // namespace AKX_MATH_NS

@@ -134,6 +134,17 @@ def box_mesh(etype, nx, ny, nz, h=(1.0, 1.0, 1.0), jitter=0.0, seed=0):
                    {"jitter": jitter, "seed": seed})
 
 
+def cells_for(etype, nb_elements, aspect=1.0):
+    """Cells (nx, ny, nz) of a box with about ``nb_elements`` elements and
+    x : y : z = aspect : 1 : 1."""
+    if isinstance(etype, str):
+        etype = ELEMENT_TYPES[etype]
+    cells = nb_elements / (6 if etype == TET4 else 1)
+    ny = max(1, int(round((cells / aspect) ** (1.0 / 3.0))))
+    nx = max(1, int(round(cells / (ny * ny))))
+    return nx, ny, ny
+
+
 def clamped_face_x0(mesh: BoxMesh) -> np.ndarray:
     """Blocked DOFs [nn][3] (uint8): the x = 0 face clamped in x, y and z."""
     blocked = np.zeros((mesh.nb_nodes, 3), dtype=np.uint8)

@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the explicit internal-force step on one MI355X.
 
-Workload (the single-GPU configuration the headline metric is quoted on):
+Default (``--form plain``), the workload the headline metric is quoted on:
 200 x 100 x 100 structured _hexahedron_8 grid (2,000,000 elements, 8 Gauss
 points each = 16,000,000 element quadrature points), interior nodes jittered
 by U(-0.1, 0.1) h (seed 1), compressible neo-Hookean E = 1, nu = 0.3,
@@ -13,7 +13,19 @@ gradient -> neo-Hookean PK2 -> (B_L + B_NL)^T S -> integrate -> gather
 assembly, residual, lumped solve, corrector).  Inputs are resident in HBM
 before the timed region.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-baseline/--no-cpu-baseline]
+Other forms (each prints its own line):
+  --form precision  1000 x 100 x 100 cubes x 6 Tet4 (60 M), per-element
+                    E ~ logU(0.5, 2), nu ~ U(0.2, 0.35), linear elastic:
+                    FP32 form timed, plain FP64 timed beside it
+  --form batch      512 independent Tet4 boxes, 5e3-4e5 elements
+                    (log-uniform), aspect U(1, 8), linear isotropic hardening
+                    with per-instance parameters, FP32 form, one persistent
+                    launch per substep in a hipGraph, autotuned; FP64 batch
+                    timed beside it
+  --form serve      Poisson stream of solve requests served by continuous
+                    batching (akantu2_amd.serving)
+
+    python bench.py [--form F] [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -39,6 +51,7 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix, FMA = 2 flops)
 HEX8_NEOHOOKEAN_FLOP_PER_QP = 1363
 SHAPE = (200, 100, 100)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_hex8_neohookean_f64.json")
+PL = "plastic_linear_isotropic_hardening"
 
 
 def build_c2(shape=SHAPE):
@@ -49,7 +62,26 @@ def build_c2(shape=SHAPE):
     return mesh, blocked, v0
 
 
-def cpu_baseline(shape=(20, 100, 100), steps=10):
+def copy_bandwidth():
+    from akantu2_amd import _native
+    copy_bytes = 1 << 30
+    copy_ms = _native.calibration_copy(copy_bytes, repeats=20)
+    return 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+
+
+def rel_errors(a, b):
+    """max-norm and L2 relative error of a against the reference b."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    d = a - b
+    mx = float(np.abs(d).max() / max(np.abs(b).max(), 1e-300))
+    l2 = float(np.linalg.norm(d) / max(np.linalg.norm(b), 1e-300))
+    return {"max": mx, "l2": l2}
+
+
+# --------------------------------------------------------------------------
+# plain form (headline)
+# --------------------------------------------------------------------------
+def cpu_baseline_plain(shape=(20, 100, 100), steps=10):
     """The CPU oracle (plain C, one thread, FP64, reference dataflow with
     precomputed shape derivatives) timed on a bounded sample of the same
     workload: the first 20 cells in x of the 200 x 100 x 100 grid."""
@@ -71,27 +103,16 @@ def cpu_baseline(shape=(20, 100, 100), steps=10):
             "seconds": el}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--shape", type=str, default=None,
-                    help="override cells per direction, e.g. 20,10,10")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is designed, not built)"}))
-        sys.exit(2)
-
+def run_plain(args):
     from akantu2_amd import SolidMechanicsModel
 
+    steps = args.steps or 1000
     shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else SHAPE
     t = time.perf_counter()
     mesh, blocked, v0 = build_c2(shape)
     model = SolidMechanicsModel(mesh)
     model.addMaterial("neohookean", rho=1.0, E=1.0, nu=0.3)
+    model.setPrecision(args.precision)
     model.initFull()
     model.getBlockedDOFs()[:] = blocked
     model.getVelocity()[:] = v0
@@ -99,19 +120,20 @@ def main():
     model.setTimeStep(dt)
     setup_s = time.perf_counter() - t
     print(f"# setup {setup_s:.1f} s, {mesh.nb_elements} elements, "
-          f"{mesh.nb_nodes} nodes, dt {dt:.6g}", file=sys.stderr, flush=True)
+          f"{mesh.nb_nodes} nodes, dt {dt:.6g}, {args.precision}", file=sys.stderr,
+          flush=True)
 
     model.solveStep(args.warmup)
     model.synchronize()
     model.enableKernelTiming(True)
     model.synchronize()
     t0 = time.perf_counter()
-    model.solveStep(args.steps)
+    model.solveStep(steps)
     model.synchronize()
     elapsed = time.perf_counter() - t0
     nqp = mesh.nb_elements * 8
-    value = nqp * args.steps / elapsed
-    ms_step = 1e3 * elapsed / args.steps
+    value = nqp * steps / elapsed
+    ms_step = 1e3 * elapsed / steps
 
     kern = {}
     names = {0: "element_pass", 1: "gather_update", 2: "predictor"}
@@ -120,35 +142,40 @@ def main():
         if n:
             kern[name] = {"avg_ms": ms / n, "launches": n, "bytes": model.kernelBytes(k)}
     el = kern["element_pass"]
-    flops = HEX8_NEOHOOKEAN_FLOP_PER_QP * nqp
-    tflops = flops / (el["avg_ms"] * 1e-3) / 1e12
     hbm_gbs = el["bytes"] / (el["avg_ms"] * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as fh:
-            tr = json.load(fh)
-        if tr.get("shape") == list(shape):
-            traffic = tr.get("element_pass_bytes")
-    from akantu2_amd import _native
-    copy_bytes = 1 << 30
-    copy_ms = _native.calibration_copy(copy_bytes, repeats=20)
-    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
-    # The dominant kernel (the element pass, ~85 % of the step) is bound by
-    # FP64 arithmetic, not by HBM: it recomputes dN/dX and w detJ from the
-    # node coordinates every step instead of streaming them.  MI355X's FP64
-    # peak is the same 78.6 TFLOP/s on the matrix cores and the vector ALUs;
-    # the reference operation order is FMA-free, so its own ceiling is half.
-    roofline = {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
-                "traffic": traffic,
-                "kernel": "k_hex8_strict<neo-Hookean> (FP64, reference operation order)",
-                "compute": "FP64 VALU, FMA-free by construction (ceiling 39.3 TFLOP/s)",
-                "algorithmic_flops_per_launch": flops,
-                "algorithmic_bytes_per_launch": el["bytes"],
-                "avg_kernel_ms": el["avg_ms"],
-                "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": hbm_gbs / HBM_PEAK_GBS,
-                        "measured_copy_gbs": copy_gbs}}
+    copy_gbs = copy_bandwidth()
+    if args.precision == "fp64":
+        flops = HEX8_NEOHOOKEAN_FLOP_PER_QP * nqp
+        tflops = flops / (el["avg_ms"] * 1e-3) / 1e12
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            with open(TRAFFIC_FILE) as fh:
+                tr = json.load(fh)
+            if tr.get("shape") == list(shape):
+                traffic = tr.get("element_pass_bytes")
+        # The dominant kernel (the element pass, ~85 % of the step) is bound
+        # by FP64 arithmetic, not by HBM: it recomputes dN/dX and w detJ from
+        # the node coordinates every step instead of streaming them.  MI355X's
+        # FP64 peak is the same 78.6 TFLOP/s on the matrix cores and the
+        # vector ALUs; the reference operation order is FMA-free, so its own
+        # ceiling is half.
+        roofline = {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
+                    "traffic": traffic,
+                    "kernel": "k_hex8_strict<neo-Hookean> (FP64, reference operation order)",
+                    "compute": "FP64 VALU, FMA-free by construction (ceiling 39.3 TFLOP/s)",
+                    "algorithmic_flops_per_launch": flops,
+                    "algorithmic_bytes_per_launch": el["bytes"],
+                    "avg_kernel_ms": el["avg_ms"],
+                    "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": hbm_gbs / HBM_PEAK_GBS,
+                            "measured_copy_gbs": copy_gbs}}
+    else:
+        roofline = {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": None,
+                    "kernel": f"k_hex8_fast<neo-Hookean> ({args.precision})",
+                    "algorithmic_bytes_per_launch": el["bytes"],
+                    "avg_kernel_ms": el["avg_ms"], "measured_copy_gbs": copy_gbs}
     gu = kern["gather_update"]
     gu["achieved_gbs"] = gu["bytes"] / (gu["avg_ms"] * 1e-3) / 1e9
     gu["frac_hbm"] = gu["achieved_gbs"] / HBM_PEAK_GBS
@@ -157,17 +184,19 @@ def main():
         "value": value,
         "unit": "elem*qp/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp64_fast": "f64"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"Hex8 {shape[0]}x{shape[1]}x{shape[2]} jittered grid, "
                                "compressible neo-Hookean, explicit central difference "
-                               "with lumped mass, plain FP64",
+                               "with lumped mass, "
+                               + {"fp64": "plain FP64", "fp32": "FP32 form",
+                                  "fp64_fast": "FP64 FMA form"}[args.precision],
                    "elements": mesh.nb_elements, "nodes": mesh.nb_nodes,
                    "quadrature_points": nqp, "dt": dt},
         "roofline": roofline,
@@ -175,10 +204,331 @@ def main():
     }
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline()
+            out["cpu_baseline"] = cpu_baseline_plain()
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"error": repr(e)}
-    print(json.dumps(out), flush=True)
+    return out
+
+
+# --------------------------------------------------------------------------
+# precision form: 60 M Tet4, per-element E and nu
+# --------------------------------------------------------------------------
+def build_c3(shape, seed=3):
+    from akantu2_amd import mesh as M
+    mesh = M.box_mesh("_tetrahedron_4", *shape, jitter=0.1, seed=seed)
+    rng = np.random.default_rng(seed)
+    E = np.exp(rng.uniform(np.log(0.5), np.log(2.0), mesh.nb_elements))
+    nu = rng.uniform(0.2, 0.35, mesh.nb_elements)
+    return mesh, M.clamped_face_x0(mesh), M.initial_velocity(mesh, 1e-3, seed), E, nu
+
+
+def _timed_model(mesh, blocked, v0, E, nu, precision, warmup, steps):
+    from akantu2_amd import SolidMechanicsModel
+    m = SolidMechanicsModel(mesh)
+    m.addMaterial("elastic", rho=1.0, E=1.0, nu=0.3, E_el=E, nu_el=nu)
+    m.setPrecision(precision)
+    m.initFull()
+    m.getBlockedDOFs()[:] = blocked
+    m.getVelocity()[:] = v0
+    m.setTimeStep(0.5 * m.getStableTimeStep())
+    m.solveStep(warmup)
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.solveStep(steps)
+    m.synchronize()
+    el = time.perf_counter() - t0
+    m.enableKernelTiming(True)
+    m.solveStep(10)
+    m.synchronize()
+    kern = {}
+    for k, name in {0: "element_pass", 1: "gather_update"}.items():
+        ms, n = m.kernelTiming(k)
+        b = m.kernelBytes(k)
+        kern[name] = {"avg_ms": ms / n, "bytes": b,
+                      "achieved_gbs": b / (ms / n * 1e-3) / 1e9}
+    return m, el, kern
+
+
+def run_precision(args):
+    steps = args.steps or 2000
+    shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else (1000, 100, 100)
+    t = time.perf_counter()
+    mesh, blocked, v0, E, nu = build_c3(shape)
+    print(f"# mesh {time.perf_counter() - t:.1f} s, {mesh.nb_elements} Tet4",
+          file=sys.stderr, flush=True)
+    res = {}
+    for prec in ("fp32", "fp64"):
+        m, el, kern = _timed_model(mesh, blocked, v0, E, nu, prec, args.warmup, steps)
+        res[prec] = {"value": mesh.nb_elements * steps / el, "ms_per_step": 1e3 * el / steps,
+                     "kernels": kern}
+        print(f"# {prec}: {1e3 * el / steps:.3f} ms/step", file=sys.stderr, flush=True)
+        del m
+    el = res["fp32"]["kernels"]["element_pass"]
+    out = {
+        "metric": "elem*qp stress+force updates/s", "value": res["fp32"]["value"],
+        "unit": "elem*qp/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": res["fp32"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"Tet4 {shape[0]}x{shape[1]}x{shape[2]} cubes x 6 jittered, "
+                               "linear elastic with per-element E, nu, FP32 form "
+                               "(FP64 compensated nodal update)",
+                   "elements": mesh.nb_elements, "nodes": mesh.nb_nodes},
+        "roofline": {"bound": "hbm", "achieved": el["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": el["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_tet4_fast<elastic, f32>",
+                     "algorithmic_bytes_per_launch": el["bytes"],
+                     "avg_kernel_ms": el["avg_ms"]},
+        "forms": res,
+        "fp32_over_fp64": res["fp32"]["value"] / res["fp64"]["value"],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_precision()
+    return out
+
+
+def cpu_baseline_precision(shape=(60, 10, 10), steps=200):
+    """The oracle on a 36,000-Tet4 box of the same workload (timed, 1 thread),
+    and the FP32 form's per-field error against it on that box."""
+    import oracle
+    mesh, blocked, v0, E, nu = build_c3(shape)
+    o = oracle.OracleModel(oracle.TET4, mesh.nodes, mesh.conn, "elastic", E_el=E, nu_el=nu)
+    o.init_full()
+    o.blocked[:] = blocked
+    o.v[:] = v0
+    dt = 0.5 * o.stable_dt()
+    o.set_time_step(dt)
+    t0 = time.perf_counter()
+    o.solve_steps(steps)
+    el = time.perf_counter() - t0
+    from akantu2_amd import SolidMechanicsModel
+    g = SolidMechanicsModel(mesh)
+    g.addMaterial("elastic", rho=1.0, E=1.0, nu=0.3, E_el=E, nu_el=nu)
+    g.setPrecision("fp32")
+    g.initFull()
+    g.getBlockedDOFs()[:] = blocked
+    g.getVelocity()[:] = v0
+    g.setTimeStep(dt)
+    g.solveStep(steps)
+    err = {"u": rel_errors(g.getDisplacement(), o.u), "v": rel_errors(g.getVelocity(), o.v),
+           "f_int": rel_errors(g.getInternalForce(), o.f_int)}
+    return {"value": mesh.nb_elements * steps / el, "unit": "elem*qp/s", "cores": 1,
+            "kind": "port", "seconds": el,
+            "sample": f"{steps} steps of a {shape[0]}x{shape[1]}x{shape[2]} x 6 Tet4 box "
+                      "of the same workload, oracle/akantu_oracle.c, 1 thread",
+            "fp32_error_vs_oracle": err}
+
+
+# --------------------------------------------------------------------------
+# batching form: 512 Tet4 plasticity instances
+# --------------------------------------------------------------------------
+def c4_instances(n=512, seed=4, scale=1.0):
+    """Instance descriptions: cells, material, seed (scale shrinks the sizes
+    for a quick run)."""
+    from akantu2_amd import mesh as M
+    rng = np.random.default_rng(seed)
+    nel = np.exp(rng.uniform(np.log(5e3), np.log(4e5), n)) * scale
+    aspect = rng.uniform(1.0, 8.0, n)
+    E = rng.uniform(0.5, 2.0, n)
+    nu = rng.uniform(0.2, 0.35, n)
+    sy = rng.uniform(1e-3, 5e-3, n)
+    hh = rng.uniform(0.0, 0.1, n)
+    out = []
+    for k in range(n):
+        cells = M.cells_for("_tetrahedron_4", max(nel[k], 6.0), aspect[k])
+        out.append(dict(cells=cells, mat=dict(E=float(E[k]), nu=float(nu[k]),
+                                              sigma_y=float(sy[k]), h=float(hh[k])),
+                        seed=1000 + k))
+    return out
+
+
+def c4_mesh(spec):
+    from akantu2_amd import mesh as M
+    mesh = M.box_mesh("_tetrahedron_4", *spec["cells"])
+    return mesh, M.clamped_face_x0(mesh), M.initial_velocity(mesh, 1e-2, spec["seed"])
+
+
+def _fill_batch(precision, specs):
+    from akantu2_amd.batch import Batch
+    nn = sum((c[0] + 1) * (c[1] + 1) * (c[2] + 1) + 255 for c in (s["cells"] for s in specs))
+    nt = sum(6 * c[0] * c[1] * c[2] for c in (s["cells"] for s in specs))
+    b = Batch(precision, nodes=nn, tets=nt, plastic=True)
+    ids = []
+    for s in specs:
+        mesh, blocked, v0 = c4_mesh(s)
+        ids.append(b.add(mesh, PL, blocked=blocked, velocity=v0, dt_factor=0.5, **s["mat"]))
+    return b, ids
+
+
+def _time_batch(b, warmup, steps, tune):
+    best = b.autotune(repeats=2) if tune else b.launch()
+    b.solveStep(warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.solveStep(steps)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    b.enableKernelTiming(True)
+    b.solveStep(10)
+    b.synchronize()
+    kern = {}
+    for k, name in {0: "element_pass", 1: "gather_update"}.items():
+        ms, n = b.kernelTiming(k)
+        by = b.kernelBytes(k)
+        kern[name] = {"avg_ms": ms / n, "bytes": by, "achieved_gbs": by / (ms / n * 1e-3) / 1e9}
+    b.enableKernelTiming(False)
+    return el, kern, best
+
+
+def run_batch(args):
+    steps = args.steps or 500
+    specs = c4_instances(args.instances, scale=args.scale)
+    t = time.perf_counter()
+    res = {}
+    for prec in ("fp32", "fp64"):
+        b, ids = _fill_batch(prec, specs)
+        st = b.stats()
+        print(f"# {prec} batch: {st['live']} instances, {st['elements']} Tet4, "
+              f"{st['nodes']} nodes, setup {time.perf_counter() - t:.1f} s",
+              file=sys.stderr, flush=True)
+        el, kern, best = _time_batch(b, args.warmup, steps, tune=True)
+        res[prec] = {"value": st["quadrature_points"] * steps / el,
+                     "ms_per_step": 1e3 * el / steps, "kernels": kern, "launch": best}
+        print(f"# {prec}: {1e3 * el / steps:.3f} ms/step {best}", file=sys.stderr, flush=True)
+        if prec == "fp32" and not args.no_cpu_baseline:
+            # the smallest instance, for the error check against the oracle
+            k = int(np.argmin([np.prod(s["cells"]) for s in specs]))
+            gpu_u = b.getDisplacement(ids[k])
+            gpu_steps = b.steps(ids[k])
+        b.close()
+        t = time.perf_counter()
+    el = res["fp32"]["kernels"]["element_pass"]
+    out = {
+        "metric": "elem*qp stress+force updates/s", "value": res["fp32"]["value"],
+        "unit": "elem*qp/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": res["fp32"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(specs)} independent Tet4 boxes (5e3-4e5 elements "
+                               "log-uniform x scale, aspect U(1,8)), linear isotropic "
+                               "hardening with per-instance E, nu, sigma_y, h, FP32 form, "
+                               "persistent launches in a hipGraph, autotuned",
+                   "instances": len(specs), "scale": args.scale,
+                   "elements": int(st["elements"])},
+        "roofline": {"bound": "hbm", "achieved": el["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": el["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_batch_tet4_f32<plastic>",
+                     "algorithmic_bytes_per_launch": el["bytes"],
+                     "avg_kernel_ms": el["avg_ms"]},
+        "forms": res,
+        "fp32_over_fp64": res["fp32"]["value"] / res["fp64"]["value"],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batch(specs[k], gpu_u, gpu_steps)
+    return out
+
+
+def cpu_baseline_batch(spec, gpu_u, steps):
+    """The oracle on one instance of the batch (the smallest), timed on one
+    thread, and the FP32 batch's displacement error against it."""
+    import oracle
+    mesh, blocked, v0 = c4_mesh(spec)
+    o = oracle.OracleModel(oracle.TET4, mesh.nodes, mesh.conn, PL, **spec["mat"])
+    o.init_full()
+    o.blocked[:] = blocked
+    o.v[:] = v0
+    o.set_time_step(0.5 * o.stable_dt())
+    t0 = time.perf_counter()
+    o.solve_steps(steps)
+    el = time.perf_counter() - t0
+    return {"value": mesh.nb_elements * steps / el, "unit": "elem*qp/s", "cores": 1,
+            "kind": "port", "seconds": el,
+            "sample": f"{steps} steps of the smallest batch instance "
+                      f"({mesh.nb_elements} Tet4, plastic), oracle/akantu_oracle.c, 1 thread",
+            "fp32_u_error_vs_oracle": rel_errors(gpu_u, o.u)}
+
+
+# --------------------------------------------------------------------------
+# serving form
+# --------------------------------------------------------------------------
+def run_serve(args):
+    from akantu2_amd import serving as S
+    reqs = S.synthetic_requests(args.requests, rate=args.rate, seed=5, scale=args.scale)
+    t = time.perf_counter()
+    for r in reqs:
+        r.materialize()
+    print(f"# {len(reqs)} requests generated in {time.perf_counter() - t:.1f} s",
+          file=sys.stderr, flush=True)
+    server = S.Server(precision="fp32", hbm_bytes=int(args.hbm_gib * (1 << 30)))
+    rep = server.run(reqs)
+    tt = np.array([r.time_to_result for r in rep.done])
+    out = {
+        "metric": "elem*qp stress+force updates/s", "value": rep.throughput,
+        "unit": "elem*qp/s", "n_gpus": 1, "steps": int(sum(r.steps for r in rep.done)),
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"Poisson stream of {len(reqs)} requests at {args.rate}/s, "
+                               "Tet4 or Hex8 boxes of 1e3-2e6 elements (log-uniform x "
+                               "scale), elastic or neo-Hookean, 200-5000 steps, FP32 form, "
+                               "continuous batching with HBM-footprint admission",
+                   "requests": len(reqs), "rate_per_s": args.rate, "scale": args.scale},
+        "latency_s": {"p50": float(np.percentile(tt, 50)), "p99": float(np.percentile(tt, 99)),
+                      "max": float(tt.max())},
+        "makespan_s": rep.makespan, "chunks": rep.chunks, "failed": len(rep.failed),
+        "max_live": rep.max_live,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_serve(rep)
+    return out
+
+
+def cpu_baseline_serve(rep):
+    """The oracle on the smallest served request (timed, 1 thread) and the
+    served result's error against it."""
+    import oracle
+    from akantu2_amd.batch import law_id
+    r = min(rep.done, key=lambda q: q.mesh.nb_elements * q.steps)
+    law = {0: "elastic", 2: "neohookean"}[law_id(r.law)]
+    o = oracle.OracleModel(r.mesh.etype, r.mesh.nodes, r.mesh.conn, law, **r.material)
+    o.init_full()
+    o.blocked[:] = r.blocked
+    o.v[:] = r.velocity
+    o.set_time_step(0.5 * o.stable_dt())
+    t0 = time.perf_counter()
+    o.solve_steps(r.steps)
+    el = time.perf_counter() - t0
+    return {"value": r.mesh.nb_elements * r.mesh.nqp * r.steps / el, "unit": "elem*qp/s",
+            "cores": 1, "kind": "port", "seconds": el,
+            "sample": f"the smallest served request ({r.mesh.nb_elements} elements, "
+                      f"{r.steps} steps, {law}), oracle/akantu_oracle.c, 1 thread",
+            "u_error_vs_oracle": rel_errors(r.result["displacement"], o.u)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--form", choices=["plain", "precision", "batch", "serve"],
+                    default="plain")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "fp64_fast"], default="fp64",
+                    help="plain form only")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--shape", type=str, default=None,
+                    help="override cells per direction, e.g. 20,10,10")
+    ap.add_argument("--instances", type=int, default=512, help="batch form")
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="batch / serve forms: multiply the instance sizes")
+    ap.add_argument("--requests", type=int, default=64, help="serve form")
+    ap.add_argument("--rate", type=float, default=4.0, help="serve form: requests/s")
+    ap.add_argument("--hbm-gib", type=float, default=16.0,
+                    help="serve form: device memory the server may use")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is designed, not built)"}))
+        sys.exit(2)
+    run = {"plain": run_plain, "precision": run_precision, "batch": run_batch,
+           "serve": run_serve}[args.form]
+    print(json.dumps(run(args)), flush=True)
 
 
 if __name__ == "__main__":

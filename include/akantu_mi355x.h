@@ -240,6 +240,8 @@ int akx_batch_capacity_left(akx_batch *b, int64_t *nodes, int64_t *tets,
 int akx_batch_enable_kernel_timing(akx_batch *b, int enable);
 int akx_batch_get_kernel_timing(akx_batch *b, int kernel_class,
                                 double *total_ms, int64_t *launches);
+/* Unique bytes one launch of the class moves over the live instances. */
+int akx_batch_get_kernel_bytes(akx_batch *b, int kernel_class, double *bytes);
 
 #ifdef __cplusplus
 }
