@@ -211,6 +211,66 @@ private:
   std::deque<Material> materials; // stable references
 };
 
+// Many independent models stepped together (akx_batch_*): each instance is
+// one SolidMechanicsModel with one material and zero external force.
+class Batch {
+public:
+  enum Precision { fp64 = AKX_PRECISION_FP64, fp32 = AKX_PRECISION_FP32 };
+  Batch(Precision p, std::int64_t node_capacity, std::int64_t tet_capacity,
+        std::int64_t hex_capacity, bool plasticity = false, int device = 0) {
+    check(akx_batch_create(device, p, node_capacity, tet_capacity, hex_capacity,
+                           plasticity, &h));
+  }
+  ~Batch() { akx_batch_destroy(h); }
+  Batch(const Batch &) = delete;
+  Batch &operator=(const Batch &) = delete;
+
+  // SolidMechanicsModel(mesh) + addMaterial + initFull + setTimeStep(
+  // dt_factor * getStableTimeStep()); returns the instance id
+  int addInstance(const Mesh &mesh, const std::string &law, const MaterialParameters &p,
+                  const Array<std::uint8_t> *blocked = nullptr,
+                  const Array<double> *velocity = nullptr, double dt_factor = 0.5,
+                  bool finite_deformation = false) {
+    int lid = 0, id = -1;
+    check(akx_material_law_by_name(law.c_str(), finite_deformation, &lid));
+    double params[5] = {p.rho, p.E, p.nu, p.sigma_y, p.h};
+    check(akx_batch_add_instance(h, mesh.type, lid, params, mesh.nodes.size(),
+                                 mesh.nodes.data(), mesh.connectivity.size(),
+                                 mesh.connectivity.data(),
+                                 blocked ? blocked->data() : nullptr,
+                                 velocity ? velocity->data() : nullptr, dt_factor, &id));
+    if ((int)nb_nodes.size() <= id)
+      nb_nodes.resize(id + 1);
+    nb_nodes[id] = mesh.nodes.size();
+    return id;
+  }
+  void removeInstance(int id) { check(akx_batch_remove_instance(h, id)); }
+  // nb_steps solveStep() calls of every live instance
+  void solveStep(std::int64_t nb_steps = 1) { check(akx_batch_solve_step(h, nb_steps)); }
+  double getTimeStep(int id) {
+    double dt = 0.;
+    check(akx_batch_get_time_step(h, id, &dt, nullptr));
+    return dt;
+  }
+  Array<double> getDisplacement(int id) { return field(id, AKX_FIELD_DISPLACEMENT); }
+  Array<double> getVelocity(int id) { return field(id, AKX_FIELD_VELOCITY); }
+  Array<double> getInternalForce(int id) { return field(id, AKX_FIELD_INTERNAL_FORCE); }
+  // launch shape chosen by timing the element pass of the live instances
+  void autotune(int repeats = 3) {
+    check(akx_batch_autotune(h, repeats, nullptr, nullptr, nullptr, nullptr));
+  }
+  akx_batch *handle() { return h; }
+
+private:
+  Array<double> field(int id, int f) {
+    Array<double> out(nb_nodes.at(id), 3);
+    check(akx_batch_get_field(h, id, f, out.data()));
+    return out;
+  }
+  akx_batch *h = nullptr;
+  std::vector<std::int64_t> nb_nodes;
+};
+
 inline Array<double> Material::getGradU() const {
   model->push();
   check(akx_model_compute_all_stresses(model->h));

@@ -4,7 +4,11 @@
 // loop, read the nodal arrays back.  Inputs and outputs are raw binary files
 // written and read by tests/test_gpu_cpp_api.py.
 //
-//   drive_explicit <dir> <etype 0|1> <law name> <finite_deformation 0|1> <steps>
+//   drive_explicit <dir> <etype 0|1> <law name> <finite_deformation 0|1> <steps> [batch]
+//
+// With "batch" the same simulation runs as one instance of an FP64 Batch
+// (include/akantu_mi355x.hpp), beside a second instance (the same mesh at
+// rest) that must not change its result.
 //
 // <dir>/nodes.f64 [nn][3], conn.i32 [nel][nnpe], blocked.u8 [nn][3],
 // v0.f64 [nn][3], params.f64 {rho, E, nu, sigma_y, h, dt_factor}
@@ -37,10 +41,12 @@ template <class T> static void save(const std::string &path, const T *p, size_t 
 }
 
 int main(int argc, char **argv) {
-  if (argc != 6) {
-    std::cerr << "usage: drive_explicit <dir> <etype> <law> <finite_deformation> <steps>\n";
+  if (argc != 6 && argc != 7) {
+    std::cerr << "usage: drive_explicit <dir> <etype> <law> <finite_deformation> <steps> "
+                 "[batch]\n";
     return 2;
   }
+  const bool batch = argc == 7 && std::string(argv[6]) == "batch";
   const std::string dir = argv[1];
   const int etype = std::atoi(argv[2]);
   const std::string law = argv[3];
@@ -60,6 +66,30 @@ int main(int argc, char **argv) {
     std::copy(X.begin(), X.end(), mesh.nodes.data());
     std::copy(C.begin(), C.end(), mesh.connectivity.data());
 
+    if (batch) {
+      Array<std::uint8_t> blocked(nn, 3);
+      Array<double> velocity(nn, 3);
+      std::copy(B.begin(), B.end(), blocked.data());
+      std::copy(V.begin(), V.end(), velocity.data());
+      const std::int64_t tets = etype == 0 ? 2 * nel : 0, hexes = etype == 1 ? 2 * nel : 0;
+      Batch b(Batch::fp64, 2 * nn + 512, tets, hexes);
+      const int other = b.addInstance(mesh, law, {P[0], P[1], P[2], P[3], P[4]}, &blocked,
+                                      nullptr, P[5], fd);
+      const int id = b.addInstance(mesh, law, {P[0], P[1], P[2], P[3], P[4]}, &blocked,
+                                   &velocity, P[5], fd);
+      b.solveStep(steps / 2);
+      b.solveStep(steps - steps / 2);
+      save(dir + "/u.f64", b.getDisplacement(id).data(), 3 * nn);
+      save(dir + "/v.f64", b.getVelocity(id).data(), 3 * nn);
+      save(dir + "/fint.f64", b.getInternalForce(id).data(), 3 * nn);
+      const double dt = b.getTimeStep(id);
+      save(dir + "/dt.f64", &dt, 1);
+      auto rest = b.getDisplacement(other);
+      for (std::int64_t i = 0; i < 3 * nn; ++i)
+        if (rest.data()[i] != 0.)
+          throw std::runtime_error("the instance at rest moved");
+      return 0;
+    }
     SolidMechanicsModel model(mesh);
     model.addMaterial(law, {P[0], P[1], P[2], P[3], P[4]}, fd);
     model.initFull(_explicit_lumped_mass);

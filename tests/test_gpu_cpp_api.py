@@ -44,3 +44,28 @@ def test_cpp_driver_matches_oracle(tmp_path, etype, law, fd, shape, steps):
     oracle.lib().orc_compute_all_stresses(ctypes.byref(o._s))
     stress = np.fromfile(os.path.join(d, "stress.f64")).reshape(-1, 9)
     assert relerr(stress, o.stress) <= 1e-12
+
+
+@pytest.mark.parametrize("etype,law,fd,shape,steps", [
+    ("_tetrahedron_4", "elastic", 0, (12, 4, 4), 301),
+    ("_hexahedron_8", "neohookean", 0, (6, 4, 3), 200),
+])
+def test_cpp_batch_matches_oracle(tmp_path, etype, law, fd, shape, steps):
+    """The C++ Batch class: one instance among two, against the oracle."""
+    mesh, blocked, v0 = workload(etype, shape, jitter=0.1, seed=4)
+    d = str(tmp_path)
+    mesh.nodes.astype(np.float64).tofile(os.path.join(d, "nodes.f64"))
+    mesh.conn.astype(np.int32).tofile(os.path.join(d, "conn.i32"))
+    blocked.astype(np.uint8).tofile(os.path.join(d, "blocked.u8"))
+    v0.astype(np.float64).tofile(os.path.join(d, "v0.f64"))
+    np.array([1.0, 1.0, 0.3, 0.0, 0.0, 0.5]).tofile(os.path.join(d, "params.f64"))
+    r = subprocess.run([DRIVER, d, str(mesh.etype), law, str(fd), str(steps), "batch"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o, dt = make_oracle(mesh, blocked, v0, law)
+    assert np.fromfile(os.path.join(d, "dt.f64"))[0] == dt
+    o.solve_steps(steps)
+    rd = lambda n: np.fromfile(os.path.join(d, n)).reshape(-1, 3)
+    assert relerr(rd("u.f64"), o.u) <= 1e-12
+    assert relerr(rd("v.f64"), o.v) <= 1e-12
+    assert relerr(rd("fint.f64"), o.f_int) <= 1e-12
