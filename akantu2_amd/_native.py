@@ -68,6 +68,7 @@ def _declare(lib):
         "akx_model_compute_all_stresses": (C.c_int, [_vp]),
         "akx_model_get_internal": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
         "akx_model_synchronize": (C.c_int, [_vp]),
+        "akx_model_get_energy": (C.c_int, [_vp, C.c_char_p, C.c_int, _dp]),
         "akx_model_enable_kernel_timing": (C.c_int, [_vp, C.c_int]),
         "akx_model_get_kernel_timing": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(_i64)]),
         "akx_model_get_kernel_bytes": (C.c_int, [_vp, C.c_int, _dp]),

@@ -29,6 +29,9 @@ struct ElemArgs {
   // plastic history, [9][n*nqp] and [n*nqp]: previous (read) / current (write)
   const double *p_grad, *p_sig, *p_eps, *p_iso;
   double *c_grad, *c_sig, *c_eps, *c_iso;
+  // plastic energy [n*nqp]: previous (read) / current (write), or null
+  const double *p_pe;
+  double *c_pe;
   int64_t nblocks_padded;
 };
 
@@ -43,12 +46,15 @@ template <class HT> struct QpViewT {
   HT *c_grad, *c_sig, *c_eps, *c_iso;
   double *grad_u, *stress;
   int64_t nq_all;
+  const HT *p_pe; // accumulated plastic energy, or null (not tracked)
+  HT *c_pe;
 };
 using QpView = QpViewT<double>;
 
 __host__ __device__ inline QpView qp_view(const ElemArgs &A, int nqp) {
-  return QpView{A.p_grad, A.p_sig, A.p_eps, A.p_iso, A.c_grad, A.c_sig,
-                A.c_eps,  A.c_iso, A.grad_u, A.stress, A.n * nqp};
+  return QpView{A.p_grad, A.p_sig,  A.p_eps,  A.p_iso,       A.c_grad,
+                A.c_sig,  A.c_eps,  A.c_iso,  A.grad_u,      A.stress,
+                A.n * nqp, A.p_pe,  A.c_pe};
 }
 
 } // namespace akx

@@ -78,7 +78,8 @@ __device__ __forceinline__ QpViewT<HT> batch_view(const BatchElemArgs &A, int pa
                      (HT *)A.hist[par ^ 1][0],   (HT *)A.hist[par ^ 1][1],
                      (HT *)A.hist[par ^ 1][2],   (HT *)A.hist[par ^ 1][3],
                      nullptr,                    nullptr,
-                     A.hcap};
+                     A.hcap,                     nullptr,
+                     nullptr};
 }
 
 // the compact FP32 plastic state of one step (read `par`, write par ^ 1)

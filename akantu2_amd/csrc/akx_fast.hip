@@ -98,6 +98,15 @@ __device__ __forceinline__ void plastic_at(const QpViewT<HT> &V, int64_t iq, CT 
       V.c_eps[o] = HT(eps[ii][jj]);
     }
   V.c_iso[iq] = HT(iso);
+  if (V.c_pe) { // afterSolveStep -> updateEnergies (material_plastic.cc:86-99)
+    CT dd = CT(0);
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii)
+        dd += (S[ii][jj] + ps[ii][jj]) * (eps[ii][jj] - pe[ii][jj]);
+    V.c_pe[iq] = HT(CT(V.p_pe[iq]) + dd / CT(2));
+  }
 }
 
 // The total-strain form with the compact state (plastic strain in Voigt

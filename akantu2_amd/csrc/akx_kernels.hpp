@@ -292,6 +292,25 @@ __device__ __forceinline__ void btd_finite(const R (&g)[3][3], const R (&S)[3][3
   r[2] = btd_finite_k<2>(g, S, b);
 }
 
+// afterSolveStep -> Material::updateEnergies of the plastic law
+// (material.cc:740-750, material_plastic.cc:86-99):
+// pe + (sigma + sigma_prev) : (eps_p - eps_p_prev) / 2, the double dot summed
+// in column-major order (cwiseProduct().sum()).
+template <class CT>
+__device__ __forceinline__ CT plastic_energy_update(CT pe_prev, const CT (&S)[3][3],
+                                                    const CT (&ps)[3][3],
+                                                    const CT (&eps)[3][3],
+                                                    const CT (&peps)[3][3]) {
+  CT dd = (S[0][0] + ps[0][0]) * (eps[0][0] - peps[0][0]);
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii)
+      if (ii + jj > 0)
+        dd += (S[ii][jj] + ps[ii][jj]) * (eps[ii][jj] - peps[ii][jj]);
+  return pe_prev + dd / CT(2);
+}
+
 // Constitutive law of one Gauss point in the reference's operation order
 // (material_elastic_inline_impl.hh:31-73, material_neohookean_inline_impl.hh
 // :39-50, material_linear_isotropic_hardening_inline_impl.hh:31-86), with the
@@ -330,6 +349,8 @@ __device__ __forceinline__ void strict_law(const QpViewT<HT> &V, int64_t iq, CT 
         V.c_eps[o] = HT(eps[ii][jj]);
       }
     V.c_iso[iq] = HT(iso);
+    if (V.c_pe)
+      V.c_pe[iq] = HT(plastic_energy_update(CT(V.p_pe[iq]), S, ps, eps, pe));
   }
   if (V.grad_u) {
 #pragma unroll

@@ -229,6 +229,171 @@ __global__ void k_gather_mass(int64_t nn, const int64_t *ell_off,
   mass[n] = m;
 }
 
+// ---------------------------------------------------------------------------
+// Energies (solid_mechanics_model.cc:476-606, material.cc:604-649,
+// material_plastic.cc:47-99).  Per-node or per-element values, then one
+// deterministic tree sum (fixed shape, so repeated calls give the same bits).
+// ---------------------------------------------------------------------------
+template <class R> __device__ __forceinline__ R ddot3(const R (&a)[3][3], const R (&b)[3][3]) {
+  R s = a[0][0] * b[0][0]; // cwiseProduct().sum(), column-major order
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (i + j > 0)
+        s += a[i][j] * b[i][j];
+  return s;
+}
+
+// Potential energy density of one Gauss point:
+//   elastic          .5 sigma : grad u  (material_elastic_inline_impl.hh:113-116)
+//   elastic finite   .5 S : E(grad u)   (material_elastic.cc:138-160)
+//   neo-Hookean      lambda (ln J)^2 + mu (-ln J + (tr C - 3) / 2) / 2, as the
+//                    reference writes it (material_neohookean_inline_impl.hh
+//                    :119-126)
+//   plastic          .5 sigma : (grad u - eps_p)  (material_plastic.cc:71-83)
+template <int LAW>
+__device__ __forceinline__ double epot_density(double lambda, double mu,
+                                               const double (&g)[3][3],
+                                               const double (&S)[3][3],
+                                               const double (&ep)[3][3]) {
+  if constexpr (LAW == LAW_ELASTIC) {
+    return .5 * ddot3(S, g);
+  } else if constexpr (LAW == LAW_ELASTIC_FD) {
+    double E[3][3];
+    green_strain(g, E);
+    return .5 * ddot3(S, E);
+  } else if constexpr (LAW == LAW_NEOHOOKEAN) {
+    double F[3][3], C[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        F[i][j] = (i == j) ? 1. + g[i][j] : g[i][j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double c = F[0][i] * F[0][j];
+        c += F[1][i] * F[1][j];
+        c += F[2][i] * F[2][j];
+        C[i][j] = c;
+      }
+    const double lj = log(det3(F));
+    const double trC = C[0][0] + C[1][1] + C[2][2];
+    return lambda * (lj * lj) + mu * (-lj + (trC - 3) / 2.) / 2.;
+  } else {
+    double el[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        el[i][j] = g[i][j] - ep[i][j];
+    return .5 * ddot3(S, el);
+  }
+}
+
+struct EnergyArgs {
+  int64_t n, first, nn, nq_all;
+  const int32_t *filter, *conn;
+  const double *X;
+  double lambda, mu;
+  const double *lam_el, *mu_el;
+  const double *g, *s, *eps; // [9][nq_all] current grad u, stress, plastic strain
+  const double *pe;          // [nq_all] accumulated plastic energy, or null
+  double *out;               // [n] integral over each element
+};
+
+// FEEngine::integrate of the density: sum_q f_q wJ_q per element
+// (integrator_gauss_inline_impl.hh:401-441), wJ as at init
+template <int ET, int LAW> __global__ void k_energy_elements(EnergyArgs A) {
+  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n)
+    return;
+  const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+  double X[NN][3];
+#pragma unroll
+  for (int n = 0; n < NN; ++n) {
+    const int32_t node = A.conn[e * NN + n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      X[n][c] = A.X[(int64_t)c * A.nn + node];
+  }
+  const double lambda = A.lam_el ? A.lam_el[i] : A.lambda;
+  const double mu = A.mu_el ? A.mu_el[i] : A.mu;
+  double sum = 0.;
+  for (int q = 0; q < NQ; ++q) {
+    double Ji[3][3], detJ;
+    const double wJ = Shape<ET, double>::jacobian_at(X, q, Ji, detJ);
+    const int64_t iq = i * NQ + q;
+    double f;
+    if (A.pe) {
+      f = A.pe[iq];
+    } else {
+      double g[3][3], S[3][3], ep[3][3];
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          const int64_t o = (int64_t)(ii + 3 * jj) * A.nq_all + iq;
+          g[ii][jj] = A.g[o];
+          S[ii][jj] = A.s[o];
+          ep[ii][jj] = A.eps ? A.eps[o] : 0.;
+        }
+      f = epot_density<LAW>(lambda, mu, g, S, ep);
+    }
+    sum = (q == 0) ? f * wJ : sum + f * wJ;
+  }
+  A.out[i] = sum;
+}
+
+// kinetic energy per node: sum_i v_i v_i m (m > epsilon) (solid_mechanics_model
+// .cc:476-522); external work per node: blocked ? -f_int v : f_ext v
+// (solid_mechanics_model.cc:545-583)
+__global__ void k_node_energy(int64_t nn, int kind, const double *v, const double *mass,
+                              const double *fint, const double *fext,
+                              const uint8_t *blocked, double *out) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn)
+    return;
+  double w = 0.;
+  for (int c = 0; c < 3; ++c) {
+    const int64_t o = (int64_t)c * nn + n;
+    if (kind == 0) {
+      const double m = mass[n];
+      if (m > 2.220446049250313e-16)
+        w += v[o] * v[o] * m;
+    } else if ((blocked[n] >> c) & 1) {
+      w -= fint[o] * v[o];
+    } else {
+      w += (fext ? fext[o] : 0.) * v[o];
+    }
+  }
+  out[n] = w;
+}
+
+// one level of the tree sum: block b sums in[b*2048 .. b*2048+2047]
+__global__ void __launch_bounds__(256) k_tree_sum(const double *in, int64_t n, double *out) {
+  __shared__ double red[256];
+  const int64_t base = (int64_t)blockIdx.x * 2048;
+  double s = 0.;
+  for (int k = 0; k < 8; ++k) {
+    const int64_t i = base + threadIdx.x + 256 * k;
+    if (i < n)
+      s += in[i];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    out[blockIdx.x] = red[0];
+}
+
 } // namespace akx
 
 using namespace akx;
@@ -251,6 +416,7 @@ struct Material {
   // materialised internals and plastic history [2][9][n*nqp] / [2][n*nqp]
   DevBuf<double> grad, stress; // materialised by compute_all_stresses
   DevBuf<double> hgrad[2], hsig[2], heps[2], hiso[2];
+  DevBuf<double> hpe[2]; // accumulated plastic energy [n*nqp]
   int cur = 0; // index of the current plastic state
 };
 
@@ -431,6 +597,8 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
     A.c_sig = mat.hsig[c].p;
     A.c_eps = mat.heps[c].p;
     A.c_iso = mat.hiso[c].p;
+    A.p_pe = mat.hpe[p].p;
+    A.c_pe = mat.hpe[c].p;
   }
   A.nblocks_padded = padded_blocks(mat.n);
   return A;
@@ -703,10 +871,12 @@ void alloc_internals(akx_model *m, Material &mat) {
       mat.hsig[s].alloc(9 * nq);
       mat.heps[s].alloc(9 * nq);
       mat.hiso[s].alloc(nq);
+      mat.hpe[s].alloc(nq);
       zero_async(m, mat.hgrad[s].p, 9 * nq);
       zero_async(m, mat.hsig[s].p, 9 * nq);
       zero_async(m, mat.heps[s].p, 9 * nq);
       zero_async(m, mat.hiso[s].p, nq);
+      zero_async(m, mat.hpe[s].p, nq);
     }
     mat.cur = 0;
   }
@@ -1192,6 +1362,149 @@ int akx_model_compute_all_stresses(akx_model *m) {
     launch_element_pass(m, true);
     flip_plastic_state(m);
     HIPCHECK(hipStreamSynchronize(m->stream));
+  });
+}
+
+} // extern "C"
+
+namespace {
+
+// deterministic sum of n device values (buffer a is overwritten)
+double device_sum(akx_model *m, DevBuf<double> &a, DevBuf<double> &b, int64_t n) {
+  if (n == 0)
+    return 0.;
+  double *in = a.p, *out = b.p;
+  while (n > 1) {
+    const int64_t nb = (n + 2047) / 2048;
+    hipLaunchKernelGGL(k_tree_sum, dim3((unsigned)nb), dim3(256), 0, m->stream,
+                       (const double *)in, n, out);
+    HIPCHECK(hipGetLastError());
+    std::swap(in, out);
+    n = nb;
+  }
+  double r = 0.;
+  HIPCHECK(hipMemcpyAsync(&r, in, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIPCHECK(hipStreamSynchronize(m->stream));
+  return r;
+}
+
+template <int ET>
+void launch_energy_elements(akx_model *m, int law, const EnergyArgs &A) {
+  dim3 grid(grid_for(A.n)), block(TPB);
+  switch (law) {
+  case AKX_LAW_ELASTIC:
+    hipLaunchKernelGGL((k_energy_elements<ET, LAW_ELASTIC>), grid, block, 0, m->stream, A);
+    break;
+  case AKX_LAW_ELASTIC_FD:
+    hipLaunchKernelGGL((k_energy_elements<ET, LAW_ELASTIC_FD>), grid, block, 0, m->stream,
+                       A);
+    break;
+  case AKX_LAW_NEOHOOKEAN:
+    hipLaunchKernelGGL((k_energy_elements<ET, LAW_NEOHOOKEAN>), grid, block, 0, m->stream,
+                       A);
+    break;
+  default:
+    hipLaunchKernelGGL((k_energy_elements<ET, LAW_PLASTIC>), grid, block, 0, m->stream, A);
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+// Material::getEnergy("potential" / "plastic") of one material
+double material_energy(akx_model *m, Material &mat, bool plastic_energy,
+                       DevBuf<double> &a, DevBuf<double> &b) {
+  const bool plastic = mat.law == AKX_LAW_PLASTIC_LIH;
+  if (plastic_energy && !plastic)
+    return 0.; // Material::getEnergy gives 0 for ids it does not know
+  if (mat.n == 0)
+    return 0.;
+  EnergyArgs A{};
+  A.n = mat.n;
+  A.first = mat.first;
+  A.nn = m->nn;
+  A.nq_all = mat.n * m->nqp;
+  A.filter = mat.filter.p;
+  A.conn = m->conn.p;
+  A.X = m->X.p;
+  A.lambda = mat.lambda;
+  A.mu = mat.mu;
+  A.lam_el = mat.per_element ? mat.lam_el.p : nullptr;
+  A.mu_el = mat.per_element ? mat.mu_el.p : nullptr;
+  if (plastic) {
+    A.g = mat.hgrad[mat.cur].p;
+    A.s = mat.hsig[mat.cur].p;
+    A.eps = mat.heps[mat.cur].p;
+    if (plastic_energy)
+      A.pe = mat.hpe[mat.cur].p;
+  } else {
+    A.g = mat.grad.p;
+    A.s = mat.stress.p;
+  }
+  require(A.conn && A.X && (A.pe || (A.g && A.s)), "energy inputs missing");
+  a.reserve(mat.n);
+  b.reserve((mat.n + 2047) / 2048 + 1);
+  A.out = a.p;
+  if (m->etype == AKX_TET4)
+    launch_energy_elements<ET_TET4>(m, mat.law, A);
+  else
+    launch_energy_elements<ET_HEX8>(m, mat.law, A);
+  return device_sum(m, a, b, mat.n);
+}
+
+} // namespace
+
+extern "C" {
+
+int akx_model_get_energy(akx_model *m, const char *energy_id, int material_id,
+                         double *energy) {
+  return guard([&] {
+    check_ready(m);
+    require(energy_id != nullptr && energy != nullptr, "null argument");
+    require(material_id >= -1 && material_id < (int)m->mats.size(), "bad material id");
+    m->set_device();
+    const std::string id(energy_id);
+    DevBuf<double> a, b;
+    if (id == "kinetic" || id == "external work") {
+      require(material_id == -1, "kinetic energy and external work are model energies");
+      const int kind = id == "kinetic" ? 0 : 1;
+      a.reserve(m->nn);
+      b.reserve((m->nn + 2047) / 2048 + 1);
+      hipLaunchKernelGGL(k_node_energy, dim3(grid_for(m->nn)), dim3(TPB), 0, m->stream,
+                         m->nn, kind, (const double *)m->v.p, (const double *)m->mass.p,
+                         (const double *)m->fint.p,
+                         (const double *)(m->has_fext ? m->fext.p : nullptr),
+                         (const uint8_t *)m->blocked.p, a.p);
+      HIPCHECK(hipGetLastError());
+      const double s = device_sum(m, a, b, m->nn);
+      *energy = kind == 0 ? s / 2. : s * m->dt;
+      return;
+    }
+    if (id != "potential" && id != "plastic") {
+      *energy = 0.; // the reference's Material::getEnergy returns 0 for other ids
+      return;
+    }
+    const bool plastic_energy = id == "plastic";
+    if (!plastic_energy) {
+      // the potential energy reads the stress of the current displacement
+      bool need = false;
+      for (auto &mp : m->mats)
+        need = need || mp->law != AKX_LAW_PLASTIC_LIH;
+      if (need) {
+        for (auto &mp : m->mats)
+          if (mp->law != AKX_LAW_PLASTIC_LIH) {
+            mp->grad.alloc(9 * mp->n * m->nqp);
+            mp->stress.alloc(9 * mp->n * m->nqp);
+          } else {
+            mp->cur = 1 - mp->cur;
+          }
+        launch_element_pass(m, true);
+        flip_plastic_state(m);
+      }
+    }
+    double e = 0.;
+    for (int k = 0; k < (int)m->mats.size(); ++k)
+      if (material_id == -1 || material_id == k)
+        e += material_energy(m, *m->mats[k], plastic_energy, a, b);
+    *energy = e;
   });
 }
 

@@ -83,6 +83,10 @@ class Material:
     def getIsoHardening(self):
         return self._internal(N.INTERNAL_ISO_HARDENING, 1)[:, 0]
 
+    def getEnergy(self, energy_id: str) -> float:
+        """Material::getEnergy: "potential" or "plastic" (0 otherwise)."""
+        return self._model._energy(energy_id, self.id)
+
 
 class SolidMechanicsModel:
     """One mesh of one element type on one gfx950 device."""
@@ -242,6 +246,21 @@ class SolidMechanicsModel:
 
     def getBlockedDOFs(self):
         return self._get("blocked_dofs")
+
+    # ------------------------------------------------------------- energies
+    def _energy(self, energy_id, material_id):
+        self._push()
+        out = C.c_double()
+        N.check(N.lib().akx_model_get_energy(self._h, energy_id.encode(), int(material_id),
+                                             C.byref(out)))
+        if energy_id == "potential":
+            self._stresses_fresh = True
+        return out.value
+
+    def getEnergy(self, energy_id: str) -> float:
+        """SolidMechanicsModel::getEnergy: "kinetic", "external work",
+        "potential", "plastic" (0 for other ids)."""
+        return self._energy(energy_id, -1)
 
     # ----------------------------------------------------------- measurement
     def enableKernelTiming(self, enable=True):

@@ -150,6 +150,16 @@ int akx_model_compute_all_stresses(akx_model *m);
 int akx_model_get_internal(akx_model *m, int material_id, int internal,
                            double *host);
 
+/* SolidMechanicsModel::getEnergy(energy_id) (material_id = -1) or
+ * Material::getEnergy(energy_id) of one material, on the device state:
+ * "kinetic", "external work" (solid_mechanics_model.cc:476-606; model
+ * energies only), "potential" (material.cc:604-619 and the laws' densities),
+ * "plastic" (material_plastic.cc:47-66: the plastic work accumulated after
+ * every step, material_plastic.cc:86-99).  Other ids give 0, as in the
+ * reference.  Sums are deterministic tree sums. */
+int akx_model_get_energy(akx_model *m, const char *energy_id, int material_id,
+                         double *energy);
+
 /* Synchronise the model's stream (all queued steps finished). */
 int akx_model_synchronize(akx_model *m);
 

@@ -41,6 +41,7 @@ class _Model(C.Structure):
         ("grad_u", _dp), ("stress", _dp),
         ("prev_grad_u", _dp), ("prev_stress", _dp), ("eps_p", _dp),
         ("prev_eps_p", _dp), ("iso", _dp), ("prev_iso", _dp),
+        ("plastic_energy", _dp),
         ("dt", C.c_double), ("step", C.c_long),
     ]
 
@@ -72,6 +73,12 @@ def lib():
             getattr(L, fn).argtypes = [M]
         L.orc_solve_steps.restype = None
         L.orc_solve_steps.argtypes = [M, C.c_long]
+        for fn in ("orc_kinetic_energy", "orc_external_work", "orc_potential_energy",
+                   "orc_plastic_energy"):
+            getattr(L, fn).restype = C.c_double
+            getattr(L, fn).argtypes = [M]
+        L.orc_math_reduce.restype = C.c_double
+        L.orc_math_reduce.argtypes = [_dp, C.c_long]
         L.orc_stable_dt.restype = C.c_double
         L.orc_stable_dt.argtypes = [M]
         L.orc_element_inradius.restype = C.c_double
@@ -132,6 +139,7 @@ class OracleModel:
         self.prev_eps_p = np.zeros((nel * nqp, 9)) if plastic else None
         self.iso = np.zeros(nel * nqp) if plastic else None
         self.prev_iso = np.zeros(nel * nqp) if plastic else None
+        self.plastic_energy = np.zeros(nel * nqp) if plastic else None
         cvt = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
         self.E_el, self.nu_el = cvt(E_el), cvt(nu_el)
         self.sigma_y_el, self.h_el = cvt(sigma_y_el), cvt(h_el)
@@ -150,6 +158,7 @@ class OracleModel:
         s.prev_grad_u, s.prev_stress = _ptr(self.prev_grad_u), _ptr(self.prev_stress)
         s.eps_p, s.prev_eps_p = _ptr(self.eps_p), _ptr(self.prev_eps_p)
         s.iso, s.prev_iso = _ptr(self.iso), _ptr(self.prev_iso)
+        s.plastic_energy = _ptr(self.plastic_energy)
         s.dt, s.step = 0.0, 0
         self._s = s
 
@@ -175,6 +184,27 @@ class OracleModel:
     @property
     def step(self) -> int:
         return self._s.step
+
+    def energy(self, energy_id: str) -> float:
+        """SolidMechanicsModel::getEnergy(energy_id) for "kinetic",
+        "external work", "potential" and "plastic" (0 for other ids, as the
+        reference's Material::getEnergy)."""
+        L, s = lib(), C.byref(self._s)
+        if energy_id == "kinetic":
+            return L.orc_kinetic_energy(s)
+        if energy_id == "external work":
+            return L.orc_external_work(s)
+        if energy_id == "potential":
+            return L.orc_potential_energy(s)
+        if energy_id == "plastic":
+            return L.orc_plastic_energy(s)
+        return 0.0
+
+
+def math_reduce(values) -> float:
+    """The reference's Math::reduce (sort, pairwise sums) of a copy."""
+    a = np.array(values, dtype=np.float64)
+    return lib().orc_math_reduce(_ptr(a), a.size)
 
 
 def lumped_step_cb(u, v, a, blocked, mass, dt, residual):

@@ -782,7 +782,166 @@ void orc_solve_step(orc_model *m) {
   }
   solve_and_correct(ndof, m->dt, m->u, m->v, m->a, m->blocked, m->mass, r);
   free(r);
+  /* afterSolveStep -> Material::updateEnergies (material.cc:740-750,
+   * material_plastic.cc:86-99): wp = (sigma + sigma_prev) : (eps_p -
+   * eps_p_prev) / 2, pe += wp (doubleDot = cwiseProduct().sum()) */
+  if (m->law == ORC_PLASTIC_LIH && m->plastic_energy) {
+    long nq = m->nel * orc_nqp(m->type);
+    for (long iq = 0; iq < nq; ++iq) {
+      const double *S = m->stress + 9 * iq, *pS = m->prev_stress + 9 * iq;
+      const double *ep = m->eps_p + 9 * iq, *pep = m->prev_eps_p + 9 * iq;
+      double dd = 0.;
+      for (int k = 0; k < 9; ++k) {
+        double t = (S[k] + pS[k]) * (ep[k] - pep[k]);
+        dd = (k == 0) ? t : dd + t;
+      }
+      m->plastic_energy[iq] += dd / 2.;
+    }
+  }
   ++m->step;
+}
+
+/* -------------------------------------------------------------------------- */
+/* Energies */
+
+static int cmp_double(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return (x > y) - (x < y);
+}
+
+double orc_math_reduce(double *array, long nb_values) {
+  if (nb_values == 0)
+    return 0.;
+  long nb_values_to_sum = nb_values >> 1;
+  qsort(array, (size_t)nb_values, sizeof(double), cmp_double);
+  while (nb_values_to_sum != 0) {
+    long remaining = nb_values - 2 * nb_values_to_sum;
+    if (remaining != 0)
+      array[nb_values - 2] += array[nb_values - 1];
+    for (long i = 0; i < nb_values_to_sum; ++i)
+      array[i] = array[2 * i] + array[2 * i + 1];
+    nb_values = nb_values_to_sum;
+    nb_values_to_sum >>= 1;
+  }
+  return array[0];
+}
+
+/* solid_mechanics_model.cc:476-522 (lumped mass branch) */
+double orc_kinetic_energy(const orc_model *m) {
+  double ekin = 0.;
+  for (long n = 0; n < m->nn; ++n) {
+    double mv2 = 0.;
+    for (int i = 0; i < 3; ++i)
+      if (m->mass[3 * n + i] > DBL_EPSILON)
+        mv2 += m->v[3 * n + i] * m->v[3 * n + i] * m->mass[3 * n + i];
+    ekin += mv2;
+  }
+  return ekin / 2.;
+}
+
+/* solid_mechanics_model.cc:545-583 (dynamic: velocities, times dt) */
+double orc_external_work(const orc_model *m) {
+  double work = 0.;
+  for (long n = 0; n < m->nn; ++n)
+    for (int i = 0; i < 3; ++i) {
+      long d = 3 * n + i;
+      if (m->blocked[d])
+        work -= m->f_int[d] * m->v[d];
+      else
+        work += m->f_ext[d] * m->v[d];
+    }
+  return work * m->dt;
+}
+
+static double double_dot(const double *a, const double *b) {
+  double s = a[0] * b[0];
+  for (int k = 1; k < 9; ++k)
+    s += a[k] * b[k];
+  return s;
+}
+
+/* Potential energy density at one quadrature point:
+ *   elastic small strain  .5 sigma : grad_u (material_elastic_inline_impl
+ *                         .hh:113-116)
+ *   elastic finite        .5 S : E(grad_u) (material_elastic.cc:138-160)
+ *   neo-Hookean           lambda (ln J)^2 + mu (-ln J + (tr C - 3) / 2) / 2
+ *                         (material_neohookean_inline_impl.hh:119-126)
+ *   plastic               .5 sigma : (grad_u - eps_p) (material_plastic.cc
+ *                         :71-83) */
+static double epot_on_quad(const orc_model *m, long iq, double lambda, double mu) {
+  const double *g = m->grad_u + 9 * iq, *S = m->stress + 9 * iq;
+  switch (m->law) {
+  case ORC_ELASTIC:
+    return .5 * double_dot(S, g);
+  case ORC_ELASTIC_FD: {
+    double E[9];
+    grad_u_to_green(g, E);
+    return .5 * double_dot(S, E);
+  }
+  case ORC_NEOHOOKEAN: {
+    mat3 F, C;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        F[i][j] = (i == j ? 1. : 0.) + T(g, i, j);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.;
+        for (int k = 0; k < 3; ++k)
+          s += F[k][i] * F[k][j];
+        C[i][j] = s;
+      }
+    double J = det3_fixed(F);
+    double trC = C[0][0] + C[1][1] + C[2][2];
+    return lambda * pow(log(J), 2.) + mu * (-log(J) + (trC - 3) / 2.) / 2.;
+  }
+  default: {
+    double el[9];
+    for (int k = 0; k < 9; ++k)
+      el[k] = g[k] - m->eps_p[9 * iq + k];
+    return .5 * double_dot(S, el);
+  }
+  }
+}
+
+/* FEEngine::integrate (integrator_gauss_inline_impl.hh:401-487) of a
+ * per-quadrature-point density: per element sum_q f_q wJ_q, then
+ * Math::reduce over the elements. */
+static double integrate_density(const orc_model *m, int plastic) {
+  int nnpe = orc_nnpe(m->type), nqp = orc_nqp(m->type);
+  double *per_el = (double *)malloc(sizeof(double) * (size_t)(m->nel > 0 ? m->nel : 1));
+  double Bl[8 * 24], wJl[8];
+  for (long e = 0; e < m->nel; ++e) {
+    const double *wJ;
+    if (m->wJ) {
+      wJ = m->wJ + e * nqp;
+    } else {
+      double Xe[24];
+      gather_element(m, e, m->X, Xe);
+      element_shape_derivatives(m->type, Xe, Bl, wJl);
+      wJ = wJl;
+    }
+    double lambda, mu, sy, h;
+    element_params(m, e, &lambda, &mu, &sy, &h);
+    double s = 0.;
+    for (int q = 0; q < nqp; ++q) {
+      long iq = e * nqp + q;
+      double f = plastic ? m->plastic_energy[iq] : epot_on_quad(m, iq, lambda, mu);
+      s = (q == 0) ? f * wJ[q] : s + f * wJ[q];
+    }
+    per_el[e] = s;
+  }
+  double r = orc_math_reduce(per_el, m->nel);
+  free(per_el);
+  (void)nnpe;
+  return r;
+}
+
+double orc_potential_energy(const orc_model *m) { return integrate_density(m, 0); }
+
+double orc_plastic_energy(const orc_model *m) {
+  if (m->law != ORC_PLASTIC_LIH || !m->plastic_energy)
+    return 0.;
+  return integrate_density(m, 1);
 }
 
 void orc_solve_steps(orc_model *m, long nsteps) {

@@ -58,6 +58,9 @@ typedef struct orc_model {
   /* plasticity history (NULL for the other laws) */
   double *prev_grad_u, *prev_stress, *eps_p, *prev_eps_p; /* [nel*nqp][9] */
   double *iso, *prev_iso;                                 /* [nel*nqp] */
+  /* plastic energy accumulated by afterSolveStep (material_plastic.cc
+   * :86-99), [nel*nqp]; NULL: not tracked */
+  double *plastic_energy;
 
   double dt;
   long step;
@@ -101,6 +104,17 @@ typedef void (*orc_residual_cb)(void *ctx, double *r /* [ndof], zeroed */);
 void orc_lumped_step_cb(long ndof, double dt, double *u, double *v, double *a,
                         const unsigned char *blocked, const double *mass,
                         orc_residual_cb cb, void *ctx);
+
+/* Energies (solid_mechanics_model.cc:476-606, material.cc:604-649,
+ * material_plastic.cc:47-99): kinetic and external work on the current
+ * nodal state; potential on the current grad_u / stress (plastic: and
+ * inelastic strain); plastic from the accumulated plastic_energy. */
+double orc_kinetic_energy(const orc_model *m);
+double orc_external_work(const orc_model *m);
+double orc_potential_energy(const orc_model *m);
+double orc_plastic_energy(const orc_model *m);
+/* Math::reduce (aka_math.cc:79-107): sort ascending, then pairwise sums. */
+double orc_math_reduce(double *values, long n);
 
 /* Small helpers exposed for closed-form tests. */
 void orc_law_stress(int law, double lambda, double mu, double sigma_y,
