@@ -17,7 +17,7 @@ AKX_OK = 0
 ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "negative jacobian",
           -4: "device out of memory", -5: "unsupported"}
 
-TET4, HEX8 = 0, 1
+TET4, HEX8, TRI3, QUAD4 = 0, 1, 2, 3
 LAW_ELASTIC, LAW_ELASTIC_FD, LAW_NEOHOOKEAN, LAW_PLASTIC_LIH = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_FP32, PRECISION_FP64_FAST = 0, 1, 2
 FIELD_DISPLACEMENT, FIELD_VELOCITY, FIELD_ACCELERATION = 0, 1, 2
@@ -69,6 +69,7 @@ def _declare(lib):
         "akx_model_get_internal": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
         "akx_model_synchronize": (C.c_int, [_vp]),
         "akx_model_get_energy": (C.c_int, [_vp, C.c_char_p, C.c_int, _dp]),
+        "akx_model_apply_traction": (C.c_int, [_vp, C.c_int, _i64, _vp, _dp]),
         "akx_model_enable_kernel_timing": (C.c_int, [_vp, C.c_int]),
         "akx_model_get_kernel_timing": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(_i64)]),
         "akx_model_get_kernel_bytes": (C.c_int, [_vp, C.c_int, _dp]),

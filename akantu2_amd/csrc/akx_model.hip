@@ -1304,6 +1304,120 @@ int akx_model_get_field(akx_model *m, int field, void *host) {
   });
 }
 
+} // extern "C"
+
+namespace {
+
+// BC::Neumann::FromTraction on facets (boundary_condition_tmpl.hh:100-160,
+// boundary_condition_functor_inline_impl.hh:109-113): per facet
+// intf(i, a) = sum_q N_a(q) t_i wJ_q, then f_ext += intf in facet order
+// (dof_manager.cc:65-120).  Facet classes: _triangle_3 (one Gauss point
+// (1/3, 1/3), weight 1/2; element_class_triangle_3_inline_impl.hh:52-90,
+// gauss_integration.cc:56-57) and _quadrangle_4 (2 x 2 Gauss points, xi
+// fastest; element_class_quadrangle_4_inline_impl.hh:64-110); surface
+// jacobian |J_xi x J_eta| (element_class_tmpl.hh:352-371).  Host code: it runs
+// once per boundary condition, on the facets only.
+void traction_on_facets(int facet_type, int64_t nb_facets, const int32_t *conn,
+                        const double *X /* [nn][3] */, int64_t nn, const double *t,
+                        double *fext /* [nn][3] */) {
+  const int nfn = facet_type == AKX_TRI3 ? 3 : 4, nq = facet_type == AKX_TRI3 ? 1 : 4;
+  double xi[4][2], w[4];
+  if (facet_type == AKX_TRI3) {
+    xi[0][0] = 1. / 3.;
+    xi[0][1] = 1. / 3.;
+    w[0] = 1. / 2.;
+  } else {
+    const double g = 1. / std::sqrt(3.), pos[2] = {-g, g};
+    for (int q = 0; q < 4; ++q) {
+      xi[q][0] = pos[q & 1];
+      xi[q][1] = pos[q >> 1];
+      w[q] = 1. * 1.;
+    }
+  }
+  for (int64_t f = 0; f < nb_facets; ++f) {
+    double Xe[4][3];
+    for (int a = 0; a < nfn; ++a) {
+      const int32_t nd = conn[f * nfn + a];
+      require(nd >= 0 && nd < nn, "facet refers to a missing node");
+      for (int c = 0; c < 3; ++c)
+        Xe[a][c] = X[3 * (int64_t)nd + c];
+    }
+    double intf[12];
+    for (int q = 0; q < nq; ++q) {
+      double N[4], d[2][4];
+      const double x = xi[q][0], y = xi[q][1];
+      if (facet_type == AKX_TRI3) {
+        N[0] = 1 - x - y;
+        N[1] = x;
+        N[2] = y;
+        d[0][0] = -1.; d[0][1] = 1.; d[0][2] = 0.;
+        d[1][0] = -1.; d[1][1] = 0.; d[1][2] = 1.;
+      } else {
+        N[0] = 1. / 4. * (1. - x) * (1. - y);
+        N[1] = 1. / 4. * (1. + x) * (1. - y);
+        N[2] = 1. / 4. * (1. + x) * (1. + y);
+        N[3] = 1. / 4. * (1. - x) * (1. + y);
+        d[0][0] = -1. / 4. * (1. - y);
+        d[0][1] = 1. / 4. * (1. - y);
+        d[0][2] = 1. / 4. * (1. + y);
+        d[0][3] = -1. / 4. * (1. + y);
+        d[1][0] = -1. / 4. * (1. - x);
+        d[1][1] = -1. / 4. * (1. + x);
+        d[1][2] = 1. / 4. * (1. + x);
+        d[1][3] = 1. / 4. * (1. - x);
+      }
+      double J[2][3];
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double sum = d[r][0] * Xe[0][c];
+          for (int a = 1; a < nfn; ++a)
+            sum += d[r][a] * Xe[a][c];
+          J[r][c] = sum;
+        }
+      const double cx = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+      const double cy = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+      const double cz = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      const double wJ = std::sqrt(cx * cx + cy * cy + cz * cz) * w[q];
+      for (int a = 0; a < nfn; ++a)
+        for (int i = 0; i < 3; ++i) {
+          const double v = N[a] * t[i] * wJ;
+          intf[i + 3 * a] = (q == 0) ? v : intf[i + 3 * a] + v;
+        }
+    }
+    for (int a = 0; a < nfn; ++a)
+      for (int i = 0; i < 3; ++i)
+        fext[3 * (int64_t)conn[f * nfn + a] + i] += 1. * intf[i + 3 * a];
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+int akx_model_apply_traction(akx_model *m, int facet_type, int64_t nb_facets,
+                             const int32_t *facet_conn, const double *traction) {
+  return guard([&] {
+    require(m != nullptr && m->etype >= 0, "set the mesh first");
+    require(facet_type == AKX_TRI3 || facet_type == AKX_QUAD4, "facet type must be "
+                                                               "AKX_TRI3 or AKX_QUAD4");
+    require(nb_facets >= 0 && (nb_facets == 0 || facet_conn) && traction,
+            "null argument");
+    for (int i = 0; i < 3; ++i)
+      require(std::isfinite(traction[i]), "non-finite traction");
+    m->set_device();
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    std::vector<double> X(3 * m->nn), f(3 * m->nn, 0.);
+    download_soa(m, (const double *)m->X.p, X.data(), m->nn, 3);
+    if (m->has_fext)
+      download_soa(m, (const double *)m->fext.p, f.data(), m->nn, 3);
+    traction_on_facets(facet_type, nb_facets, facet_conn, X.data(), m->nn, traction,
+                       f.data());
+    m->fext.alloc(3 * m->nn);
+    m->has_fext = true;
+    upload_soa(m, (const double *)f.data(), m->fext.p, m->nn, 3);
+  });
+}
+
 int akx_model_set_field(akx_model *m, int field, const void *host) {
   return guard([&] {
     require(m != nullptr && m->etype >= 0, "set the mesh first");

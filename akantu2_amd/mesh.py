@@ -24,6 +24,8 @@ import numpy as np
 
 TET4 = 0
 HEX8 = 1
+TRI3 = 2   # _triangle_3, boundary facet of Tet4
+QUAD4 = 3  # _quadrangle_4, boundary facet of Hex8
 ELEMENT_TYPES = {"_tetrahedron_4": TET4, "_hexahedron_8": HEX8}
 NNPE = {TET4: 4, HEX8: 8}
 NQP = {TET4: 1, HEX8: 8}
@@ -143,6 +145,36 @@ def cells_for(etype, nb_elements, aspect=1.0):
     ny = max(1, int(round((cells / aspect) ** (1.0 / 3.0))))
     nx = max(1, int(round(cells / (ny * ny))))
     return nx, ny, ny
+
+
+# local faces of each element type, each in a cyclic order (facet node order)
+_FACES = {TET4: [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)],
+          HEX8: [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5),
+                 (2, 3, 7, 6), (3, 0, 4, 7)]}
+
+
+def boundary_facets(mesh: BoxMesh) -> np.ndarray:
+    """Faces that belong to exactly one element, [n][3] (Tet4) or [n][4]
+    (Hex8), in the element's local cyclic order."""
+    faces = np.concatenate([mesh.conn[:, list(f)] for f in _FACES[mesh.etype]])
+    key = np.sort(faces, axis=1)
+    _, idx, counts = np.unique(key, axis=0, return_index=True, return_counts=True)
+    return np.ascontiguousarray(faces[np.sort(idx[counts == 1])], dtype=np.int32)
+
+
+def add_group(mesh: BoxMesh, name, group):
+    """Register an element / node group (Mesh::createElementGroup)."""
+    mesh.meta.setdefault("groups", {})[name] = group
+
+
+def face_group(mesh: BoxMesh, axis: int, value: float, tol: float = 1e-12) -> dict:
+    """The boundary facets lying on the plane X[axis] = value, as a group
+    {"facet_type", "facets", "nodes"}."""
+    f = boundary_facets(mesh)
+    on = np.all(np.abs(mesh.nodes[f, axis] - value) <= tol, axis=1)
+    f = f[on]
+    return {"facet_type": TRI3 if mesh.etype == TET4 else QUAD4, "facets": f,
+            "nodes": np.unique(f)}
 
 
 def clamped_face_x0(mesh: BoxMesh) -> np.ndarray:

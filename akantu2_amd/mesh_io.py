@@ -3,15 +3,20 @@
 
 gmsh element type 4 is ``_tetrahedron_4`` and 5 is ``_hexahedron_8``; both
 keep gmsh's node order (mesh_io_msh.cc:200-263 leaves them unpermuted).
-Node tags are renumbered to 0-based indices in file order.
+Node tags are renumbered to 0-based indices in file order.  Boundary facets
+(2 = ``_triangle_3``, 3 = ``_quadrangle_4``) become element groups named by
+their physical name, or their physical tag, or (tag 0) their elementary tag,
+as the reference's groups built from the mesh data "tag_0" / "tag_1".
 """
 from __future__ import annotations
 
 import numpy as np
 
-from .mesh import HEX8, TET4, BoxMesh
+from .mesh import HEX8, TET4, BoxMesh, add_group
 
 _MSH_TYPES = {4: (TET4, 4), 5: (HEX8, 8)}
+# boundary facets: gmsh 2 = _triangle_3, 3 = _quadrangle_4
+_MSH_FACETS = {2: (2, 3), 3: (3, 4)}
 
 
 def read_msh(path, etype=None):
@@ -23,6 +28,7 @@ def read_msh(path, etype=None):
         lines = fh.read().split("\n")
     it = iter(range(len(lines)))
     nodes, tag_to_idx, elems = [], {}, {}
+    facets, names = {}, {}
     i = 0
     while i < len(lines):
         tok = lines[i].strip()
@@ -38,6 +44,12 @@ def read_msh(path, etype=None):
                 tag_to_idx[int(p[0])] = k
                 nodes.append([float(p[1]), float(p[2]), float(p[3])])
             i += 2 + n
+        elif tok == "$PhysicalNames":
+            n = int(lines[i + 1])
+            for k in range(n):
+                p = lines[i + 2 + k].split(maxsplit=2)
+                names[int(p[1])] = p[2].strip().strip('"')
+            i += 2 + n
         elif tok == "$Elements":
             n = int(lines[i + 1])
             for k in range(n):
@@ -47,6 +59,12 @@ def read_msh(path, etype=None):
                     et, nn = _MSH_TYPES[mtype]
                     c = p[3 + ntags: 3 + ntags + nn]
                     elems.setdefault(et, []).append([tag_to_idx[t] for t in c])
+                elif mtype in _MSH_FACETS and ntags >= 1:
+                    ft, nn = _MSH_FACETS[mtype]
+                    # physical tag, or the elementary one when it is 0
+                    tag = p[3] if p[3] != 0 or ntags < 2 else p[4]
+                    c = p[3 + ntags: 3 + ntags + nn]
+                    facets.setdefault((tag, ft), []).append([tag_to_idx[t] for t in c])
             i += 2 + n
         else:
             i += 1
@@ -57,7 +75,12 @@ def read_msh(path, etype=None):
         etype = sorted(elems)[0]
     conn = np.asarray(elems[etype], dtype=np.int32)
     X = np.asarray(nodes, dtype=np.float64)
-    return BoxMesh(etype, X, conn, shape=(0, 0, 0), meta={"file": str(path)})
+    mesh = BoxMesh(etype, X, conn, shape=(0, 0, 0), meta={"file": str(path)})
+    for (tag, ft), fc in sorted(facets.items()):
+        fc = np.asarray(fc, dtype=np.int32)
+        add_group(mesh, names.get(tag, str(tag)),
+                  {"facet_type": ft, "facets": fc, "nodes": np.unique(fc)})
+    return mesh
 
 
 def boundary_nodes(mesh) -> np.ndarray:

@@ -247,6 +247,26 @@ class SolidMechanicsModel:
     def getBlockedDOFs(self):
         return self._get("blocked_dofs")
 
+    # -------------------------------------------------- boundary conditions
+    def applyBC(self, bc, group):
+        """BoundaryCondition::applyBC(functor, group): ``group`` is a group
+        name of the mesh or a group dict {"nodes", "facet_type", "facets"}."""
+        from . import bc as BC
+        if isinstance(group, str):
+            group = self.mesh.meta["groups"][group]
+        if isinstance(bc, BC.Neumann.FromTraction):
+            f = np.ascontiguousarray(group["facets"], np.int32)
+            t = np.ascontiguousarray(bc.traction, np.float64)
+            self._push()
+            N.check(N.lib().akx_model_apply_traction(
+                self._h, int(group["facet_type"]), f.shape[0], _p(f),
+                t.ctypes.data_as(C.POINTER(C.c_double))))
+            self._stale.add("external_force")
+            self._mirrors.pop("external_force", None)
+            return
+        nodes = np.asarray(group["nodes"])
+        bc.apply(self.getBlockedDOFs(), self.getDisplacement(), nodes)
+
     # ------------------------------------------------------------- energies
     def _energy(self, energy_id, material_id):
         self._push()

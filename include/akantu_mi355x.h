@@ -48,8 +48,10 @@ extern "C" {
 #define AKX_ERR_UNSUPPORTED (-5)
 
 /* ElementType subset (src/common/aka_element_classes_info.hh) */
-#define AKX_TET4 0 /* _tetrahedron_4 */
-#define AKX_HEX8 1 /* _hexahedron_8 */
+#define AKX_TET4 0  /* _tetrahedron_4 */
+#define AKX_HEX8 1  /* _hexahedron_8 */
+#define AKX_TRI3 2  /* _triangle_3, boundary facet of Tet4 */
+#define AKX_QUAD4 3 /* _quadrangle_4, boundary facet of Hex8 */
 
 /* Constitutive laws registered in the material factory under the
  * reference's names: "elastic" (material_elastic.cc:216-217), "neohookean"
@@ -144,6 +146,14 @@ int akx_model_assemble_internal_forces(akx_model *m);
 int akx_model_get_nb_nodes(akx_model *m, int64_t *nb_nodes);
 int akx_model_get_field(akx_model *m, int field, void *host);
 int akx_model_set_field(akx_model *m, int field, const void *host);
+/* BC::Neumann::FromTraction applied on a group of boundary facets
+ * (boundary_condition_tmpl.hh:100-160, boundary_condition_functor_inline_impl
+ * .hh:109-113): f_ext += the integral of N^T t over the facets, added in
+ * facet order.  facet_type AKX_TRI3 ([nb][3]) or AKX_QUAD4 ([nb][4]);
+ * traction[3] constant. */
+int akx_model_apply_traction(akx_model *m, int facet_type, int64_t nb_facets,
+                             const int32_t *facet_connectivity,
+                             const double *traction);
 /* Material::computeAllStresses on the current displacement, then copy one
  * internal of one material to the host. */
 int akx_model_compute_all_stresses(akx_model *m);

@@ -77,6 +77,8 @@ def lib():
                    "orc_plastic_energy"):
             getattr(L, fn).restype = C.c_double
             getattr(L, fn).argtypes = [M]
+        L.orc_apply_traction.restype = None
+        L.orc_apply_traction.argtypes = [M, C.c_int, C.c_long, _ip, _dp]
         L.orc_math_reduce.restype = C.c_double
         L.orc_math_reduce.argtypes = [_dp, C.c_long]
         L.orc_stable_dt.restype = C.c_double
@@ -184,6 +186,14 @@ class OracleModel:
     @property
     def step(self) -> int:
         return self._s.step
+
+    def apply_traction(self, facet_type, facets, traction):
+        """BC::Neumann::FromTraction on facets [n][3] (Tri3, type 2) or
+        [n][4] (Quad4, type 3)."""
+        f = np.ascontiguousarray(facets, dtype=np.int32)
+        t = np.ascontiguousarray(traction, dtype=np.float64)
+        lib().orc_apply_traction(C.byref(self._s), int(facet_type), f.shape[0],
+                                 _ptr(f, _ip), _ptr(t))
 
     def energy(self, energy_id: str) -> float:
         """SolidMechanicsModel::getEnergy(energy_id) for "kinetic",

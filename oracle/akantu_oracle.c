@@ -802,6 +802,85 @@ void orc_solve_step(orc_model *m) {
 }
 
 /* -------------------------------------------------------------------------- */
+/* Neumann boundary condition, constant traction on facets.
+ * Facet shapes: element_class_triangle_3_inline_impl.hh:52-90 (N = 1 - xi -
+ * eta, xi, eta; one Gauss point (1/3, 1/3), weight 1/2, gauss_integration.cc
+ * :56-57) and element_class_quadrangle_4_inline_impl.hh:64-110 (bilinear, 2 x 2
+ * Gauss points +-1/sqrt(3), xi fastest, weights 1).  The surface jacobian is
+ * |J_xi x J_eta| (element_class_tmpl.hh:352-371), times the weight
+ * (integrator_gauss_inline_impl.hh:379-398).  Per facet: computeNtb gives
+ * N_a t_i at i + 3a, integrate sums over q (q order), the result is added to
+ * f_ext node by node (dof_manager.cc:65-120). */
+void orc_apply_traction(orc_model *m, int facet_type, long nb_facets, const int *conn,
+                        const double *t) {
+  const int nn = facet_type == 2 ? 3 : 4, nq = facet_type == 2 ? 1 : 4;
+  double xi[4][2], w[4];
+  if (facet_type == 2) {
+    xi[0][0] = 1. / 3.;
+    xi[0][1] = 1. / 3.;
+    w[0] = 1. / 2.;
+  } else {
+    const double g = 1. / sqrt(3.), pos[2] = {-g, g};
+    for (int q = 0; q < 4; ++q) {
+      xi[q][0] = pos[q & 1];
+      xi[q][1] = pos[q >> 1];
+      w[q] = 1. * 1.;
+    }
+  }
+  for (long f = 0; f < nb_facets; ++f) {
+    double Xe[4][3];
+    for (int a = 0; a < nn; ++a)
+      for (int c = 0; c < 3; ++c)
+        Xe[a][c] = m->X[3 * (long)conn[f * nn + a] + c];
+    double intf[12];
+    for (int q = 0; q < nq; ++q) {
+      double N[4], d[2][4];
+      const double x = xi[q][0], y = xi[q][1];
+      if (facet_type == 2) {
+        N[0] = 1 - x - y;
+        N[1] = x;
+        N[2] = y;
+        d[0][0] = -1.; d[0][1] = 1.; d[0][2] = 0.;
+        d[1][0] = -1.; d[1][1] = 0.; d[1][2] = 1.;
+      } else {
+        N[0] = 1. / 4. * (1. - x) * (1. - y);
+        N[1] = 1. / 4. * (1. + x) * (1. - y);
+        N[2] = 1. / 4. * (1. + x) * (1. + y);
+        N[3] = 1. / 4. * (1. - x) * (1. + y);
+        d[0][0] = -1. / 4. * (1. - y);
+        d[0][1] = 1. / 4. * (1. - y);
+        d[0][2] = 1. / 4. * (1. + y);
+        d[0][3] = -1. / 4. * (1. + y);
+        d[1][0] = -1. / 4. * (1. - x);
+        d[1][1] = -1. / 4. * (1. + x);
+        d[1][2] = 1. / 4. * (1. + x);
+        d[1][3] = 1. / 4. * (1. - x);
+      }
+      double J[2][3];
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double s = d[r][0] * Xe[0][c];
+          for (int a = 1; a < nn; ++a)
+            s += d[r][a] * Xe[a][c];
+          J[r][c] = s;
+        }
+      const double cx = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+      const double cy = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+      const double cz = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      const double wJ = sqrt(cx * cx + cy * cy + cz * cz) * w[q];
+      for (int a = 0; a < nn; ++a)
+        for (int i = 0; i < 3; ++i) {
+          const double v = N[a] * t[i] * wJ;
+          intf[i + 3 * a] = (q == 0) ? v : intf[i + 3 * a] + v;
+        }
+    }
+    for (int a = 0; a < nn; ++a)
+      for (int i = 0; i < 3; ++i)
+        m->f_ext[3 * (long)conn[f * nn + a] + i] += 1. * intf[i + 3 * a];
+  }
+}
+
+/* -------------------------------------------------------------------------- */
 /* Energies */
 
 static int cmp_double(const void *a, const void *b) {

@@ -113,6 +113,14 @@ double orc_kinetic_energy(const orc_model *m);
 double orc_external_work(const orc_model *m);
 double orc_potential_energy(const orc_model *m);
 double orc_plastic_energy(const orc_model *m);
+/* BC::Neumann::FromTraction on a set of boundary facets
+ * (boundary_condition_tmpl.hh:100-160, boundary_condition_functor_inline_impl
+ * .hh:109-113): f_ext += assemble(integrate(N^T t)) over the facets, in facet
+ * order.  facet_type 2 = _triangle_3 (1 Gauss point), 3 = _quadrangle_4
+ * (2 x 2); conn [nb_facets][3 or 4]. */
+void orc_apply_traction(orc_model *m, int facet_type, long nb_facets, const int *conn,
+                        const double *traction);
+
 /* Math::reduce (aka_math.cc:79-107): sort ascending, then pairwise sums. */
 double orc_math_reduce(double *values, long n);
 
