@@ -149,6 +149,7 @@ class SolidMechanicsModel:
                 self._h, mid.value, *[_p(a) for a in arrs]))
         mat = Material(self, mid.value, lid,
                        dict(rho=rho, E=E, nu=nu, sigma_y=sigma_y, h=h), n)
+        mat.element_ids = None if filt is None else filt.copy()
         self.materials.append(mat)
         return mat
 
@@ -246,6 +247,26 @@ class SolidMechanicsModel:
 
     def getBlockedDOFs(self):
         return self._get("blocked_dofs")
+
+    # ---------------------------------------------------------------- dumps
+    def setBaseName(self, name, directory="paraview"):
+        """Model::setBaseName: ParaView output under ``directory``."""
+        from .dumper import DumperParaview
+        fields = self._dumper.fields if getattr(self, "_dumper", None) else []
+        self._dumper = DumperParaview(self, name, directory)
+        for f in fields:
+            self._dumper.addDumpField(f)
+
+    def addDumpField(self, name):
+        if getattr(self, "_dumper", None) is None:
+            self.setBaseName("solid_mechanics_model")
+        self._dumper.addDumpField(name)
+
+    def dump(self, time=None, step=None):
+        """Model::dump: write the registered fields; returns the file."""
+        if getattr(self, "_dumper", None) is None:
+            self.setBaseName("solid_mechanics_model")
+        return self._dumper.dump(time, step)
 
     # -------------------------------------------------- boundary conditions
     def applyBC(self, bc, group):

@@ -46,3 +46,34 @@ def test_traction_loaded_bar(etype, law):
     assert np.abs(o.u).max() > 0
     assert g.getEnergy("external work") == pytest.approx(o.energy("external work"),
                                                          rel=1e-12)
+
+
+def test_paraview_dump_of_device_fields(tmp_path):
+    import base64
+    import struct
+    import xml.etree.ElementTree as ET
+    mesh = M.box_mesh("_hexahedron_8", 4, 2, 2, jitter=0.1, seed=1)
+    g = SolidMechanicsModel(mesh)
+    g.addMaterial("neohookean", rho=1.0, E=1.0, nu=0.3)
+    g.initFull()
+    g.getBlockedDOFs()[:] = M.clamped_face_x0(mesh)
+    g.getVelocity()[:] = M.initial_velocity(mesh, 1e-3, 1)
+    g.setTimeStep(0.5 * g.getStableTimeStep())
+    g.setBaseName("bar", str(tmp_path))
+    for f in ("displacement", "velocity", "internal_force", "stress"):
+        g.addDumpField(f)
+    g.solveStep(20)
+    path = g.dump()
+    root = ET.parse(path).getroot()
+    arrays = {}
+    for da in root.iter("DataArray"):
+        raw = base64.b64decode(da.text.strip())
+        n = struct.unpack("<Q", raw[:8])[0]
+        arrays[da.get("Name")] = np.frombuffer(raw[8:8 + n], dtype=np.float64 if
+                                               da.get("type") == "Float64" else np.int64)
+    np.testing.assert_array_equal(arrays["displacement"].reshape(-1, 3), g.getDisplacement())
+    np.testing.assert_array_equal(arrays["internal_force"].reshape(-1, 3),
+                                  g.getInternalForce())
+    np.testing.assert_array_equal(arrays["stress"].reshape(mesh.nb_elements, -1),
+                                  g.getMaterial(0).getStress().reshape(mesh.nb_elements, -1))
+    assert (tmp_path / "bar.pvd").exists()
