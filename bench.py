@@ -81,10 +81,20 @@ def rel_errors(a, b):
 # --------------------------------------------------------------------------
 # plain form (headline)
 # --------------------------------------------------------------------------
+def host_threads():
+    """Host cores the CPU baseline may use (the box's share for one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline_plain(shape=(20, 100, 100), steps=10):
-    """The CPU oracle (plain C, one thread, FP64, reference dataflow with
-    precomputed shape derivatives) timed on a bounded sample of the same
-    workload: the first 20 cells in x of the 200 x 100 x 100 grid."""
+    """The CPU oracle (plain C, FP64, reference dataflow with precomputed
+    shape derivatives) timed on a bounded sample of the same workload: the
+    first 20 cells in x of the 200 x 100 x 100 grid, on one thread and on
+    the host's cores (OpenMP element and DOF loops; results identical)."""
     import oracle
     mesh, blocked, v0 = build_c2(shape)
     o = oracle.OracleModel(oracle.HEX8, mesh.nodes, mesh.conn, "neohookean")
@@ -92,15 +102,23 @@ def cpu_baseline_plain(shape=(20, 100, 100), steps=10):
     o.blocked[:] = blocked
     o.v[:] = v0
     o.set_time_step(0.5 * o.stable_dt())
-    t0 = time.perf_counter()
-    o.solve_steps(steps)
-    el = time.perf_counter() - t0
-    value = mesh.nb_elements * 8 * steps / el
-    return {"value": value, "unit": "elem*qp/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps of a {shape[0]}x{shape[1]}x{shape[2]} Hex8 "
-                      f"slab ({mesh.nb_elements} elements) of the same jittered "
-                      "neo-Hookean workload, oracle/akantu_oracle.c, 1 thread",
-            "seconds": el}
+    runs = {}
+    nt = host_threads()
+    for threads, nsteps in ((1, steps), (nt, 4 * steps)):
+        oracle.set_threads(threads)
+        t0 = time.perf_counter()
+        o.solve_steps(nsteps)
+        runs[threads] = (mesh.nb_elements * 8 * nsteps / (time.perf_counter() - t0),
+                         time.perf_counter() - t0, nsteps)
+    oracle.set_threads(1)
+    v1, s1, n1 = runs[1]
+    vn, sn, nn_ = runs[nt]
+    return {"value": vn, "unit": "elem*qp/s", "cores": nt, "kind": "port",
+            "sample": f"{nn_} steps of a {shape[0]}x{shape[1]}x{shape[2]} Hex8 slab "
+                      f"({mesh.nb_elements} elements) of the same jittered neo-Hookean "
+                      f"workload, oracle/akantu_oracle.c, {nt} OpenMP threads",
+            "seconds": sn,
+            "single_core": {"value": v1, "cores": 1, "steps": n1, "seconds": s1}}
 
 
 def run_plain(args):

@@ -77,6 +77,10 @@ def lib():
                    "orc_plastic_energy"):
             getattr(L, fn).restype = C.c_double
             getattr(L, fn).argtypes = [M]
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
+        L.orc_get_threads.argtypes = []
         L.orc_apply_traction.restype = None
         L.orc_apply_traction.argtypes = [M, C.c_int, C.c_long, _ip, _dp]
         L.orc_math_reduce.restype = C.c_double
@@ -209,6 +213,11 @@ class OracleModel:
         if energy_id == "plastic":
             return L.orc_plastic_energy(s)
         return 0.0
+
+
+def set_threads(n: int):
+    """Threads of the oracle's element and DOF loops (results unchanged)."""
+    lib().orc_set_threads(int(n))
 
 
 def math_reduce(values) -> float:

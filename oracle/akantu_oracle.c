@@ -27,6 +27,13 @@
 #include <string.h>
 
 /* -------------------------------------------------------------------------- */
+/* Threads of the element and DOF loops (the CPU baseline's multi-core run).
+ * Every element's arithmetic and the assembly order are those of the
+ * one-thread run, so the results are bit-identical for any count. */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+int orc_get_threads(void) { return g_threads; }
+
 int orc_nnpe(int type) { return type == ORC_TET4 ? 4 : 8; }
 int orc_nqp(int type) { return type == ORC_TET4 ? 1 : 8; }
 
@@ -553,8 +560,9 @@ static int law_is_fd(int law) {
  * then computeStress (material.hh:78). */
 void orc_compute_all_stresses(orc_model *m) {
   int nnpe = orc_nnpe(m->type), nqp = orc_nqp(m->type);
-  double Bl[8 * 24], wJl[8];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
   for (long e = 0; e < m->nel; ++e) {
+    double Bl[8 * 24], wJl[8];
     double ue[24];
     gather_element(m, e, m->u, ue);
     const double *B;
@@ -646,13 +654,45 @@ static void fd_qp_force(int nnpe, const double *Bq, const double *g,
  * then DOFManager::assembleElementalArrayLocalArray (dof_manager.cc:65-120)
  * with scale -1 in ascending element order.  f_int is zeroed first
  * (solid_mechanics_model.cc:338). */
+static void element_force(const orc_model *m, long e, double *re);
+
 void orc_assemble_internal_forces(orc_model *m) {
+  int nnpe = orc_nnpe(m->type);
+  int ncol = 3 * nnpe;
+  memset(m->f_int, 0, sizeof(double) * 3 * m->nn);
+  if (g_threads > 1) {
+    /* elemental forces in parallel, then the scatter in element order */
+    double *all = (double *)malloc(sizeof(double) * (size_t)ncol * (size_t)(m->nel > 0 ? m->nel : 1));
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (long e = 0; e < m->nel; ++e)
+      element_force(m, e, all + e * ncol);
+    for (long e = 0; e < m->nel; ++e)
+      for (int n = 0; n < nnpe; ++n) {
+        long node = m->conn[e * nnpe + n];
+        for (int i = 0; i < 3; ++i)
+          m->f_int[3 * node + i] += -1. * all[e * ncol + i + 3 * n];
+      }
+    free(all);
+    return;
+  }
+  for (long e = 0; e < m->nel; ++e) {
+    double re[24];
+    element_force(m, e, re);
+    for (int n = 0; n < nnpe; ++n) {
+      long node = m->conn[e * nnpe + n];
+      for (int i = 0; i < 3; ++i)
+        m->f_int[3 * node + i] += -1. * re[i + 3 * n];
+    }
+  }
+}
+
+/* r_e = sum_q B_q^T sigma_q wJ_q (or the finite-deformation form) */
+static void element_force(const orc_model *m, long e, double *re) {
   int nnpe = orc_nnpe(m->type), nqp = orc_nqp(m->type);
   int ncol = 3 * nnpe;
   double Bl[8 * 24], wJl[8];
-  memset(m->f_int, 0, sizeof(double) * 3 * m->nn);
   int fd = law_is_fd(m->law);
-  for (long e = 0; e < m->nel; ++e) {
+  {
     const double *B;
     const double *wJ = m->wJ + e * nqp;
     if (m->B) {
@@ -681,17 +721,11 @@ void orc_assemble_internal_forces(orc_model *m) {
           }
       }
     }
-    double re[24];
     for (int c = 0; c < ncol; ++c) {
       double s = 0.;
       for (int q = 0; q < nqp; ++q)
         s += fq[q][c] * wJ[q];
       re[c] = s;
-    }
-    for (int n = 0; n < nnpe; ++n) {
-      long node = m->conn[e * nnpe + n];
-      for (int i = 0; i < 3; ++i)
-        m->f_int[3 * node + i] += -1. * re[i + 3 * n];
     }
   }
 }
@@ -715,6 +749,7 @@ void orc_internal_forces(orc_model *m) {
 static void predictor(long ndof, double dt, double *u, double *v, double *a,
                       const unsigned char *blocked) {
   const double k = 0., h = 0., alpha = 0., beta = .5;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
   for (long d = 0; d < ndof; ++d) {
     if (!blocked[d]) {
       double dt_a_n = dt * a[d];
@@ -730,12 +765,14 @@ static void solve_and_correct(long ndof, double dt, double *u, double *v,
                               const double *mass, double *r) {
   const double alpha = 0., beta = .5;
   /* r += M * cache, cache = 0 + (-1) * a */
+#pragma omp parallel for num_threads(g_threads) schedule(static)
   for (long d = 0; d < ndof; ++d) {
     double cache = 0.;
     cache += -1. * a[d];
     r[d] += mass[d] * cache;
   }
   double c = 1., dd = beta * dt, e = alpha * beta * dt * dt;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
   for (long d = 0; d < ndof; ++d) {
     double x = 0.;
     if (!blocked[d])

@@ -66,6 +66,11 @@ typedef struct orc_model {
   long step;
 } orc_model;
 
+/* Threads of the element and DOF loops (default 1); results are
+ * bit-identical for every count. */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 int orc_nnpe(int type);
 int orc_nqp(int type);
 

@@ -383,3 +383,28 @@ def test_c1_energy_is_conserved():
         es.append(0.5 * np.sum(o.mass * o.v * o.v) - 0.5 * np.sum(o.u * o.f_int))
     drift = np.abs(np.array(es) - e0).max() / e0
     assert drift < 2e-2
+
+
+def test_threaded_oracle_is_bit_identical():
+    """The oracle's OpenMP loops (the multi-core CPU baseline) leave every
+    element's arithmetic and the assembly order as in the one-thread run."""
+    from akantu2_amd import mesh as M
+    out = []
+    for threads in (1, 4):
+        oracle.set_threads(threads)
+        try:
+            for etype, law, kw in (("_hexahedron_8", "neohookean", {}),
+                                   ("_tetrahedron_4", "plastic_linear_isotropic_hardening",
+                                    dict(sigma_y=2e-3, h=0.05))):
+                m = M.box_mesh(etype, 6, 3, 3, jitter=0.1, seed=4)
+                o = oracle.OracleModel(m.etype, m.nodes, m.conn, law, **kw)
+                o.init_full()
+                o.blocked[:] = M.clamped_face_x0(m)
+                o.v[:] = M.initial_velocity(m, 3e-2, 4)
+                o.set_time_step(0.5 * o.stable_dt())
+                o.solve_steps(60)
+                out.append((o.u.copy(), o.f_int.copy()))
+        finally:
+            oracle.set_threads(1)
+    for a, b in zip(out[:2], out[2:]):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
