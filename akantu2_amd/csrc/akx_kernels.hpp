@@ -623,7 +623,8 @@ __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
 // :260-278, non_linear_solver_lumped.cc:76-89, newmark-beta.cc:47-77,164-197.
 // ---------------------------------------------------------------------------
 struct GatherArgs {
-  int64_t nn;
+  int64_t nn;               // nodes of the mesh (stride of the [3][nn] fields)
+  int64_t node0, node_end;  // nodes of this launch: [node0, node_end)
   const int64_t *ell_off; // [nn/256 + 1] first entry of each 256-node slice
   const uint8_t *deg;      // [nn] elements around each node
   const void *fe;          // [entry][3]
@@ -646,8 +647,8 @@ __device__ __forceinline__ void kahan_add(double &s, float &c, double inc) {
 
 template <class FT, bool UPDATE, bool PREDICT, bool COMP>
 __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
-  const int64_t n = xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
-  if (n >= A.nn)
+  const int64_t n = A.node0 + xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
+  if (n >= A.node_end)
     return;
   const FT *fe = reinterpret_cast<const FT *>(A.fe);
   double f0 = 0., f1 = 0., f2 = 0.;
@@ -711,8 +712,8 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
 // Predictor alone (first step of a solve_step call): newmark-beta.cc:47-77
 template <bool COMP>
 __global__ void __launch_bounds__(256) k_predictor(GatherArgs A) {
-  const int64_t n = xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
-  if (n >= A.nn)
+  const int64_t n = A.node0 + xcd_block(A.nblocks_padded) * blockDim.x + threadIdx.x;
+  if (n >= A.node_end)
     return;
   const uint8_t blk = A.blocked[n];
 #pragma unroll

@@ -442,6 +442,15 @@ struct akx_model {
   DevBuf<unsigned long long> scratch_u64;
   DevBuf<long long> scratch_i64;
   std::vector<std::unique_ptr<Material>> mats;
+  // pipelined schedule (akx_model_set_pipeline): element parts and the node
+  // groups whose gather they complete, a second stream and the events
+  int pipe_parts = 0;
+  bool pipe_planned = false, pipe_ok = false;
+  std::vector<int64_t> pipe_el;   // [P + 1] element ranges of the parts
+  std::vector<int64_t> pipe_node; // [P + 1] node ranges of the gather groups
+  std::vector<int> pipe_need;     // [P] last group part p's nodes belong to
+  hipStream_t stream2 = nullptr;
+  std::vector<hipEvent_t> ev_elem, ev_gather[2];
   // kernel timing
   bool timing = false;
   double t_ms[3] = {0, 0, 0};
@@ -453,6 +462,13 @@ struct akx_model {
     drain_timing();
     for (auto e : event_pool)
       (void)hipEventDestroy(e);
+    for (auto e : ev_elem)
+      (void)hipEventDestroy(e);
+    for (auto &v : ev_gather)
+      for (auto e : v)
+        (void)hipEventDestroy(e);
+    if (stream2)
+      (void)hipStreamDestroy(stream2);
     if (stream)
       (void)hipStreamDestroy(stream);
   }
@@ -719,6 +735,8 @@ void flip_plastic_state(akx_model *m) {
 GatherArgs gather_args(akx_model *m) {
   GatherArgs G{};
   G.nn = m->nn;
+  G.node0 = 0;
+  G.node_end = m->nn;
   G.ell_off = m->ell_off.p;
   G.deg = m->deg.p;
   G.fe = m->fe.p;
@@ -779,6 +797,170 @@ void launch_predictor(akx_model *m) {
       hipLaunchKernelGGL((k_predictor<true>), grid, block, 0, m->stream, G);
   });
   HIPCHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined step (opt-in): the elements are cut in P consecutive parts and the
+// nodes in P groups, group q = the 256-node slices whose last element (in
+// assembly order) lies in part q.  Then
+//   gather of group q (step k)        needs the element pass of parts <= q
+//   element part p (step k + 1)       needs the gather of the groups its nodes
+//                                     belong to (the last is pipe_need[p])
+// so the memory-bound gather of group q runs on a second stream while the
+// compute-bound element pass of part q + 1 runs.  Values and summation
+// orders are those of the plain schedule: it reorders launches, not
+// arithmetic.  Planned only for one contiguous non-plastic material whose
+// slice groups are monotone (otherwise the plain schedule is used).
+// ---------------------------------------------------------------------------
+void plan_pipeline(akx_model *m) {
+  m->pipe_planned = true;
+  m->pipe_ok = false;
+  const int P = m->pipe_parts;
+  if (P < 2 || m->mats.size() != 1)
+    return;
+  Material &mat = *m->mats[0];
+  if (!mat.contiguous || mat.law == AKX_LAW_PLASTIC_LIH || mat.first != 0 ||
+      mat.n != m->nel || mat.pos_offset != 0 || m->nel < P)
+    return;
+  const int64_t nel = m->nel, nn = m->nn, nnpe = m->nnpe;
+  auto part_of = [&](int64_t e) { return (int)(e * P / nel); };
+  std::vector<int64_t> maxel(nn, -1);
+  for (int64_t e = 0; e < nel; ++e)
+    for (int a = 0; a < nnpe; ++a) {
+      const int32_t nd = m->conn_host[e * nnpe + a];
+      maxel[nd] = std::max<int64_t>(maxel[nd], e);
+    }
+  const int64_t nsl = (nn + 255) / 256;
+  std::vector<int> lp(nsl, 0);
+  int last = 0;
+  for (int64_t s = 0; s < nsl; ++s) {
+    int v = -1;
+    for (int64_t n = s * 256; n < std::min<int64_t>(nn, s * 256 + 256); ++n)
+      if (maxel[n] >= 0)
+        v = std::max(v, part_of(maxel[n]));
+    if (v < 0)
+      v = last;
+    if (v < last)
+      return; // groups would not be contiguous node ranges
+    lp[s] = last = v;
+  }
+  m->pipe_el.assign(P + 1, 0);
+  m->pipe_node.assign(P + 1, nn);
+  for (int p = 0; p <= P; ++p)
+    m->pipe_el[p] = (int64_t)p * nel / P;
+  // part boundaries must agree with part_of
+  for (int p = 0; p < P; ++p)
+    if (m->pipe_el[p] < m->pipe_el[p + 1] && part_of(m->pipe_el[p]) != p)
+      return;
+  m->pipe_node[0] = 0;
+  for (int p = 1; p < P; ++p) {
+    int64_t s = 0;
+    while (s < nsl && lp[s] < p)
+      ++s;
+    m->pipe_node[p] = std::min<int64_t>(nn, s * 256);
+  }
+  m->pipe_node[P] = nn;
+  m->pipe_need.assign(P, 0);
+  for (int p = 0; p < P; ++p) {
+    int64_t smax = 0;
+    for (int64_t e = m->pipe_el[p]; e < m->pipe_el[p + 1]; ++e)
+      for (int a = 0; a < nnpe; ++a)
+        smax = std::max<int64_t>(smax, m->conn_host[e * nnpe + a] / 256);
+    m->pipe_need[p] = lp[smax];
+  }
+  if (!m->stream2)
+    HIPCHECK(hipStreamCreateWithFlags(&m->stream2, hipStreamNonBlocking));
+  auto make = [&](std::vector<hipEvent_t> &v) {
+    while ((int)v.size() < P) {
+      hipEvent_t e;
+      HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      v.push_back(e);
+    }
+  };
+  make(m->ev_elem);
+  make(m->ev_gather[0]);
+  make(m->ev_gather[1]);
+  m->pipe_ok = true;
+}
+
+void launch_element_part(akx_model *m, int p) {
+  Material &mat = *m->mats[0];
+  const int64_t e0 = m->pipe_el[p], cnt = m->pipe_el[p + 1] - e0;
+  if (cnt <= 0)
+    return;
+  const bool strict = m->precision == AKX_PRECISION_FP64;
+  ElemArgs A = elem_args(m, mat, false);
+  A.n = cnt;
+  A.first = mat.first + e0;
+  A.pos_offset = mat.pos_offset + e0;
+  if (A.lam_el) {
+    A.lam_el += e0;
+    A.mu_el += e0;
+  }
+  if (A.sy_el)
+    A.sy_el += e0;
+  if (A.h_el)
+    A.h_el += e0;
+  int64_t epb = strict ? (m->etype == AKX_HEX8 ? HEX_STRICT_EPB : TPB)
+                       : akxf::elements_per_block(m->etype);
+  A.nblocks_padded = ((cnt + epb - 1) / epb + 7) / 8 * 8;
+  check_elem_args(A, mat.law);
+  if (strict) {
+    if (m->etype == AKX_TET4)
+      launch_elem_law<ET_TET4, double, double>(m, mat, A);
+    else
+      launch_hex8_strict(m, mat, A);
+  } else {
+    HIPCHECK(akxf::launch_elements(m->etype, mat.law, m->precision == AKX_PRECISION_FP32,
+                                   A, m->stream));
+  }
+}
+
+void launch_gather_group(akx_model *m, int p, bool predict) {
+  GatherArgs G = gather_args(m);
+  G.node0 = m->pipe_node[p];
+  G.node_end = m->pipe_node[p + 1];
+  if (G.node_end <= G.node0)
+    return;
+  G.nblocks_padded = padded_blocks(G.node_end - G.node0);
+  const bool f64 = m->precision != AKX_PRECISION_FP32;
+  check_gather_args(G, true, !f64);
+  dim3 grid((unsigned)G.nblocks_padded), block(TPB);
+  if (f64) {
+    if (predict)
+      hipLaunchKernelGGL((k_gather_update<double, true, true, false>), grid, block, 0,
+                         m->stream2, G);
+    else
+      hipLaunchKernelGGL((k_gather_update<double, true, false, false>), grid, block, 0,
+                         m->stream2, G);
+  } else {
+    if (predict)
+      hipLaunchKernelGGL((k_gather_update<float, true, true, true>), grid, block, 0,
+                         m->stream2, G);
+    else
+      hipLaunchKernelGGL((k_gather_update<float, true, false, true>), grid, block, 0,
+                         m->stream2, G);
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+void solve_pipelined(akx_model *m, int64_t nb_steps) {
+  const int P = m->pipe_parts;
+  launch_predictor(m);
+  for (int64_t s = 0; s < nb_steps; ++s) {
+    const int cur = (int)(s & 1), prev = cur ^ 1;
+    for (int p = 0; p < P; ++p) {
+      if (s > 0) // the gather of the previous step on this part's nodes
+        HIPCHECK(hipStreamWaitEvent(m->stream, m->ev_gather[prev][m->pipe_need[p]], 0));
+      launch_element_part(m, p);
+      HIPCHECK(hipEventRecord(m->ev_elem[p], m->stream));
+      HIPCHECK(hipStreamWaitEvent(m->stream2, m->ev_elem[p], 0));
+      launch_gather_group(m, p, s + 1 < nb_steps);
+      HIPCHECK(hipEventRecord(m->ev_gather[cur][p], m->stream2));
+    }
+  }
+  // join: the model's stream ends after the last gather
+  HIPCHECK(hipStreamWaitEvent(m->stream, m->ev_gather[(nb_steps - 1) & 1][P - 1], 0));
 }
 
 InitArgs init_args(akx_model *m, Material *mat) {
@@ -1221,12 +1403,33 @@ int akx_model_solve_step(akx_model *m, int64_t nb_steps) {
     if (nb_steps == 0)
       return;
     m->set_device();
+    if (m->pipe_parts > 1 && !m->pipe_planned)
+      plan_pipeline(m);
+    if (m->pipe_parts > 1 && m->pipe_ok && !m->timing) {
+      solve_pipelined(m, nb_steps);
+      return;
+    }
     launch_predictor(m);
     for (int64_t s = 0; s < nb_steps; ++s) {
       launch_element_pass(m, false);
       flip_plastic_state(m);
       launch_gather(m, true, s + 1 < nb_steps);
     }
+  });
+}
+
+int akx_model_set_pipeline(akx_model *m, int parts, int *active) {
+  return guard([&] {
+    check_ready(m);
+    require(parts >= 0 && parts <= 64, "parts must be in [0, 64]");
+    m->set_device();
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    m->pipe_parts = parts;
+    m->pipe_planned = false;
+    if (parts > 1)
+      plan_pipeline(m);
+    if (active)
+      *active = parts > 1 && m->pipe_ok;
   });
 }
 

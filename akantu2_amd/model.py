@@ -303,6 +303,13 @@ class SolidMechanicsModel:
         "potential", "plastic" (0 for other ids)."""
         return self._energy(energy_id, -1)
 
+    def setPipeline(self, parts: int) -> bool:
+        """Pipelined launch schedule (see akx_model_set_pipeline); returns
+        whether it applies to this model."""
+        active = C.c_int(0)
+        N.check(N.lib().akx_model_set_pipeline(self._h, int(parts), C.byref(active)))
+        return bool(active.value)
+
     # ----------------------------------------------------------- measurement
     def enableKernelTiming(self, enable=True):
         N.check(N.lib().akx_model_enable_kernel_timing(self._h, int(enable)))

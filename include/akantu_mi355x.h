@@ -170,6 +170,15 @@ int akx_model_get_internal(akx_model *m, int material_id, int internal,
 int akx_model_get_energy(akx_model *m, const char *energy_id, int material_id,
                          double *energy);
 
+/* Launch schedule of solve_step, no counterpart in the reference: with
+ * parts > 1 the elements are cut in `parts` consecutive parts and the nodal
+ * gather of the nodes a part completes runs on a second stream while the
+ * next part's element pass runs (same values and summation orders as the
+ * plain schedule).  Available for one contiguous non-plastic material whose
+ * node numbering follows the elements (structured meshes); *active reports
+ * whether it applies.  parts = 0 or 1: the plain schedule (default). */
+int akx_model_set_pipeline(akx_model *m, int parts, int *active);
+
 /* Synchronise the model's stream (all queued steps finished). */
 int akx_model_synchronize(akx_model *m);
 

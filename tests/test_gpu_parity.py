@@ -204,3 +204,26 @@ def test_fixture_meshes_mass_and_dt(fixtures_dir):
         g, dtg = make_gpu(mesh, blocked, v0, "elastic", rho=7800.0, E=2.1e11)
         assert dtg == dt
         np.testing.assert_array_equal(g.getMass(), o.mass)
+
+
+@pytest.mark.parametrize("etype,law,shape", [
+    ("_hexahedron_8", "neohookean", (24, 6, 5)),
+    ("_tetrahedron_4", "elastic", (20, 5, 4)),
+])
+def test_pipelined_schedule_is_invisible(etype, law, shape):
+    """The pipelined launch schedule (element parts overlapped with the
+    gather on a second stream) gives the plain schedule's bits and the
+    oracle's values."""
+    mesh, blocked, v0 = workload(etype, shape, jitter=0.1, seed=9)
+    o, dt = make_oracle(mesh, blocked, v0, law)
+    plain, _ = make_gpu(mesh, blocked, v0, law, dt=dt)
+    piped, _ = make_gpu(mesh, blocked, v0, law, dt=dt)
+    assert piped.setPipeline(4)
+    for n in (1, 2, 57):
+        plain.solveStep(n)
+        piped.solveStep(n)
+        o.solve_steps(n)
+        for get in ("getDisplacement", "getVelocity", "getAcceleration",
+                    "getInternalForce"):
+            assert np.array_equal(getattr(plain, get)(), getattr(piped, get)()), get
+        _compare(piped, o, f"pipelined {etype} after {n}")
