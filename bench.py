@@ -141,9 +141,10 @@ def run_plain(args):
           f"{mesh.nb_nodes} nodes, dt {dt:.6g}, {args.precision}", file=sys.stderr,
           flush=True)
 
+    schedule = "plain"
+    if args.pipeline > 1 and model.setPipeline(args.pipeline):
+        schedule = f"pipelined, {args.pipeline} parts"
     model.solveStep(args.warmup)
-    model.synchronize()
-    model.enableKernelTiming(True)
     model.synchronize()
     t0 = time.perf_counter()
     model.solveStep(steps)
@@ -152,6 +153,11 @@ def run_plain(args):
     nqp = mesh.nb_elements * 8
     value = nqp * steps / elapsed
     ms_step = 1e3 * elapsed / steps
+    # per-kernel times: a separate pass with HIP events around every launch
+    # (plain schedule), so the timed loop above carries no event overhead
+    model.enableKernelTiming(True)
+    model.solveStep(min(steps, 100))
+    model.synchronize()
 
     kern = {}
     names = {0: "element_pass", 1: "gather_update", 2: "predictor"}
@@ -216,7 +222,7 @@ def run_plain(args):
                                + {"fp64": "plain FP64", "fp32": "FP32 form",
                                   "fp64_fast": "FP64 FMA form"}[args.precision],
                    "elements": mesh.nb_elements, "nodes": mesh.nb_nodes,
-                   "quadrature_points": nqp, "dt": dt},
+                   "quadrature_points": nqp, "dt": dt, "schedule": schedule},
         "roofline": roofline,
         "kernels": kern,
     }
@@ -527,6 +533,8 @@ def main():
                     default="plain")
     ap.add_argument("--precision", choices=["fp64", "fp32", "fp64_fast"], default="fp64",
                     help="plain form only")
+    ap.add_argument("--pipeline", type=int, default=0,
+                    help="plain form: element parts of the pipelined schedule (0 = off)")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
