@@ -57,4 +57,35 @@ __host__ __device__ inline QpView qp_view(const ElemArgs &A, int nqp) {
                 A.n * nqp, A.p_pe,  A.c_pe};
 }
 
+// ---------------------------------------------------------------------------
+// Where the element pass finds the coordinates X and displacement u of a
+// node: structure-of-arrays [3][stride] (single model) or one 48-byte record
+// per node {X0, X1, X2, u0, u1, u2} read with three 16-byte loads (batches).
+// ---------------------------------------------------------------------------
+struct NodesSoA {
+  const double *X, *u;
+  int64_t stride;
+  __device__ __forceinline__ void load(int32_t n, double (&x)[3], double (&d)[3]) const {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x[c] = X[(int64_t)c * stride + n];
+      d[c] = u[(int64_t)c * stride + n];
+    }
+  }
+};
+
+struct NodesXU {
+  const double *xu; // [n][6]
+  __device__ __forceinline__ void load(int32_t n, double (&x)[3], double (&d)[3]) const {
+    const double2 *p = reinterpret_cast<const double2 *>(xu + 6 * (int64_t)n);
+    const double2 a = p[0], b = p[1], c = p[2];
+    x[0] = a.x;
+    x[1] = a.y;
+    x[2] = b.x;
+    d[0] = b.y;
+    d[1] = c.x;
+    d[2] = c.y;
+  }
+};
+
 } // namespace akx

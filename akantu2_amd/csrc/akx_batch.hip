@@ -5,8 +5,10 @@
 // retired between chunks of steps while the others stay resident).
 //
 // Memory: arenas sized at creation.
-//   nodes    SoA [3][node_cap] X, u, v, a, f_int; M [node_cap]; blocked bits;
-//            an instance owns a node range aligned to 256 (whole ELL slices)
+//   nodes    one 48-byte record {X, u} per node (the element pass reads a node
+//            with three 16-byte loads); SoA [3][node_cap] v, a, f_int;
+//            M [node_cap]; blocked bits; an instance owns a node range
+//            aligned to 256 (whole ELL slices)
 //   elements AoS connectivity and table positions per element type
 //   table    node-major sliced-ELL elemental forces (FP64 or FP32)
 //   history  plastic ping-pong: FP64 form grad u, stress, plastic strain
@@ -54,8 +56,8 @@ __global__ void __launch_bounds__(256) k_batch_tet4(BatchElemArgs A) {
     for (int j = threadIdx.x; j < T.count; j += blockDim.x) {
       const int64_t e = T.first + j;
       element_forces_one<ET_TET4, LAW, double, double, double>(
-          A.conn + e * 4, A.X, A.u, A.ncap, P.lambda, P.mu, P.sigma_y, P.h, V,
-          A.hoff + e, A.pos + e * 4, A.fe);
+          A.conn + e * 4, NodesXU{A.xu}, P.lambda, P.mu, P.sigma_y, P.h, V, A.hoff + e,
+          A.pos + e * 4, A.fe);
     }
   }
 }
@@ -80,8 +82,8 @@ __global__ void __launch_bounds__(EPB * 8) k_batch_hex8(BatchElemArgs A) {
       const bool active = base + el < T.count;
       const int64_t e = T.first + (active ? base + el : 0);
       hex8_strict_group<LAW, EPB, double>(sN, sR, sD, wq, el, q, active, A.conn + e * 8,
-                                          A.X, A.u, A.ncap, P.lambda, P.mu, P.sigma_y,
-                                          P.h, V, A.hoff + e * 8 + q, A.pos + e * 8,
+                                          NodesXU{A.xu}, P.lambda, P.mu, P.sigma_y, P.h,
+                                          V, A.hoff + e * 8 + q, A.pos + e * 8,
                                           reinterpret_cast<double *>(A.fe));
     }
   }
@@ -99,7 +101,8 @@ struct BatchGatherArgs {
   const int64_t *ell_off; // [ncap / 256] first entry of each slice
   const uint8_t *deg;
   const void *fe;
-  double *fint, *u, *v, *a; // [3][ncap]
+  double *fint, *v, *a; // [3][ncap]
+  double *xu;           // [ncap][6] node records {X, u}
   const double *mass;
   const uint8_t *blocked;
   float *cu, *cv; // compensation terms (FP32 form) or null
@@ -147,20 +150,22 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
         float cv = G.cv[o];
         kahan_add(v, cv, dd * x);
         if constexpr (PREDICT) {
-          double u = G.u[o];
+          double *up = G.xu + 6 * n + 3 + c;
+          double u = *up;
           float cu = G.cu[o];
           const double dt_a = dt * a;
           kahan_add(u, cu, dt * v + (.5 * dt) * dt_a);
           kahan_add(v, cv, dt_a);
-          G.u[o] = u;
+          *up = u;
           G.cu[o] = cu;
         }
         G.cv[o] = cv;
       } else {
         v += dd * x;
         if constexpr (PREDICT) {
+          double *up = G.xu + 6 * n + 3 + c;
           const double dt_a = dt * a;
-          G.u[o] = G.u[o] + (dt * v + (.5 * dt) * dt_a);
+          *up = *up + (dt * v + (.5 * dt) * dt_a);
           v = v + dt_a;
         }
       }
@@ -188,7 +193,8 @@ __global__ void __launch_bounds__(256) k_batch_predictor(BatchGatherArgs G) {
         continue;
       const int64_t o = (int64_t)c * G.ncap + n;
       const double a = G.a[o];
-      double v = G.v[o], u = G.u[o];
+      double *up = G.xu + 6 * n + 3 + c;
+      double v = G.v[o], u = *up;
       const double dt_a = dt * a;
       if constexpr (COMP) {
         float cu = G.cu[o], cv = G.cv[o];
@@ -200,7 +206,7 @@ __global__ void __launch_bounds__(256) k_batch_predictor(BatchGatherArgs G) {
         u = u + (dt * v + (.5 * dt) * dt_a);
         v = v + dt_a;
       }
-      G.u[o] = u;
+      *up = u;
       G.v[o] = v;
     }
   }
@@ -215,7 +221,7 @@ __global__ void __launch_bounds__(256) k_batch_predictor(BatchGatherArgs G) {
 struct BatchInitArgs {
   int64_t first, n;
   const int32_t *conn, *pos;
-  const double *X;
+  const double *xu; // [ncap][6] node records
   int64_t ncap;
   double rho, lambda, mu;
   const double *hex_tab;
@@ -233,7 +239,7 @@ __device__ __forceinline__ void batch_load_X(const BatchInitArgs &A, int64_t e,
     const int32_t node = A.conn[e * Elem<ET>::nnpe + n];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      X[n][c] = A.X[(int64_t)c * A.ncap + node];
+      X[n][c] = A.xu[6 * (int64_t)node + c];
   }
 }
 
@@ -329,23 +335,25 @@ __global__ void k_batch_gather_mass(int64_t node0, int64_t n, const int64_t *ell
   mass[nd] = m;
 }
 
-// host [n][3] <-> arena [3][ncap] at node0
-__global__ void k_batch_scatter_soa(const double *aos, double *soa, int64_t ncap,
-                                    int64_t node0, int64_t n) {
+// host [n][3] <-> arena [3][ncap] (stride ncap, offset 0) or the node
+// records [ncap][6] (stride 1 per component, 6 per node, offset 0 for X and
+// 3 for u), at node0
+__global__ void k_batch_put3(const double *aos, double *dst, int64_t comp_stride,
+                             int64_t node_stride, int64_t node0, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 3 * n)
     return;
   const int64_t r = i / 3, c = i % 3;
-  soa[c * ncap + node0 + r] = aos[i];
+  dst[c * comp_stride + (node0 + r) * node_stride] = aos[i];
 }
 
-__global__ void k_batch_gather_aos(const double *soa, double *aos, int64_t ncap,
-                                   int64_t node0, int64_t n) {
+__global__ void k_batch_get3(const double *src, double *aos, int64_t comp_stride,
+                             int64_t node_stride, int64_t node0, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 3 * n)
     return;
   const int64_t r = i / 3, c = i % 3;
-  aos[i] = soa[c * ncap + node0 + r];
+  aos[i] = src[c * comp_stride + (node0 + r) * node_stride];
 }
 
 template <class T> __global__ void k_batch_fill(T *p, int64_t n, T v) {
@@ -435,7 +443,8 @@ struct akx_batch {
   bool plastic = false;
   int64_t ncap = 0, tcap = 0, hexcap = 0, ellcap = 0;
   // arenas
-  DevBuf<double> X, u, v, a, fint, mass;
+  DevBuf<double> xu; // [ncap][6] node records {X, u}
+  DevBuf<double> v, a, fint, mass;
   DevBuf<uint8_t> blocked, deg;
   DevBuf<float> cu, cv;
   DevBuf<int32_t> conn_t, pos_t, conn_h, pos_h;
@@ -562,8 +571,7 @@ BatchElemArgs elem_args(akx_batch *b, int etype, int64_t t0, int64_t t1, int par
   A.inst = b->d_inst.p;
   A.conn = etype == AKX_TET4 ? b->conn_t.p : b->conn_h.p;
   A.pos = etype == AKX_TET4 ? b->pos_t.p : b->pos_h.p;
-  A.X = b->X.p;
-  A.u = b->u.p;
+  A.xu = b->xu.p;
   A.ncap = b->ncap;
   A.fe = b->fe.p;
   for (int s = 0; s < 2; ++s)
@@ -597,7 +605,7 @@ void element_pass(akx_batch *b, int parity) {
       continue;
     const int etype = g / 4, law = g % 4;
     BatchElemArgs A = elem_args(b, etype, t0, t1, parity);
-    require(A.conn && A.pos && A.X && A.u && A.fe && A.inst && A.tiles && A.hex_tab,
+    require(A.conn && A.pos && A.xu && A.fe && A.inst && A.tiles && A.hex_tab,
             "batch element arena not allocated");
     if (law == AKX_LAW_PLASTIC_LIH)
       for (int s = 0; s < 2; ++s)
@@ -626,7 +634,7 @@ BatchGatherArgs gather_args(akx_batch *b) {
   G.deg = b->deg.p;
   G.fe = b->fe.p;
   G.fint = b->fint.p;
-  G.u = b->u.p;
+  G.xu = b->xu.p;
   G.v = b->v.p;
   G.a = b->a.p;
   G.mass = b->mass.p;
@@ -635,7 +643,7 @@ BatchGatherArgs gather_args(akx_batch *b) {
   G.cv = b->cv.p;
   G.ncap = b->ncap;
   G.nblocks = persistent_grid(G.nslices, (int64_t)b->cus * 8);
-  require(G.slices && G.ell_off && G.deg && G.fe && G.fint && G.u && G.v && G.a &&
+  require(G.slices && G.ell_off && G.deg && G.fe && G.fint && G.xu && G.v && G.a &&
               G.mass && G.blocked,
           "batch node arena not allocated");
   require(!b->fp32() || (G.cu && G.cv), "compensation terms not allocated");
@@ -821,7 +829,8 @@ int akx_batch_create(int device, int precision, int64_t node_capacity,
     require(b->ncap < INT32_MAX && b->ellcap < INT32_MAX && 8 * b->hexcap < INT32_MAX &&
                 4 * b->tcap < INT32_MAX,
             "capacities exceed the int32 indices of the arenas");
-    for (auto *f : {&b->X, &b->u, &b->v, &b->a, &b->fint})
+    b->xu.alloc(6 * b->ncap);
+    for (auto *f : {&b->v, &b->a, &b->fint})
       f->alloc(3 * b->ncap);
     b->mass.alloc(b->ncap);
     b->blocked.alloc(b->ncap);
@@ -986,15 +995,19 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
       HIPCHECK(hipMemcpy(b->deg.p + I.node0, deg8.data(), I.nn_pad, hipMemcpyHostToDevice));
       HIPCHECK(hipMemcpy(b->blocked.p + I.node0, bits.data(), I.nn_pad,
                          hipMemcpyHostToDevice));
-      // nodal state: X and v from the host, u = a = f_int = 0
+      // nodal state: X and v from the host, u = a = f_int = 0 (the records
+      // of the padding nodes are zeroed too)
+      hipLaunchKernelGGL((k_batch_fill<double>), dim3(gridn(6 * I.nn_pad)), dim3(TPB), 0,
+                         b->stream, b->xu.p + 6 * I.node0, 6 * I.nn_pad, 0.);
       b->staging.reserve(3 * nb_nodes);
+      HIPCHECK(hipStreamSynchronize(b->stream));
       HIPCHECK(hipMemcpy(b->staging.p, nodes, 3 * nb_nodes * sizeof(double),
                          hipMemcpyHostToDevice));
-      hipLaunchKernelGGL(k_batch_scatter_soa, dim3(gridn(3 * nb_nodes)), dim3(TPB), 0,
-                         b->stream, (const double *)b->staging.p, b->X.p, b->ncap, I.node0,
-                         nb_nodes);
+      hipLaunchKernelGGL(k_batch_put3, dim3(gridn(3 * nb_nodes)), dim3(TPB), 0, b->stream,
+                         (const double *)b->staging.p, b->xu.p, (int64_t)1, (int64_t)6,
+                         I.node0, nb_nodes);
       HIPCHECK(hipGetLastError());
-      for (auto *f : {&b->u, &b->v, &b->a, &b->fint})
+      for (auto *f : {&b->v, &b->a, &b->fint})
         for (int c = 0; c < 3; ++c)
           hipLaunchKernelGGL((k_batch_fill<double>), dim3(gridn(I.nn_pad)), dim3(TPB), 0,
                              b->stream, f->p + c * b->ncap + I.node0, I.nn_pad, 0.);
@@ -1008,9 +1021,9 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
       if (velocity) {
         HIPCHECK(hipMemcpy(b->staging.p, velocity, 3 * nb_nodes * sizeof(double),
                            hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_batch_scatter_soa, dim3(gridn(3 * nb_nodes)), dim3(TPB), 0,
+        hipLaunchKernelGGL(k_batch_put3, dim3(gridn(3 * nb_nodes)), dim3(TPB), 0,
                            b->stream, (const double *)b->staging.p, b->v.p, b->ncap,
-                           I.node0, nb_nodes);
+                           (int64_t)1, I.node0, nb_nodes);
         HIPCHECK(hipGetLastError());
       }
       if (law == AKX_LAW_PLASTIC_LIH) {
@@ -1038,7 +1051,7 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
       A.n = I.nel;
       A.conn = dconn;
       A.pos = dpos;
-      A.X = b->X.p;
+      A.xu = b->xu.p;
       A.ncap = b->ncap;
       A.rho = I.rho;
       A.lambda = I.lambda;
@@ -1165,18 +1178,27 @@ int akx_batch_get_field(akx_batch *b, int id, int field, double *host) {
     require(host != nullptr, "null output");
     HIPCHECK(hipSetDevice(b->device));
     const double *src = nullptr;
+    int64_t comp_stride = b->ncap, node_stride = 1;
     switch (field) {
-    case AKX_FIELD_DISPLACEMENT: src = b->u.p; break;
+    case AKX_FIELD_DISPLACEMENT:
+      src = b->xu.p + 3;
+      comp_stride = 1;
+      node_stride = 6;
+      break;
     case AKX_FIELD_VELOCITY: src = b->v.p; break;
     case AKX_FIELD_ACCELERATION: src = b->a.p; break;
     case AKX_FIELD_INTERNAL_FORCE: src = b->fint.p; break;
-    case AKX_FIELD_NODES: src = b->X.p; break;
+    case AKX_FIELD_NODES:
+      src = b->xu.p;
+      comp_stride = 1;
+      node_stride = 6;
+      break;
     default: throw Error(AKX_ERR_INVALID, "field not available for batch instances");
     }
     sync(b);
     b->staging.reserve(3 * I.nn);
-    hipLaunchKernelGGL(k_batch_gather_aos, dim3(gridn(3 * I.nn)), dim3(TPB), 0, b->stream,
-                       src, b->staging.p, b->ncap, I.node0, I.nn);
+    hipLaunchKernelGGL(k_batch_get3, dim3(gridn(3 * I.nn)), dim3(TPB), 0, b->stream, src,
+                       b->staging.p, comp_stride, node_stride, I.node0, I.nn);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(host, b->staging.p, 3 * I.nn * sizeof(double),
                             hipMemcpyDeviceToHost, b->stream));

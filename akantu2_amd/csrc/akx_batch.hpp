@@ -43,7 +43,7 @@ struct BatchElemArgs {
   const BInst *inst;
   const int32_t *conn; // [elem][nnpe] arena node ids
   const int32_t *pos;  // [elem][nnpe] table entries
-  const double *X, *u; // [3][ncap]
+  const double *xu;    // [ncap][6] node records {X, u}
   int64_t ncap;
   void *fe;            // [entry][3] elemental forces (FP64 or FP32)
   // plastic history, two buffers (ping-pong by step parity).  FP64 form:

@@ -31,6 +31,8 @@ namespace akxf {
 
 using akx::BatchElemArgs;
 using akx::BInst;
+using akx::NodesSoA;
+using akx::NodesXU;
 using akx::BTile;
 using akx::ElemArgs;
 using akx::QpCompact;
@@ -193,20 +195,20 @@ __device__ __forceinline__ void element_params(const ElemArgs &A, int64_t i,
 // ---------------------------------------------------------------------------
 constexpr int HEX_EPB = 32; // elements per 256-thread block
 
-template <int LAW, class CT, class FT, class VT, int EPB>
+template <int LAW, class CT, class FT, class VT, int EPB, class NA>
 __device__ __forceinline__ void hex8_group(double (&sN)[EPB][8][6], CT (&sQ)[EPB][8][9],
                                            const CT (&sD)[8][3][8], int el, int q,
                                            bool active, const int32_t *conn_e,
-                                           const double *Xs, const double *us,
-                                           int64_t stride, CT lambda, CT mu, CT sy,
+                                           const NA &nodes_of, CT lambda, CT mu, CT sy,
                                            CT hh, const VT &V, int64_t iq,
                                            const int32_t *pos_e, FT *fe) {
   if (active) {
-    const int32_t node = conn_e[q];
+    double x[3], d[3];
+    nodes_of.load(conn_e[q], x, d);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      sN[el][q][c] = Xs[(int64_t)c * stride + node];
-      sN[el][q][3 + c] = us[(int64_t)c * stride + node];
+      sN[el][q][c] = x[c];
+      sN[el][q][3 + c] = d[c];
     }
   }
   __syncthreads();
@@ -311,7 +313,8 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
     element_params(A, i, lambda, mu, sy, hh);
   }
   hex8_group<LAW, CT, FT, QpViewT<double>, HEX_EPB>(
-      sN, sQ, sD, el, q, active, A.conn + e * 8, A.X, A.u, A.nn, lambda, mu, sy, hh,
+      sN, sQ, sD, el, q, active, A.conn + e * 8, NodesSoA{A.X, A.u, A.nn}, lambda, mu, sy,
+      hh,
       qp_view(A, 8), i * 8 + q, A.pos + (A.pos_offset + (active ? i : 0)) * 8,
       reinterpret_cast<FT *>(A.fe));
 }
@@ -319,21 +322,16 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
 // ---------------------------------------------------------------------------
 // Tet4: one thread per element
 // ---------------------------------------------------------------------------
-template <int LAW, class CT, class FT, class VT>
-__device__ __forceinline__ void tet4_one(const int32_t *conn_e, const double *Xs,
-                                         const double *us, int64_t stride, CT lambda,
-                                         CT mu, CT sy, CT hh, const VT &V,
+template <int LAW, class CT, class FT, class VT, class NA>
+__device__ __forceinline__ void tet4_one(const int32_t *conn_e, const NA &nodes_of,
+                                         CT lambda, CT mu, CT sy, CT hh, const VT &V,
                                          int64_t iq, const int32_t *pos_e, FT *fe) {
   const int4 nd = *reinterpret_cast<const int4 *>(conn_e);
   const int32_t nodes[4] = {nd.x, nd.y, nd.z, nd.w};
   double X[4][3], U[4][3];
 #pragma unroll
   for (int n = 0; n < 4; ++n)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      X[n][c] = Xs[(int64_t)c * stride + nodes[n]];
-      U[n][c] = us[(int64_t)c * stride + nodes[n]];
-    }
+    nodes_of.load(nodes[n], X[n], U[n]);
   // natural directions: xi -> node 3, eta -> node 1, zeta -> node 2
   const int nm[3] = {3, 1, 2};
   CT J[3][3], H[3][3];
@@ -389,7 +387,8 @@ __global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
   const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
   CT lambda, mu, sy, hh;
   element_params(A, i, lambda, mu, sy, hh);
-  tet4_one<LAW, CT, FT, QpViewT<double>>(A.conn + e * 4, A.X, A.u, A.nn, lambda, mu, sy, hh,
+  tet4_one<LAW, CT, FT, QpViewT<double>>(A.conn + e * 4, NodesSoA{A.X, A.u, A.nn}, lambda,
+                                         mu, sy, hh,
                                 qp_view(A, 1), i, A.pos + (A.pos_offset + i) * 4,
                                 reinterpret_cast<FT *>(A.fe));
 }
@@ -409,7 +408,7 @@ __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
     const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
     for (int j = threadIdx.x; j < T.count; j += blockDim.x) {
       const int64_t e = T.first + j;
-      tet4_one<LAW, float, float, QpCompact>(A.conn + e * 4, A.X, A.u, A.ncap,
+      tet4_one<LAW, float, float, QpCompact>(A.conn + e * 4, NodesXU{A.xu},
                                          float(P.lambda), float(P.mu), float(P.sigma_y),
                                          float(P.h), V, A.hoff + e, A.pos + e * 4,
                                          reinterpret_cast<float *>(A.fe));
@@ -436,7 +435,7 @@ __global__ void __launch_bounds__(EPB * 8) k_batch_hex8_f32(BatchElemArgs A) {
       const bool active = base + el < T.count;
       const int64_t e = T.first + (active ? base + el : 0);
       hex8_group<LAW, float, float, QpCompact, EPB>(
-          sN, sQ, sD, el, q, active, A.conn + e * 8, A.X, A.u, A.ncap, float(P.lambda),
+          sN, sQ, sD, el, q, active, A.conn + e * 8, NodesXU{A.xu}, float(P.lambda),
           float(P.mu), float(P.sigma_y), float(P.h), V, A.hoff + e * 8 + q,
           A.pos + e * 8, reinterpret_cast<float *>(A.fe));
     }

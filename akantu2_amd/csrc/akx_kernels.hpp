@@ -370,9 +370,9 @@ __device__ __forceinline__ void strict_law(const QpViewT<HT> &V, int64_t iq, CT 
 // over the Gauss points in ascending order.  Shared by the single-model
 // launch (one material) and the batched persistent launch.
 // ---------------------------------------------------------------------------
-template <int ET, int LAW, class CT, class FT, class HT>
+template <int ET, int LAW, class CT, class FT, class HT, class NA>
 __device__ __forceinline__ void element_forces_one(
-    const int32_t *conn_e, const double *Xs, const double *us, int64_t stride,
+    const int32_t *conn_e, const NA &nodes_of,
     CT lambda, CT mu, CT sy, CT hh, const QpViewT<HT> &V, int64_t iq0,
     const int32_t *pos_e, void *fe_out) {
   constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
@@ -391,12 +391,15 @@ __device__ __forceinline__ void element_forces_one(
 
   CT X[NN][3], U[NN][3];
 #pragma unroll
-  for (int n = 0; n < NN; ++n)
+  for (int n = 0; n < NN; ++n) {
+    double x[3], d[3];
+    nodes_of.load(nodes[n], x, d);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      X[n][c] = Xs[(int64_t)c * stride + nodes[n]];
-      U[n][c] = us[(int64_t)c * stride + nodes[n]];
+      X[n][c] = CT(x[c]);
+      U[n][c] = CT(d[c]);
     }
+  }
 
   CT re[NN][3];
 
@@ -466,8 +469,8 @@ k_element_forces(ElemArgs A) {
     sy = CT(A.sy_el[i]);
   if (A.h_el)
     hh = CT(A.h_el[i]);
-  element_forces_one<ET, LAW, CT, FT, double>(A.conn + e * NN, A.X, A.u, A.nn, lambda, mu, sy,
-                                      hh, qp_view(A, NQ), i * NQ,
+  element_forces_one<ET, LAW, CT, FT, double>(A.conn + e * NN, NodesSoA{A.X, A.u, A.nn},
+                                              lambda, mu, sy, hh, qp_view(A, NQ), i * NQ,
                                       A.pos + (A.pos_offset + i) * NN, A.fe);
 }
 
@@ -487,18 +490,19 @@ k_element_forces(ElemArgs A) {
 // ---------------------------------------------------------------------------
 constexpr int HEX_STRICT_EPB = 8;
 
-template <int LAW, int EPB, class HT>
+template <int LAW, int EPB, class HT, class NA>
 __device__ __forceinline__ void hex8_strict_group(
     double (&sN)[EPB][8][6], double (&sR)[EPB][8][13], const double (&sD)[8][3][8],
-    double wq, int el, int q, bool active, const int32_t *conn_e, const double *Xs,
-    const double *us, int64_t stride, double lambda, double mu, double sy, double hh,
+    double wq, int el, int q, bool active, const int32_t *conn_e, const NA &nodes_of,
+    double lambda, double mu, double sy, double hh,
     const QpViewT<HT> &V, int64_t iq, const int32_t *pos_e, double *fe) {
   if (active) {
-    const int32_t node = conn_e[q];
+    double x[3], d[3];
+    nodes_of.load(conn_e[q], x, d);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      sN[el][q][c] = Xs[(int64_t)c * stride + node];
-      sN[el][q][3 + c] = us[(int64_t)c * stride + node];
+      sN[el][q][c] = x[c];
+      sN[el][q][3 + c] = d[c];
     }
   }
   __syncthreads();
@@ -604,8 +608,8 @@ __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
       hh = A.h_el[i];
   }
   hex8_strict_group<LAW, HEX_STRICT_EPB, double>(
-      sN, sR, sD, c_hex8.w[q], el, q, active, A.conn + e * 8, A.X, A.u, A.nn, lambda,
-      mu, sy, hh, qp_view(A, 8), i * 8 + q, A.pos + (A.pos_offset + (active ? i : 0)) * 8,
+      sN, sR, sD, c_hex8.w[q], el, q, active, A.conn + e * 8, NodesSoA{A.X, A.u, A.nn},
+      lambda, mu, sy, hh, qp_view(A, 8), i * 8 + q, A.pos + (A.pos_offset + (active ? i : 0)) * 8,
       reinterpret_cast<double *>(A.fe));
 }
 
