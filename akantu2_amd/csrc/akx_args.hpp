@@ -74,6 +74,24 @@ struct NodesSoA {
   }
 };
 
+// Write one elemental-force entry: W = 3 values, or W = 4 with a zero pad so
+// that the entry is one aligned vector store (16 B in FP32, 2 x 16 B in FP64).
+template <int W, class FT>
+__device__ __forceinline__ void put_entry(FT *fe, int64_t entry, FT a, FT b, FT c) {
+  if constexpr (W == 3) {
+    FT *o = fe + entry * 3;
+    o[0] = a;
+    o[1] = b;
+    o[2] = c;
+  } else if constexpr (sizeof(FT) == 4) {
+    *reinterpret_cast<float4 *>(fe + entry * 4) = make_float4(a, b, c, 0.f);
+  } else {
+    double2 *o = reinterpret_cast<double2 *>(fe + entry * 4);
+    o[0] = make_double2(a, b);
+    o[1] = make_double2(c, 0.);
+  }
+}
+
 struct NodesXU {
   const double *xu; // [n][6]
   __device__ __forceinline__ void load(int32_t n, double (&x)[3], double (&d)[3]) const {

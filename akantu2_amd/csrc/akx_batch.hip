@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256) k_batch_tet4(BatchElemArgs A) {
     const QpView V = batch_view<double>(A, (P.parity_base ^ A.parity) & 1);
     for (int j = threadIdx.x; j < T.count; j += blockDim.x) {
       const int64_t e = T.first + j;
-      element_forces_one<ET_TET4, LAW, double, double, double>(
+      element_forces_one<ET_TET4, LAW, double, double, double, NodesXU, FE_W>(
           A.conn + e * 4, NodesXU{A.xu}, P.lambda, P.mu, P.sigma_y, P.h, V, A.hoff + e,
           A.pos + e * 4, A.fe);
     }
@@ -81,7 +81,8 @@ __global__ void __launch_bounds__(EPB * 8) k_batch_hex8(BatchElemArgs A) {
     for (int base = 0; base < T.count; base += EPB) {
       const bool active = base + el < T.count;
       const int64_t e = T.first + (active ? base + el : 0);
-      hex8_strict_group<LAW, EPB, double>(sN, sR, sD, wq, el, q, active, A.conn + e * 8,
+      hex8_strict_group<LAW, EPB, double, NodesXU, FE_W>(sN, sR, sD, wq, el, q, active,
+                                                         A.conn + e * 8,
                                           NodesXU{A.xu}, P.lambda, P.mu, P.sigma_y, P.h,
                                           V, A.hoff + e * 8 + q, A.pos + e * 8,
                                           reinterpret_cast<double *>(A.fe));
@@ -120,13 +121,30 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
       continue;
     const double dt = G.inst[SL.inst].dt;
     const int64_t n = SL.node0 + threadIdx.x;
-    const FT *p = reinterpret_cast<const FT *>(G.fe) + (G.ell_off[n >> 8] + (n & 255)) * 3;
+    const FT *p = reinterpret_cast<const FT *>(G.fe) + (G.ell_off[n >> 8] + (n & 255)) * FE_W;
     double f[3] = {0., 0., 0.};
     const int deg = G.deg[n];
-    for (int d = 0; d < deg; ++d, p += 256 * 3) {
-      f[0] = f[0] + -1. * double(p[0]);
-      f[1] = f[1] + -1. * double(p[1]);
-      f[2] = f[2] + -1. * double(p[2]);
+    for (int d = 0; d < deg; ++d, p += 256 * FE_W) {
+      double e0, e1, e2;
+      if constexpr (FE_W == 3) {
+        e0 = p[0];
+        e1 = p[1];
+        e2 = p[2];
+      } else if constexpr (sizeof(FT) == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
+        e0 = v.x;
+        e1 = v.y;
+        e2 = v.z;
+      } else {
+        const double2 v0 = reinterpret_cast<const double2 *>(p)[0];
+        const double2 v1 = reinterpret_cast<const double2 *>(p)[1];
+        e0 = v0.x;
+        e1 = v0.y;
+        e2 = v1.x;
+      }
+      f[0] = f[0] + -1. * e0;
+      f[1] = f[1] + -1. * e1;
+      f[2] = f[2] + -1. * e2;
     }
     const double M = G.mass[n];
     const uint8_t blk = G.blocked[n];
@@ -761,7 +779,7 @@ void do_steps(akx_batch *b, int64_t n) {
 }
 
 // device bytes an instance takes in a batch (node range padded to 256,
-// force-table entries estimated with a 25 % slice padding)
+// force-table entries of FE_W values estimated with a 25 % slice padding)
 int64_t footprint_bytes(int precision, int etype, int law, int64_t nn, int64_t nel) {
   const int nnpe = etype == AKX_TET4 ? 4 : 8, nqp = etype == AKX_TET4 ? 1 : 8;
   const bool f32 = precision == AKX_PRECISION_FP32;
@@ -769,7 +787,7 @@ int64_t footprint_bytes(int precision, int etype, int law, int64_t nn, int64_t n
   const int64_t nn_pad = (nn + 255) / 256 * 256;
   double bytes = (double)nn_pad * (5 * 3 * 8 + 8 + 1 + 1 + (f32 ? 2 * 3 * 4 : 0)) +
                  (double)nn_pad / 256 * 8;
-  bytes += (double)nel * nnpe * (4 + 4 + 1.25 * 3 * sf);
+  bytes += (double)nel * nnpe * (4 + 4 + 1.25 * FE_W * sf);
   if (law == AKX_LAW_PLASTIC_LIH) // two buffers of the history
     bytes += (double)nel * nqp * 2 * (f32 ? 7 * 4 : 28 * 8);
   return (int64_t)bytes;
@@ -848,7 +866,7 @@ int akx_batch_create(int device, int precision, int64_t node_capacity,
       b->conn_h.alloc(8 * b->hexcap);
       b->pos_h.alloc(8 * b->hexcap);
     }
-    b->fe.alloc(b->ellcap * 3 * b->fe_size());
+    b->fe.alloc(b->ellcap * FE_W * b->fe_size());
     if (b->plastic)
       for (int s = 0; s < 2; ++s)
         for (int k = 0; k < 4; ++k)
@@ -1057,9 +1075,9 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
       A.lambda = I.lambda;
       A.mu = I.mu;
       A.hex_tab = b->hex_tab.p;
-      // 8 bytes per entry fit in the 3 * 4 or 3 * 8 bytes of the entry's
-      // force; ell0 is a multiple of 256, so the address is 8-byte aligned
-      A.me = reinterpret_cast<double *>(b->fe.p + I.ell0 * 3 * b->fe_size());
+      // 8 bytes per entry fit in the FE_W * 4 or FE_W * 8 bytes of the
+      // entry's force; ell0 is a multiple of 256, so the address is aligned
+      A.me = reinterpret_cast<double *>(b->fe.p + I.ell0 * FE_W * b->fe_size());
       A.ell0 = I.ell0;
       const long long big = INT64_MAX;
       const unsigned long long dmax = 0x7FEFFFFFFFFFFFFFull;

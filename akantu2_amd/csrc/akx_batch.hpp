@@ -45,7 +45,7 @@ struct BatchElemArgs {
   const int32_t *pos;  // [elem][nnpe] table entries
   const double *xu;    // [ncap][6] node records {X, u}
   int64_t ncap;
-  void *fe;            // [entry][3] elemental forces (FP64 or FP32)
+  void *fe;            // [entry][FE_W] elemental forces (FP64 or FP32)
   // plastic history, two buffers (ping-pong by step parity).  FP64 form:
   // grad u, stress, plastic strain ([9][hcap]) and hardening ([hcap]);
   // FP32 form: plastic strain in Voigt order ([6][hcap], in the third
@@ -59,6 +59,11 @@ struct BatchElemArgs {
 };
 
 constexpr int HEX_TAB_N = 192, HEX_TAB_W = 256, HEX_TAB_SIZE = 264;
+
+// values per elemental-force entry in a batch's table.  3: a zero-padded
+// 4-wide entry (one vector store per corner) measured slower, its extra
+// bytes cost more than the saved cache accesses.
+constexpr int FE_W = 3;
 
 // [w0, w1): the part of a work list of nwork items owned by this block
 __device__ __forceinline__ void batch_chunk(int64_t nwork, int64_t nblocks, int64_t &w0,

@@ -39,6 +39,7 @@ using akx::QpCompact;
 using akx::QpViewT;
 using akx::batch_chunk;
 using akx::batch_compact_view;
+using akx::FE_W;
 using akx::qp_view;
 
 constexpr int LAW_ELASTIC = 0, LAW_ELASTIC_FD = 1, LAW_NEOHOOKEAN = 2,
@@ -195,20 +196,31 @@ __device__ __forceinline__ void element_params(const ElemArgs &A, int64_t i,
 // ---------------------------------------------------------------------------
 constexpr int HEX_EPB = 32; // elements per 256-thread block
 
-template <int LAW, class CT, class FT, class VT, int EPB, class NA>
-__device__ __forceinline__ void hex8_group(double (&sN)[EPB][8][6], CT (&sQ)[EPB][8][9],
+template <int LAW, class CT, class FT, class VT, int EPB, class NA, int W = 3>
+__device__ __forceinline__ void hex8_group(CT (&sN)[EPB][8][6], CT (&sQ)[EPB][8][9],
                                            const CT (&sD)[8][3][8], int el, int q,
                                            bool active, const int32_t *conn_e,
                                            const NA &nodes_of, CT lambda, CT mu, CT sy,
                                            CT hh, const VT &V, int64_t iq,
                                            const int32_t *pos_e, FT *fe) {
-  if (active) {
-    double x[3], d[3];
+  // lane q stages node q: absolute in FP64; in FP32 relative to node 0 of
+  // the element (exact FP64 difference, rounded once; lanes 8 el .. 8 el + 7
+  // are one element, so node 0 comes from the first lane of the 8)
+  double x[3] = {0., 0., 0.}, d[3] = {0., 0., 0.};
+  if (active)
     nodes_of.load(conn_e[q], x, d);
+  if constexpr (sizeof(CT) == 4) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      sN[el][q][c] = x[c];
-      sN[el][q][3 + c] = d[c];
+      x[c] -= __shfl(x[c], 0, 8);
+      d[c] -= __shfl(d[c], 0, 8);
+    }
+  }
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      sN[el][q][c] = CT(x[c]);
+      sN[el][q][3 + c] = CT(d[c]);
     }
   }
   __syncthreads();
@@ -221,24 +233,13 @@ __device__ __forceinline__ void hex8_group(double (&sN)[EPB][8][6], CT (&sQ)[EPB
         J[a][b] = CT(0);
         H[a][b] = CT(0);
       }
-    double x0[3], u0[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      x0[c] = sN[el][0][c];
-      u0[c] = sN[el][0][3 + c];
-    }
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
       CT xn[3], un[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        if constexpr (sizeof(CT) == 8) {
-          xn[c] = sN[el][n][c];
-          un[c] = sN[el][n][3 + c];
-        } else { // relative to node 0, exact FP64 difference rounded once
-          xn[c] = CT(sN[el][n][c] - x0[c]);
-          un[c] = CT(sN[el][n][3 + c] - u0[c]);
-        }
+        xn[c] = sN[el][n][c];
+        un[c] = sN[el][n][3 + c];
       }
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
@@ -289,16 +290,13 @@ __device__ __forceinline__ void hex8_group(double (&sN)[EPB][8][6], CT (&sQ)[EPB
         for (int k = 0; k < 3; ++k)
           r[k] += sQ[el][qq][3 * k + m] * d;
       }
-    FT *out = fe + (int64_t)pos_e[n] * 3;
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      out[k] = FT(r[k]);
+    akx::put_entry<W>(fe, (int64_t)pos_e[n], FT(r[0]), FT(r[1]), FT(r[2]));
   }
 }
 
 template <int LAW, class CT, class FT>
 __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
-  __shared__ double sN[HEX_EPB][8][6]; // X (0..2) and u (3..5) of the 8 nodes
+  __shared__ CT sN[HEX_EPB][8][6];     // X (0..2) and u (3..5) of the 8 nodes
   __shared__ CT sQ[HEX_EPB][8][9];     // Q of the 8 Gauss points
   __shared__ CT sD[8][3][8];           // dN_n/dxi_m at Gauss point q
   const int t = threadIdx.x, el = t >> 3, q = t & 7;
@@ -322,7 +320,7 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
 // ---------------------------------------------------------------------------
 // Tet4: one thread per element
 // ---------------------------------------------------------------------------
-template <int LAW, class CT, class FT, class VT, class NA>
+template <int LAW, class CT, class FT, class VT, class NA, int W = 3>
 __device__ __forceinline__ void tet4_one(const int32_t *conn_e, const NA &nodes_of,
                                          CT lambda, CT mu, CT sy, CT hh, const VT &V,
                                          int64_t iq, const int32_t *pos_e, FT *fe) {
@@ -370,13 +368,11 @@ __device__ __forceinline__ void tet4_one(const int32_t *conn_e, const NA &nodes_
       Q[k][m] = wJ * s;
     }
   const int4 pv = *reinterpret_cast<const int4 *>(pos_e);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    fe[(int64_t)pv.x * 3 + k] = FT(-(Q[k][0] + Q[k][1] + Q[k][2]));
-    fe[(int64_t)pv.y * 3 + k] = FT(Q[k][1]);
-    fe[(int64_t)pv.z * 3 + k] = FT(Q[k][2]);
-    fe[(int64_t)pv.w * 3 + k] = FT(Q[k][0]);
-  }
+  akx::put_entry<W>(fe, (int64_t)pv.x, FT(-(Q[0][0] + Q[0][1] + Q[0][2])),
+                    FT(-(Q[1][0] + Q[1][1] + Q[1][2])), FT(-(Q[2][0] + Q[2][1] + Q[2][2])));
+  akx::put_entry<W>(fe, (int64_t)pv.y, FT(Q[0][1]), FT(Q[1][1]), FT(Q[2][1]));
+  akx::put_entry<W>(fe, (int64_t)pv.z, FT(Q[0][2]), FT(Q[1][2]), FT(Q[2][2]));
+  akx::put_entry<W>(fe, (int64_t)pv.w, FT(Q[0][0]), FT(Q[1][0]), FT(Q[2][0]));
 }
 
 template <int LAW, class CT, class FT>
@@ -408,7 +404,7 @@ __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
     const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
     for (int j = threadIdx.x; j < T.count; j += blockDim.x) {
       const int64_t e = T.first + j;
-      tet4_one<LAW, float, float, QpCompact>(A.conn + e * 4, NodesXU{A.xu},
+      tet4_one<LAW, float, float, QpCompact, NodesXU, FE_W>(A.conn + e * 4, NodesXU{A.xu},
                                          float(P.lambda), float(P.mu), float(P.sigma_y),
                                          float(P.h), V, A.hoff + e, A.pos + e * 4,
                                          reinterpret_cast<float *>(A.fe));
@@ -418,7 +414,7 @@ __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
 
 template <int LAW, int EPB>
 __global__ void __launch_bounds__(EPB * 8) k_batch_hex8_f32(BatchElemArgs A) {
-  __shared__ double sN[EPB][8][6];
+  __shared__ float sN[EPB][8][6];
   __shared__ float sQ[EPB][8][9];
   __shared__ float sD[8][3][8];
   const int t = threadIdx.x, el = t >> 3, q = t & 7;
@@ -434,7 +430,7 @@ __global__ void __launch_bounds__(EPB * 8) k_batch_hex8_f32(BatchElemArgs A) {
     for (int base = 0; base < T.count; base += EPB) {
       const bool active = base + el < T.count;
       const int64_t e = T.first + (active ? base + el : 0);
-      hex8_group<LAW, float, float, QpCompact, EPB>(
+      hex8_group<LAW, float, float, QpCompact, EPB, NodesXU, FE_W>(
           sN, sQ, sD, el, q, active, A.conn + e * 8, NodesXU{A.xu}, float(P.lambda),
           float(P.mu), float(P.sigma_y), float(P.h), V, A.hoff + e * 8 + q,
           A.pos + e * 8, reinterpret_cast<float *>(A.fe));
@@ -451,25 +447,25 @@ hipError_t upload_hex8_dnds(const double *dnds) {
 
 int64_t elements_per_block(int etype) { return etype == 1 ? HEX_EPB : 256; }
 
+template <int L, class CT, class FT>
+static void launch_one(int etype, const ElemArgs &A, hipStream_t s) {
+  dim3 grid((unsigned)A.nblocks_padded), block(256);
+  if (etype == 1)
+    hipLaunchKernelGGL((k_hex8_fast<L, CT, FT>), grid, block, 0, s, A);
+  else
+    hipLaunchKernelGGL((k_tet4_fast<L, CT, FT>), grid, block, 0, s, A);
+}
+
 template <class CT, class FT>
 static hipError_t launch_t(int etype, int law, ElemArgs A, hipStream_t s) {
-  dim3 grid((unsigned)A.nblocks_padded), block(256);
-#define AKXF_CASE(L)                                                           \
-  case L:                                                                      \
-    if (etype == 1)                                                            \
-      hipLaunchKernelGGL((k_hex8_fast<L, CT, FT>), grid, block, 0, s, A);      \
-    else                                                                       \
-      hipLaunchKernelGGL((k_tet4_fast<L, CT, FT>), grid, block, 0, s, A);      \
-    break;
   switch (law) {
-    AKXF_CASE(LAW_ELASTIC)
-    AKXF_CASE(LAW_ELASTIC_FD)
-    AKXF_CASE(LAW_NEOHOOKEAN)
-    AKXF_CASE(LAW_PLASTIC)
+  case LAW_ELASTIC: launch_one<LAW_ELASTIC, CT, FT>(etype, A, s); break;
+  case LAW_ELASTIC_FD: launch_one<LAW_ELASTIC_FD, CT, FT>(etype, A, s); break;
+  case LAW_NEOHOOKEAN: launch_one<LAW_NEOHOOKEAN, CT, FT>(etype, A, s); break;
+  case LAW_PLASTIC: launch_one<LAW_PLASTIC, CT, FT>(etype, A, s); break;
   default:
     return hipErrorInvalidValue;
   }
-#undef AKXF_CASE
   return hipGetLastError();
 }
 

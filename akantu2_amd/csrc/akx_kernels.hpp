@@ -370,7 +370,7 @@ __device__ __forceinline__ void strict_law(const QpViewT<HT> &V, int64_t iq, CT 
 // over the Gauss points in ascending order.  Shared by the single-model
 // launch (one material) and the batched persistent launch.
 // ---------------------------------------------------------------------------
-template <int ET, int LAW, class CT, class FT, class HT, class NA>
+template <int ET, int LAW, class CT, class FT, class HT, class NA, int W = 3>
 __device__ __forceinline__ void element_forces_one(
     const int32_t *conn_e, const NA &nodes_of,
     CT lambda, CT mu, CT sy, CT hh, const QpViewT<HT> &V, int64_t iq0,
@@ -443,9 +443,8 @@ __device__ __forceinline__ void element_forces_one(
     const int32_t pn[4] = {pv.x, pv.y, pv.z, pv.w};
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        fe[(int64_t)pn[j] * 3 + k] = FT(re[4 * kk + j][k]);
+      put_entry<W>(fe, (int64_t)pn[j], FT(re[4 * kk + j][0]), FT(re[4 * kk + j][1]),
+                   FT(re[4 * kk + j][2]));
   }
 }
 
@@ -490,7 +489,7 @@ k_element_forces(ElemArgs A) {
 // ---------------------------------------------------------------------------
 constexpr int HEX_STRICT_EPB = 8;
 
-template <int LAW, int EPB, class HT, class NA>
+template <int LAW, int EPB, class HT, class NA, int W = 3>
 __device__ __forceinline__ void hex8_strict_group(
     double (&sN)[EPB][8][6], double (&sR)[EPB][8][13], const double (&sD)[8][3][8],
     double wq, int el, int q, bool active, const int32_t *conn_e, const NA &nodes_of,
@@ -570,15 +569,16 @@ __device__ __forceinline__ void hex8_strict_group(
     __syncthreads();
     if (active && q < 4) {
       const int n = 4 * hh2 + q;
-      double *out = fe + (int64_t)pos_e[n] * 3;
+      double r[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         double s = sR[el][0][3 * q + k];
 #pragma unroll
         for (int qq = 1; qq < 8; ++qq)
           s += sR[el][qq][3 * q + k];
-        out[k] = s;
+        r[k] = s;
       }
+      put_entry<W>(fe, (int64_t)pos_e[n], r[0], r[1], r[2]);
     }
     __syncthreads();
   }
