@@ -294,6 +294,151 @@ __device__ __forceinline__ void hex8_group(CT (&sN)[EPB][8][6], CT (&sQ)[EPB][8]
   }
 }
 
+// ---------------------------------------------------------------------------
+// Hex8, FP32 form, sum-factorised.  The trilinear shape functions at the
+// 2 x 2 x 2 Gauss points factor per natural axis: along an axis, a node pair
+// (v-, v+) maps to the two Gauss points by
+//   I: alpha v_own + beta v_partner          (interpolation)
+//   D: (v+ - v-) / 2                           (derivative)
+// with alpha, beta = (1 +- 1/sqrt(3)) / 2.  With the 8 lanes of an element
+// indexed by corner bits (bit a set = + side of axis a), node and Gauss point
+// of a lane share the bits, so every axis step is a butterfly with the lane
+// 1, 2 or 4 away:
+//   J, H:   row m of [J|H] = D on axis m, I on the other two      (nodes -> q)
+//   force:  F = sum_m (D^T on axis m, I on the other two) Q_m     (q -> nodes)
+// The lanes apply the raw steps  I' = v + r p  (r = beta / alpha) and
+// D' = v - p (own minus partner).  Then J' = c S J and H' = c H S with
+// c = 2 / alpha^2 and S = diag(s_m), s_m = the lane's sign on axis m, so
+// grad u = H' J'^-1 = H J^-1 unchanged, and the raw transposed chain applied
+// to Q' = (sigma / c^3) det(J') P J'^-1, sigma = s_0 s_1 s_2, gives the
+// nodal forces sum_q Q_q dN/dxi_q.  No LDS and no barriers: 14 exchanges
+// per component instead of 8 x 3 products over the nodes.
+// Bits -> node of the reference's numbering (counter-clockwise faces):
+// n(b) = b ^ ((b >> 1) & 1); the Gauss point index is the bits themselves.
+// ---------------------------------------------------------------------------
+template <int CTRL> __device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float lane_x1(float v) { return dpp_f<0xB1>(v); } // quad_perm 1,0,3,2
+__device__ __forceinline__ float lane_x2(float v) { return dpp_f<0x4E>(v); } // quad_perm 2,3,0,1
+__device__ __forceinline__ float lane_x4(float v) {                          // swizzle xor 4
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
+}
+
+constexpr float HEX_G = 0.57735026918962576f;          // 1 / sqrt(3)
+constexpr float HEX_R = 0.26794919243112270f;          // beta / alpha = (1 - G) / (1 + G)
+constexpr float HEX_C3 = 33.243067814700545f;          // c^3, c = 2 / alpha^2
+
+// r in a vector register: an I step is then one v_fmac_f32 with the
+// partner's value read through DPP (a constant would take the VOP3 form,
+// which cannot read through DPP on this target)
+__device__ __forceinline__ float hex_r_vgpr() {
+  float r;
+  asm volatile("v_mov_b32 %0, 0x3e8930a3" : "=v"(r)); // HEX_R
+  return r;
+}
+
+// [J' | H'] rows xi, eta, zeta at this lane's Gauss point, from the node
+// values of one component.
+__device__ __forceinline__ void hex_sf_grad(float R, float v, float &rx, float &ry,
+                                            float &rz) {
+  const float p = lane_x4(v);
+  const float a = v - p, b = v + p * R;                 // axis 2: D, I
+  const float a1 = a + lane_x2(a) * R;                  // axis 1: I, D, I
+  const float b1 = b - lane_x2(b), c1 = b + lane_x2(b) * R;
+  rz = a1 + lane_x1(a1) * R;                            // axis 0: I, I, D
+  ry = b1 + lane_x1(b1) * R;
+  rx = c1 - lane_x1(c1);
+}
+
+// One force component of this lane's node from Q'_(xi, eta, zeta).
+__device__ __forceinline__ float hex_sf_force(float R, float qx, float qy, float qz) {
+  const float A = qx - lane_x1(qx);                     // axis 0: D^T, I, I
+  const float B = qy + lane_x1(qy) * R;
+  const float C = qz + lane_x1(qz) * R;
+  const float E = (A + lane_x2(A) * R) + (B - lane_x2(B)); // axis 1: I, D^T; I
+  const float Cz = C + lane_x2(C) * R;
+  return (E + lane_x4(E) * R) + (Cz - lane_x4(Cz));        // axis 2: I, D^T
+}
+
+// All 64 lanes of the wave call it (cross-lane steps); lanes 8 e .. 8 e + 7
+// are element e, lane bits b = l % 8.  Lanes of elements beyond the range
+// pass active = false and compute on zeros.
+template <int LAW, class FT, class VT, class NA, int W = 3>
+__device__ __forceinline__ void hex8_group_sf(int b, bool active, const int32_t *conn_e,
+                                              const NA &nodes_of, float lambda, float mu,
+                                              float sy, float hh, const VT &V, int64_t iq,
+                                              const int32_t *pos_e, FT *fe) {
+  const int node = b ^ ((b >> 1) & 1);
+  double x[3] = {0., 0., 0.}, d[3] = {0., 0., 0.};
+  int32_t pq = 0;
+  if (active) {
+    nodes_of.load(conn_e[node], x, d);
+    pq = pos_e[node];
+  }
+  const float R = hex_r_vgpr();
+  // relative to node 0 (lane bits 0): exact FP64 difference, rounded once
+  float J[3][3], H[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float xr = float(x[c] - __shfl(x[c], 0, 8));
+    const float ur = float(d[c] - __shfl(d[c], 0, 8));
+    hex_sf_grad(R, xr, J[0][c], J[1][c], J[2][c]);
+    hex_sf_grad(R, ur, H[c][0], H[c][1], H[c][2]);
+  }
+  float Ji[3][3];
+  const float dJ = inv_det(J, Ji);
+  float g[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) {
+      float s = 0.f;
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        s += H[a][m] * Ji[bb][m];
+      g[a][bb] = s;
+    }
+  float S[3][3], P[3][3] = {};
+  if (active)
+    law_at<LAW, float>(V, iq, lambda, mu, sy, hh, g, S, P);
+  // sigma = s_0 s_1 s_2 = +1 when an even number of bits are clear
+  const float sig = ((3 - __builtin_popcount(b)) & 1) ? -1.f : 1.f;
+  const float w = dJ * (sig / HEX_C3);
+  float F[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float q[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      float s = 0.f;
+#pragma unroll
+      for (int J2 = 0; J2 < 3; ++J2)
+        s += P[k][J2] * Ji[J2][m];
+      q[m] = w * s;
+    }
+    F[k] = hex_sf_force(R, q[0], q[1], q[2]);
+  }
+  if (active)
+    akx::put_entry<W>(fe, (int64_t)pq, FT(F[0]), FT(F[1]), FT(F[2]));
+}
+
+template <int LAW, class FT>
+__global__ void __launch_bounds__(256) k_hex8_sf(ElemArgs A) {
+  const int t = threadIdx.x, el = t >> 3, b = t & 7;
+  const int64_t i = xcd_block(A.nblocks_padded) * HEX_EPB + el;
+  const bool active = i < A.n;
+  int64_t e = 0;
+  float lambda = 0.f, mu = 0.f, sy = 0.f, hh = 0.f;
+  if (active) {
+    e = A.filter ? (int64_t)A.filter[i] : A.first + i;
+    element_params(A, i, lambda, mu, sy, hh);
+  }
+  hex8_group_sf<LAW, FT, QpViewT<double>>(
+      b, active, A.conn + e * 8, NodesSoA{A.X, A.u, A.nn}, lambda, mu, sy, hh, qp_view(A, 8),
+      i * 8 + b, A.pos + (A.pos_offset + (active ? i : 0)) * 8, reinterpret_cast<FT *>(A.fe));
+}
+
 template <int LAW, class CT, class FT>
 __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
   __shared__ CT sN[HEX_EPB][8][6];     // X (0..2) and u (3..5) of the 8 nodes
@@ -414,13 +559,7 @@ __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
 
 template <int LAW, int EPB>
 __global__ void __launch_bounds__(EPB * 8) k_batch_hex8_f32(BatchElemArgs A) {
-  __shared__ float sN[EPB][8][6];
-  __shared__ float sQ[EPB][8][9];
-  __shared__ float sD[8][3][8];
   const int t = threadIdx.x, el = t >> 3, q = t & 7;
-  for (int k = t; k < 192; k += EPB * 8)
-    (&sD[0][0][0])[k] = float(A.hex_tab[k]);
-  __syncthreads();
   int64_t t0, t1;
   batch_chunk(A.ntiles, A.nblocks, t0, t1);
   for (int64_t tt = t0; tt < t1; ++tt) {
@@ -430,8 +569,8 @@ __global__ void __launch_bounds__(EPB * 8) k_batch_hex8_f32(BatchElemArgs A) {
     for (int base = 0; base < T.count; base += EPB) {
       const bool active = base + el < T.count;
       const int64_t e = T.first + (active ? base + el : 0);
-      hex8_group<LAW, float, float, QpCompact, EPB, NodesXU, FE_W>(
-          sN, sQ, sD, el, q, active, A.conn + e * 8, NodesXU{A.xu}, float(P.lambda),
+      hex8_group_sf<LAW, float, QpCompact, NodesXU, FE_W>(
+          q, active, A.conn + e * 8, NodesXU{A.xu}, float(P.lambda),
           float(P.mu), float(P.sigma_y), float(P.h), V, A.hoff + e * 8 + q,
           A.pos + e * 8, reinterpret_cast<float *>(A.fe));
     }
@@ -450,10 +589,12 @@ int64_t elements_per_block(int etype) { return etype == 1 ? HEX_EPB : 256; }
 template <int L, class CT, class FT>
 static void launch_one(int etype, const ElemArgs &A, hipStream_t s) {
   dim3 grid((unsigned)A.nblocks_padded), block(256);
-  if (etype == 1)
-    hipLaunchKernelGGL((k_hex8_fast<L, CT, FT>), grid, block, 0, s, A);
-  else
+  if (etype != 1)
     hipLaunchKernelGGL((k_tet4_fast<L, CT, FT>), grid, block, 0, s, A);
+  else if constexpr (sizeof(CT) == 4) // FP32: sum-factorised Hex8
+    hipLaunchKernelGGL((k_hex8_sf<L, FT>), grid, block, 0, s, A);
+  else
+    hipLaunchKernelGGL((k_hex8_fast<L, CT, FT>), grid, block, 0, s, A);
 }
 
 template <class CT, class FT>
