@@ -316,12 +316,7 @@ __device__ __forceinline__ void hex8_group(CT (&sN)[EPB][8][6], CT (&sQ)[EPB][8]
 // Bits -> node of the reference's numbering (counter-clockwise faces):
 // n(b) = b ^ ((b >> 1) & 1); the Gauss point index is the bits themselves.
 // ---------------------------------------------------------------------------
-template <int CTRL> __device__ __forceinline__ float dpp_f(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
-}
-__device__ __forceinline__ float lane_x1(float v) { return dpp_f<0xB1>(v); } // quad_perm 1,0,3,2
-__device__ __forceinline__ float lane_x2(float v) { return dpp_f<0x4E>(v); } // quad_perm 2,3,0,1
-__device__ __forceinline__ float lane_x4(float v) {                          // swizzle xor 4
+__device__ __forceinline__ float lane_x4(float v) { // value of the lane 4 away (swizzle)
   return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
 }
 
@@ -329,9 +324,34 @@ constexpr float HEX_G = 0.57735026918962576f;          // 1 / sqrt(3)
 constexpr float HEX_R = 0.26794919243112270f;          // beta / alpha = (1 - G) / (1 + G)
 constexpr float HEX_C3 = 33.243067814700545f;          // c^3, c = 2 / alpha^2
 
-// r in a vector register: an I step is then one v_fmac_f32 with the
-// partner's value read through DPP (a constant would take the VOP3 form,
-// which cannot read through DPP on this target)
+// One butterfly step as one instruction, the partner's value read through
+// DPP: acc + r v_partner (v_fmac_f32_dpp) and v - v_partner
+// (v_subrev_f32_dpp).  The compiler does not fuse a DPP move into an FMA
+// here, so the steps are written out; s_nop 4 gives the wait states a DPP
+// read needs after a VALU write of its source (2) or of EXEC (5), which the
+// compiler cannot account for inside asm.  Partners are lanes of the same
+// element, so they are active whenever the lane is.
+#define AKXF_DPP_STEPS(SUFFIX, PERM)                                                      \
+  __device__ __forceinline__ float fmac_##SUFFIX(float acc, float v, float r) {           \
+    asm("s_nop 4\n\tv_fmac_f32_dpp %0, %1, %2 quad_perm:" PERM                            \
+        " row_mask:0xf bank_mask:0xf bound_ctrl:1"                                         \
+        : "+v"(acc)                                                                        \
+        : "v"(v), "v"(r));                                                                 \
+    return acc;                                                                            \
+  }                                                                                        \
+  __device__ __forceinline__ float diff_##SUFFIX(float v) {                                \
+    float d;                                                                               \
+    asm("s_nop 4\n\tv_subrev_f32_dpp %0, %1, %1 quad_perm:" PERM                          \
+        " row_mask:0xf bank_mask:0xf bound_ctrl:1"                                         \
+        : "=v"(d)                                                                          \
+        : "v"(v));                                                                         \
+    return d;                                                                              \
+  }
+AKXF_DPP_STEPS(x1, "[1,0,3,2]")
+AKXF_DPP_STEPS(x2, "[2,3,0,1]")
+#undef AKXF_DPP_STEPS
+
+// r in a vector register (the source operand of v_fmac_f32_dpp)
 __device__ __forceinline__ float hex_r_vgpr() {
   float r;
   asm volatile("v_mov_b32 %0, 0x3e8930a3" : "=v"(r)); // HEX_R
@@ -344,21 +364,21 @@ __device__ __forceinline__ void hex_sf_grad(float R, float v, float &rx, float &
                                             float &rz) {
   const float p = lane_x4(v);
   const float a = v - p, b = v + p * R;                 // axis 2: D, I
-  const float a1 = a + lane_x2(a) * R;                  // axis 1: I, D, I
-  const float b1 = b - lane_x2(b), c1 = b + lane_x2(b) * R;
-  rz = a1 + lane_x1(a1) * R;                            // axis 0: I, I, D
-  ry = b1 + lane_x1(b1) * R;
-  rx = c1 - lane_x1(c1);
+  const float a1 = fmac_x2(a, a, R);                    // axis 1: I, D, I
+  const float b1 = diff_x2(b), c1 = fmac_x2(b, b, R);
+  rz = fmac_x1(a1, a1, R);                              // axis 0: I, I, D
+  ry = fmac_x1(b1, b1, R);
+  rx = diff_x1(c1);
 }
 
 // One force component of this lane's node from Q'_(xi, eta, zeta).
 __device__ __forceinline__ float hex_sf_force(float R, float qx, float qy, float qz) {
-  const float A = qx - lane_x1(qx);                     // axis 0: D^T, I, I
-  const float B = qy + lane_x1(qy) * R;
-  const float C = qz + lane_x1(qz) * R;
-  const float E = (A + lane_x2(A) * R) + (B - lane_x2(B)); // axis 1: I, D^T; I
-  const float Cz = C + lane_x2(C) * R;
-  return (E + lane_x4(E) * R) + (Cz - lane_x4(Cz));        // axis 2: I, D^T
+  const float A = diff_x1(qx);                          // axis 0: D^T, I, I
+  const float B = fmac_x1(qy, qy, R);
+  const float C = fmac_x1(qz, qz, R);
+  const float E = fmac_x2(A, A, R) + diff_x2(B);        // axis 1: I, D^T; I
+  const float Cz = fmac_x2(C, C, R);
+  return (E + lane_x4(E) * R) + (Cz - lane_x4(Cz));     // axis 2: I, D^T
 }
 
 // All 64 lanes of the wave call it (cross-lane steps); lanes 8 e .. 8 e + 7

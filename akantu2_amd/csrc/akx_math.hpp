@@ -248,16 +248,22 @@ template <class R> AKX_HD R akx_log1p(R x);
 template <> AKX_HD double akx_log1p<double>(double x) { return log1p(x); }
 template <> AKX_HD float akx_log1p<float>(float x) { return log1pf(x); }
 
+// E, C^{-1} and S are symmetric (C^{-1} is a function of E, so C^{-1} E is
+// too): each is formed once per pair (i <= j) and mirrored.
 template <class R>
 AKX_HD void neohookean_pk2_stable(R lambda, R mu, const R (&g)[3][3],
                                   R (&S)[3][3]) {
-  R E[3][3], C[3][3], Ci[3][3];
-  green_strain(g, E);
+  constexpr int I6[6] = {0, 1, 2, 0, 0, 1}, J6[6] = {0, 1, 2, 1, 2, 2};
+  R E[6];
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int v = 0; v < 6; ++v) {
+    const int i = I6[v], j = J6[v];
+    R p = g[0][i] * g[0][j];
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
-      C[i][j] = (i == j) ? R(1) + R(2) * E[i][j] : R(2) * E[i][j];
+    for (int k = 1; k < 3; ++k)
+      p += g[k][i] * g[k][j];
+    E[v] = (p + g[j][i] + g[i][j]) * R(0.5);
+  }
   R tr = g[0][0] + g[1][1] + g[2][2];
   R trg2 = R(0);
 #pragma unroll
@@ -267,23 +273,30 @@ AKX_HD void neohookean_pk2_stable(R lambda, R mu, const R (&g)[3][3],
       trg2 += g[i][k] * g[k][i];
   R jm1 = tr + R(0.5) * (tr * tr - trg2) + det3(g);
   R lnJ = akx_log1p(jm1);
-  inv3(C, Ci);
-  R T[3][3];
+  // C = I + 2 E (Voigt order 00 11 22 01 02 12) and its inverse by cofactors
+  const R c00 = R(1) + R(2) * E[0], c11 = R(1) + R(2) * E[1], c22 = R(1) + R(2) * E[2];
+  const R c01 = R(2) * E[3], c02 = R(2) * E[4], c12 = R(2) * E[5];
+  R k00 = c11 * c22 - c12 * c12, k11 = c00 * c22 - c02 * c02, k22 = c00 * c11 - c01 * c01;
+  R k01 = c02 * c12 - c01 * c22, k02 = c01 * c12 - c02 * c11, k12 = c01 * c02 - c00 * c12;
+  const R id = R(1) / (c00 * k00 + c01 * k01 + c02 * k02);
+  const R Ci[3][3] = {{k00 * id, k01 * id, k02 * id},
+                      {k01 * id, k11 * id, k12 * id},
+                      {k02 * id, k12 * id, k22 * id}};
+  // T = 2 mu E + lambda ln J I
+  const R m2 = R(2) * mu, ll = lambda * lnJ;
+  const R T[3][3] = {{m2 * E[0] + ll, m2 * E[3], m2 * E[4]},
+                     {m2 * E[3], m2 * E[1] + ll, m2 * E[5]},
+                     {m2 * E[4], m2 * E[5], m2 * E[2] + ll}};
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int v = 0; v < 6; ++v) {
+    const int i = I6[v], j = J6[v];
+    R s = Ci[i][0] * T[0][j];
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
-      T[i][j] = (i == j) ? R(2) * mu * E[i][j] + lambda * lnJ : R(2) * mu * E[i][j];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      R s = R(0);
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        s += Ci[i][k] * T[k][j];
-      S[i][j] = s;
-    }
+    for (int k = 1; k < 3; ++k)
+      s += Ci[i][k] * T[k][j];
+    S[i][j] = s;
+    S[j][i] = s;
+  }
 }
 
 // Linear isotropic hardening, small strain, radial return:

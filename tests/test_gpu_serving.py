@@ -80,8 +80,7 @@ def test_fp32_server_error_vs_oracle():
         o.solve_steps(r.steps)
         eu = relerr(r.result["displacement"], o.u)
         print(f"request {r.rid} {r.etype} {r.law} u={eu:.2e}", flush=True)
-        tol = 2e-5 if (r.etype == "_hexahedron_8" and r.law == "neohookean") else 1e-5
-        assert eu <= tol
+        assert eu <= 1e-5
     srv.close()
 
 
