@@ -65,7 +65,6 @@ __global__ void __launch_bounds__(256) k_batch_tet4(BatchElemArgs A) {
 template <int LAW, int EPB>
 __global__ void __launch_bounds__(EPB * 8) k_batch_hex8(BatchElemArgs A) {
   __shared__ double sN[EPB][8][6];
-  __shared__ double sR[EPB][8][13];
   __shared__ double sD[8][3][8];
   const int t = threadIdx.x, el = t >> 3, q = t & 7;
   for (int k = t; k < 192; k += EPB * 8)
@@ -81,7 +80,7 @@ __global__ void __launch_bounds__(EPB * 8) k_batch_hex8(BatchElemArgs A) {
     for (int base = 0; base < T.count; base += EPB) {
       const bool active = base + el < T.count;
       const int64_t e = T.first + (active ? base + el : 0);
-      hex8_strict_group<LAW, EPB, double, NodesXU, FE_W>(sN, sR, sD, wq, el, q, active,
+      hex8_strict_group<LAW, EPB, double, NodesXU, FE_W>(sN, sD, wq, el, q, active,
                                                          A.conn + e * 8,
                                           NodesXU{A.xu}, P.lambda, P.mu, P.sigma_y, P.h,
                                           V, A.hoff + e * 8 + q, A.pos + e * 8,
@@ -577,6 +576,11 @@ void rebuild_lists(akx_batch *b) {
   b->lists_dirty = false;
 }
 
+// persistent blocks per CU of the gather (256 threads each)
+#ifndef AKX_GATHER_BLOCKS_PER_CU
+#define AKX_GATHER_BLOCKS_PER_CU 8
+#endif
+
 int64_t persistent_grid(int64_t nwork, int64_t cap) {
   const int64_t nb = std::min<int64_t>(nwork, cap);
   return std::max<int64_t>(8, (nb + 7) / 8 * 8);
@@ -660,7 +664,7 @@ BatchGatherArgs gather_args(akx_batch *b) {
   G.cu = b->cu.p;
   G.cv = b->cv.p;
   G.ncap = b->ncap;
-  G.nblocks = persistent_grid(G.nslices, (int64_t)b->cus * 8);
+  G.nblocks = persistent_grid(G.nslices, (int64_t)b->cus * AKX_GATHER_BLOCKS_PER_CU);
   require(G.slices && G.ell_off && G.deg && G.fe && G.fint && G.xu && G.v && G.a &&
               G.mass && G.blocked,
           "batch node arena not allocated");
@@ -1272,7 +1276,7 @@ int akx_batch_autotune(akx_batch *b, int repeats, int *block, int *tile,
       upload_inst(b);
     const int blocks[3] = {64, 128, 256};
     const int tiles[5] = {256, 512, 1024, 2048, 4096};
-    const int bpcs[4] = {1, 2, 4, 8};
+    const int bpcs[4] = {2, 4, 8, 16};
     double best = 1e300;
     int bb = b->block, bt = b->tile, bc = b->blocks_per_cu;
     hipEvent_t e0, e1;

@@ -480,8 +480,10 @@ k_element_forces(ElemArgs A) {
 // and displacements are staged in LDS; each lane forms J, its LU inverse,
 // the dN/dX columns, grad u, the law and the 24 force terms of its Gauss
 // point times w*detJ; the per-node sums over the Gauss points are then taken
-// in ascending Gauss-point order (the oracle's and Eigen's GEMV order), four
-// nodes at a time, by the lanes that own the nodes.
+// in ascending Gauss-point order (the oracle's and Eigen's GEMV order), two
+// nodes at a time, by the lanes that own the nodes.  The force terms reuse
+// the LDS of the nodal values, which are dead once grad u is formed: 4.6 KB
+// of LDS per 8 elements, so registers, not LDS, bound the waves per SIMD.
 //
 // hex8_strict_group handles one group of EPB elements; every lane of the
 // block calls it (it holds barriers), inactive lanes with active = false.
@@ -491,7 +493,7 @@ constexpr int HEX_STRICT_EPB = 8;
 
 template <int LAW, int EPB, class HT, class NA, int W = 3>
 __device__ __forceinline__ void hex8_strict_group(
-    double (&sN)[EPB][8][6], double (&sR)[EPB][8][13], const double (&sD)[8][3][8],
+    double (&sN)[EPB][8][6], const double (&sD)[8][3][8],
     double wq, int el, int q, bool active, const int32_t *conn_e, const NA &nodes_of,
     double lambda, double mu, double sy, double hh,
     const QpViewT<HT> &V, int64_t iq, const int32_t *pos_e, double *fe) {
@@ -541,14 +543,16 @@ __device__ __forceinline__ void hex8_strict_group(
     }
     strict_law<LAW, double, HT>(V, iq, lambda, mu, sy, hh, g, S);
   }
-  // two halves of four nodes: the force terms of this Gauss point go to LDS,
-  // then the lanes q < 4 sum node 4h + q over the Gauss points in order
+  __syncthreads(); // every lane is done with sN: it now holds force terms
+  double(&sR)[EPB][8][6] = sN;
+  // four quarters of two nodes: the force terms of this Gauss point go to
+  // LDS, then the lanes q < 2 sum node 2 h + q over the Gauss points in order
 #pragma unroll
-  for (int hh2 = 0; hh2 < 2; ++hh2) {
+  for (int hh2 = 0; hh2 < 4; ++hh2) {
     if (active) {
 #pragma unroll
-      for (int nn = 0; nn < 4; ++nn) {
-        const int n = 4 * hh2 + nn;
+      for (int nn = 0; nn < 2; ++nn) {
+        const int n = 2 * hh2 + nn;
         double b[3], r[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -567,8 +571,8 @@ __device__ __forceinline__ void hex8_strict_group(
       }
     }
     __syncthreads();
-    if (active && q < 4) {
-      const int n = 4 * hh2 + q;
+    if (active && q < 2) {
+      const int n = 2 * hh2 + q;
       double r[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -587,7 +591,6 @@ __device__ __forceinline__ void hex8_strict_group(
 template <int LAW>
 __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
   __shared__ double sN[HEX_STRICT_EPB][8][6];
-  __shared__ double sR[HEX_STRICT_EPB][8][13]; // 12 terms (+1 pad: banks)
   __shared__ double sD[8][3][8];
   const int t = threadIdx.x, el = t >> 3, q = t & 7;
   for (int k = t; k < 192; k += 64)
@@ -608,7 +611,7 @@ __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
       hh = A.h_el[i];
   }
   hex8_strict_group<LAW, HEX_STRICT_EPB, double>(
-      sN, sR, sD, c_hex8.w[q], el, q, active, A.conn + e * 8, NodesSoA{A.X, A.u, A.nn},
+      sN, sD, c_hex8.w[q], el, q, active, A.conn + e * 8, NodesSoA{A.X, A.u, A.nn},
       lambda, mu, sy, hh, qp_view(A, 8), i * 8 + q, A.pos + (A.pos_offset + (active ? i : 0)) * 8,
       reinterpret_cast<double *>(A.fe));
 }

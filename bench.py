@@ -42,15 +42,20 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix, FMA = 2 flops)
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 (dense matrix; packed vector)
+FP32_VALU_TFLOPS = 78.6        # MI355X FP32 vector without packed instructions
 # Floating-point operations per Gauss point of the Hex8 neo-Hookean element
 # pass in the reference's operation order (adds, multiplies, the 6 divisions
-# and the logarithm counted once each; breakdown in DESIGN.md):
+# and the logarithm counted once each):
 # J 135, LU 13 + det 3, inverse 48, w detJ 1, dN/dX for grad u 120,
 # grad u 135, neo-Hookean PK2 119, dN/dX for the force 120,
 # (B_L + B_NL)^T S 624, times w detJ 24, Gauss-point sum 21.
 HEX8_NEOHOOKEAN_FLOP_PER_QP = 1363
 SHAPE = (200, 100, 100)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_hex8_neohookean_f64.json")
+# executed flops per Gauss point of the element passes, counted with
+# rocprofv3 (SQ_INSTS_VALU_FLOPS_FP32 / _FP64 x 64 lanes)
+FLOPS_FILE = os.path.join(ROOT, "profiles", "flops_hex8_neohookean_200x100x100.json")
 PL = "plastic_linear_isotropic_hardening"
 
 
@@ -189,6 +194,25 @@ def run_plain(args):
                     "kernel": "k_hex8_strict<neo-Hookean> (FP64, reference operation order)",
                     "compute": "FP64 VALU, FMA-free by construction (ceiling 39.3 TFLOP/s)",
                     "algorithmic_flops_per_launch": flops,
+                    "algorithmic_bytes_per_launch": el["bytes"],
+                    "avg_kernel_ms": el["avg_ms"],
+                    "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": hbm_gbs / HBM_PEAK_GBS,
+                            "measured_copy_gbs": copy_gbs}}
+    elif args.precision == "fp32" and os.path.exists(FLOPS_FILE):
+        # The sum-factorised FP32 element pass is bound by FP32 vector issue:
+        # its flops are the ones the counters saw per Gauss point.
+        with open(FLOPS_FILE) as fh:
+            fq = json.load(fh)["f32_sf"]["flops_per_qp"]
+        flops = fq * nqp
+        tflops = flops / (el["avg_ms"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS, "traffic": None,
+                    "kernel": "k_hex8_sf<neo-Hookean> (FP32, sum-factorised)",
+                    "compute": "FP32 VALU; packed FP32 instructions are not used, so the "
+                               f"ceiling is {FP32_VALU_TFLOPS} TFLOP/s "
+                               f"({tflops / FP32_VALU_TFLOPS:.0%} of it)",
+                    "flops_per_qp": fq, "algorithmic_flops_per_launch": flops,
                     "algorithmic_bytes_per_launch": el["bytes"],
                     "avg_kernel_ms": el["avg_ms"],
                     "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -345,9 +369,14 @@ def cpu_baseline_precision(shape=(60, 10, 10), steps=200):
 # --------------------------------------------------------------------------
 # batching form: 512 Tet4 plasticity instances
 # --------------------------------------------------------------------------
-def c4_instances(n=512, seed=4, scale=1.0):
+ETYPES = {"tet4": "_tetrahedron_4", "hex8": "_hexahedron_8"}
+
+
+def c4_instances(n=512, seed=4, scale=1.0, etype="tet4", law=PL):
     """Instance descriptions: cells, material, seed (scale shrinks the sizes
-    for a quick run)."""
+    for a quick run).  The batching workload is Tet4 with linear isotropic
+    hardening; etype / law give the same stream of sizes and aspects for
+    another element type or law."""
     from akantu2_amd import mesh as M
     rng = np.random.default_rng(seed)
     nel = np.exp(rng.uniform(np.log(5e3), np.log(4e5), n)) * scale
@@ -358,28 +387,32 @@ def c4_instances(n=512, seed=4, scale=1.0):
     hh = rng.uniform(0.0, 0.1, n)
     out = []
     for k in range(n):
-        cells = M.cells_for("_tetrahedron_4", max(nel[k], 6.0), aspect[k])
-        out.append(dict(cells=cells, mat=dict(E=float(E[k]), nu=float(nu[k]),
-                                              sigma_y=float(sy[k]), h=float(hh[k])),
-                        seed=1000 + k))
+        cells = M.cells_for(ETYPES[etype], max(nel[k], 6.0), aspect[k])
+        mat = dict(E=float(E[k]), nu=float(nu[k]))
+        if law == PL:
+            mat.update(sigma_y=float(sy[k]), h=float(hh[k]))
+        out.append(dict(cells=cells, mat=mat, seed=1000 + k, etype=etype, law=law))
     return out
 
 
 def c4_mesh(spec):
     from akantu2_amd import mesh as M
-    mesh = M.box_mesh("_tetrahedron_4", *spec["cells"])
+    mesh = M.box_mesh(ETYPES[spec.get("etype", "tet4")], *spec["cells"])
     return mesh, M.clamped_face_x0(mesh), M.initial_velocity(mesh, 1e-2, spec["seed"])
 
 
 def _fill_batch(precision, specs):
     from akantu2_amd.batch import Batch
     nn = sum((c[0] + 1) * (c[1] + 1) * (c[2] + 1) + 255 for c in (s["cells"] for s in specs))
-    nt = sum(6 * c[0] * c[1] * c[2] for c in (s["cells"] for s in specs))
-    b = Batch(precision, nodes=nn, tets=nt, plastic=True)
+    ncell = sum(c[0] * c[1] * c[2] for c in (s["cells"] for s in specs))
+    hex8 = specs[0].get("etype") == "hex8"
+    law = specs[0].get("law", PL)
+    b = Batch(precision, nodes=nn, tets=0 if hex8 else 6 * ncell, hexes=ncell if hex8 else 0,
+              plastic=law == PL)
     ids = []
     for s in specs:
         mesh, blocked, v0 = c4_mesh(s)
-        ids.append(b.add(mesh, PL, blocked=blocked, velocity=v0, dt_factor=0.5, **s["mat"]))
+        ids.append(b.add(mesh, law, blocked=blocked, velocity=v0, dt_factor=0.5, **s["mat"]))
     return b, ids
 
 
@@ -405,13 +438,14 @@ def _time_batch(b, warmup, steps, tune):
 
 def run_batch(args):
     steps = args.steps or 500
-    specs = c4_instances(args.instances, scale=args.scale)
+    law = args.law or (PL if args.etype == "tet4" else "neohookean")
+    specs = c4_instances(args.instances, scale=args.scale, etype=args.etype, law=law)
     t = time.perf_counter()
     res = {}
     for prec in ("fp32", "fp64"):
         b, ids = _fill_batch(prec, specs)
         st = b.stats()
-        print(f"# {prec} batch: {st['live']} instances, {st['elements']} Tet4, "
+        print(f"# {prec} batch: {st['live']} instances, {st['elements']} {args.etype}, "
               f"{st['nodes']} nodes, setup {time.perf_counter() - t:.1f} s",
               file=sys.stderr, flush=True)
         el, kern, best = _time_batch(b, args.warmup, steps, tune=True)
@@ -431,15 +465,19 @@ def run_batch(args):
         "unit": "elem*qp/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": res["fp32"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{len(specs)} independent Tet4 boxes (5e3-4e5 elements "
-                               "log-uniform x scale, aspect U(1,8)), linear isotropic "
-                               "hardening with per-instance E, nu, sigma_y, h, FP32 form, "
-                               "persistent launches in a hipGraph, autotuned",
+        "config": {"workload": f"{len(specs)} independent {ETYPES[args.etype]} boxes "
+                               "(5e3-4e5 elements log-uniform x scale, aspect U(1,8)), "
+                               + ("linear isotropic hardening with per-instance E, nu, "
+                                  "sigma_y, h" if law == PL else
+                                  f"{law} with per-instance E, nu")
+                               + ", FP32 form, persistent launches in a hipGraph, autotuned",
                    "instances": len(specs), "scale": args.scale,
-                   "elements": int(st["elements"])},
+                   "elements": int(st["elements"]),
+                   "quadrature_points": int(st["quadrature_points"])},
         "roofline": {"bound": "hbm", "achieved": el["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": el["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_batch_tet4_f32<plastic>",
+                     "traffic": None,
+                     "kernel": f"k_batch_{args.etype}_f32<{'plastic' if law == PL else law}>",
                      "algorithmic_bytes_per_launch": el["bytes"],
                      "avg_kernel_ms": el["avg_ms"]},
         "forms": res,
@@ -455,7 +493,10 @@ def cpu_baseline_batch(spec, gpu_u, steps):
     thread, and the FP32 batch's displacement error against it."""
     import oracle
     mesh, blocked, v0 = c4_mesh(spec)
-    o = oracle.OracleModel(oracle.TET4, mesh.nodes, mesh.conn, PL, **spec["mat"])
+    hex8 = spec.get("etype") == "hex8"
+    law = spec.get("law", PL)
+    o = oracle.OracleModel(oracle.HEX8 if hex8 else oracle.TET4, mesh.nodes, mesh.conn, law,
+                           **spec["mat"])
     o.init_full()
     o.blocked[:] = blocked
     o.v[:] = v0
@@ -463,10 +504,12 @@ def cpu_baseline_batch(spec, gpu_u, steps):
     t0 = time.perf_counter()
     o.solve_steps(steps)
     el = time.perf_counter() - t0
-    return {"value": mesh.nb_elements * steps / el, "unit": "elem*qp/s", "cores": 1,
+    nqp = 8 if hex8 else 1
+    return {"value": mesh.nb_elements * nqp * steps / el, "unit": "elem*qp/s", "cores": 1,
             "kind": "port", "seconds": el,
-            "sample": f"{steps} steps of the smallest batch instance "
-                      f"({mesh.nb_elements} Tet4, plastic), oracle/akantu_oracle.c, 1 thread",
+            "sample": f"{steps} steps of the smallest batch instance ({mesh.nb_elements} "
+                      f"{ETYPES[spec.get('etype', 'tet4')]}, {law}), oracle/akantu_oracle.c, "
+                      "1 thread",
             "fp32_u_error_vs_oracle": rel_errors(gpu_u, o.u)}
 
 
@@ -541,6 +584,11 @@ def main():
     ap.add_argument("--shape", type=str, default=None,
                     help="override cells per direction, e.g. 20,10,10")
     ap.add_argument("--instances", type=int, default=512, help="batch form")
+    ap.add_argument("--etype", choices=["tet4", "hex8"], default="tet4",
+                    help="batch form: element type of the instances")
+    ap.add_argument("--law", type=str, default=None,
+                    help="batch form: material law (default: plasticity for Tet4, "
+                         "neohookean for Hex8)")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="batch / serve forms: multiply the instance sizes")
     ap.add_argument("--requests", type=int, default=64, help="serve form")
