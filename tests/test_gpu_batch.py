@@ -153,6 +153,23 @@ def test_fp32_batch_plastic_error_bound():
     assert yielded >= 2
 
 
+def test_fp32_batch_every_law_against_oracle():
+    """FP32 batch of the seven-instance mix (both element types, all four
+    laws, the Hex8 instances on the sum-factorised pass): each instance's
+    displacement within 1e-5 of the FP64 oracle after 240 steps."""
+    b, ids = _batch("fp32", SPECS)
+    b.solveStep(240)
+    for spec, iid in zip(SPECS, ids):
+        mesh, blocked, v0, law, mat = _case(spec)
+        o, dt = make_oracle(mesh, blocked, v0, law, **mat)
+        o.solve_steps(240)
+        eu = relerr(b.getDisplacement(iid), o.u)
+        ef = relerr(b.getInternalForce(iid), o.f_int)
+        print(f"fp32 batch {spec[0]} {law} u={eu:.2e} f_int={ef:.2e}", flush=True)
+        assert eu <= 1e-5, (spec, eu)
+        assert ef <= 1e-4, (spec, ef)
+
+
 def test_autotune_keeps_state():
     ref, ids = _batch("fp64", SPECS)
     ref.solveStep(20)
