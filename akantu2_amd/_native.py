@@ -19,6 +19,7 @@ ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "negative jacobia
 
 TET4, HEX8, TRI3, QUAD4 = 0, 1, 2, 3
 LAW_ELASTIC, LAW_ELASTIC_FD, LAW_NEOHOOKEAN, LAW_PLASTIC_LIH = 0, 1, 2, 3
+LAW_USER, LAW_USER_FD = 4, 5  # computeStress supplied by the caller
 PRECISION_FP64, PRECISION_FP32, PRECISION_FP64_FAST = 0, 1, 2
 FIELD_DISPLACEMENT, FIELD_VELOCITY, FIELD_ACCELERATION = 0, 1, 2
 FIELD_INTERNAL_FORCE, FIELD_EXTERNAL_FORCE, FIELD_MASS = 3, 4, 5
@@ -43,6 +44,10 @@ _vp = C.c_void_p
 _i64 = C.c_int64
 _dp = C.POINTER(C.c_double)
 
+# int (*)(void *user, int material_id, int64_t nb_qp, const double *grad_u,
+#         double *stress)
+COMPUTE_STRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int64, _dp, _dp)
+
 
 def _declare(lib):
     sigs = {
@@ -57,6 +62,7 @@ def _declare(lib):
         "akx_model_set_element_parameters": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
         "akx_material_law_by_name": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
         "akx_model_set_precision": (C.c_int, [_vp, C.c_int]),
+        "akx_model_set_stress_callback": (C.c_int, [_vp, COMPUTE_STRESS_FN, _vp]),
         "akx_model_init_full": (C.c_int, [_vp]),
         "akx_model_get_stable_time_step": (C.c_int, [_vp, _dp]),
         "akx_model_set_time_step": (C.c_int, [_vp, C.c_double]),

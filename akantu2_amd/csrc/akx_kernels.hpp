@@ -30,6 +30,17 @@ constexpr int LAW_ELASTIC = 0;
 constexpr int LAW_ELASTIC_FD = 1;
 constexpr int LAW_NEOHOOKEAN = 2;
 constexpr int LAW_PLASTIC = 3;
+// A law computed by the caller (Material::computeStress plugin), in two
+// passes: LAW_USER_GRAD forms grad u at every Gauss point and stores it (no
+// forces); after the caller filled the stress array, LAW_USER (sigma, small
+// strain) or LAW_USER_FD (PK2, finite deformation) reads it and forms the
+// forces.
+constexpr int LAW_USER_GRAD = 4;
+constexpr int LAW_USER = 5;
+constexpr int LAW_USER_FD = 6;
+template <int LAW> constexpr bool finite_law() {
+  return LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN || LAW == LAW_USER_FD;
+}
 
 template <int ET> struct Elem;
 template <> struct Elem<ET_TET4> {
@@ -327,6 +338,23 @@ __device__ __forceinline__ void strict_law(const QpViewT<HT> &V, int64_t iq, CT 
     elastic_stress(lambda, mu, E, CT(-0.0), S);
   } else if constexpr (LAW == LAW_NEOHOOKEAN) {
     neohookean_pk2(lambda, mu, g, S);
+  } else if constexpr (LAW == LAW_USER_GRAD) {
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+        V.grad_u[(int64_t)(ii + 3 * jj) * V.nq_all + iq] = double(g[ii][jj]);
+        S[ii][jj] = CT(0);
+      }
+    return;
+  } else if constexpr (LAW == LAW_USER || LAW == LAW_USER_FD) {
+    // the stress the caller's computeStress wrote for this Gauss point
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii)
+        S[ii][jj] = CT(V.stress[(int64_t)(ii + 3 * jj) * V.nq_all + iq]);
+    return;
   } else {
     CT pg[3][3], ps[3][3], pe[3][3], eps[3][3], iso;
 #pragma unroll
@@ -421,11 +449,13 @@ __device__ __forceinline__ void element_forces_one(
     }
     CT S[3][3];
     strict_law<LAW, CT, HT>(V, iq0 + q, lambda, mu, sy, hh, g, S);
+    if constexpr (LAW == LAW_USER_GRAD)
+      continue;
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
       CT r[3], b[3];
       Shape<ET, CT>::bcol(Ji, q, n, b);
-      if constexpr (LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN)
+      if constexpr (finite_law<LAW>())
         btd_finite(g, S, b, r);
       else
         btd_small(S, b, r);
@@ -435,6 +465,8 @@ __device__ __forceinline__ void element_forces_one(
     }
   }
 
+  if constexpr (LAW == LAW_USER_GRAD)
+    return; // the gradient pass forms no forces
   FT *fe = reinterpret_cast<FT *>(fe_out);
   const int4 *p4 = reinterpret_cast<const int4 *>(pos_e);
 #pragma unroll
@@ -543,6 +575,8 @@ __device__ __forceinline__ void hex8_strict_group(
     }
     strict_law<LAW, double, HT>(V, iq, lambda, mu, sy, hh, g, S);
   }
+  if constexpr (LAW == LAW_USER_GRAD)
+    return; // the gradient pass forms no forces (uniform: no lane waits)
   __syncthreads(); // every lane is done with sN: it now holds force terms
   double(&sR)[EPB][8][6] = sN;
   // four quarters of two nodes: the force terms of this Gauss point go to
@@ -561,7 +595,7 @@ __device__ __forceinline__ void hex8_strict_group(
           s += Ji[a][2] * sD[q][2][n];
           b[a] = s;
         }
-        if constexpr (LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN)
+        if constexpr (finite_law<LAW>())
           btd_finite(g, S, b, r);
         else
           btd_small(S, b, r);

@@ -427,6 +427,8 @@ struct Material {
   DevBuf<double> hgrad[2], hsig[2], heps[2], hiso[2];
   DevBuf<double> hpe[2]; // accumulated plastic energy [n*nqp]
   int cur = 0; // index of the current plastic state
+  // a law computed by the caller (the computeStress plugin)
+  bool user() const { return law == AKX_LAW_USER || law == AKX_LAW_USER_FD; }
 };
 
 struct akx_model {
@@ -451,6 +453,16 @@ struct akx_model {
   DevBuf<unsigned long long> scratch_u64;
   DevBuf<long long> scratch_i64;
   std::vector<std::unique_ptr<Material>> mats;
+  // computeStress of the user materials, and host copies of their internals
+  akx_compute_stress_fn stress_fn = nullptr;
+  void *stress_user = nullptr;
+  std::vector<double> user_grad, user_stress;
+  bool has_user() const {
+    for (auto &mp : mats)
+      if (mp->user())
+        return true;
+    return false;
+  }
   // pipelined schedule (akx_model_set_pipeline): element parts and the node
   // groups whose gather they complete, a second stream and the events
   int pipe_parts = 0;
@@ -608,7 +620,9 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
   A.h_el = mat.h_el.p;
   A.pos = m->pos.p;
   A.fe = m->fe.p;
-  if (materialize && mat.law != AKX_LAW_PLASTIC_LIH) {
+  if (mat.user()) {
+    A.stress = mat.stress.p; // read by the force pass
+  } else if (materialize && mat.law != AKX_LAW_PLASTIC_LIH) {
     A.grad_u = mat.grad.p;
     A.stress = mat.stress.p;
   }
@@ -650,7 +664,9 @@ void check_elem_args(const ElemArgs &A, int law) {
             "plastic element pass launched without history buffers");
   if (A.lam_el || A.mu_el)
     require(A.lam_el && A.mu_el, "per-element parameters incomplete");
-  if (A.grad_u || A.stress)
+  if (law == AKX_LAW_USER || law == AKX_LAW_USER_FD)
+    require(A.stress != nullptr, "user material without its stress array");
+  else if (A.grad_u || A.stress)
     require(A.grad_u && A.stress, "materialised internals incomplete");
 }
 
@@ -674,6 +690,14 @@ void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
     hipLaunchKernelGGL((k_element_forces<ET, LAW_PLASTIC, CT, FT>), grid, block,
                        0, m->stream, A);
     break;
+  case AKX_LAW_USER:
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_USER, CT, FT>), grid, block, 0,
+                       m->stream, A);
+    break;
+  case AKX_LAW_USER_FD:
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_USER_FD, CT, FT>), grid, block, 0,
+                       m->stream, A);
+    break;
   default:
     throw Error(AKX_ERR_UNSUPPORTED, "unknown law");
   }
@@ -694,6 +718,12 @@ void launch_hex8_strict(akx_model *m, Material &mat, const ElemArgs &A) {
     break;
   case AKX_LAW_PLASTIC_LIH:
     hipLaunchKernelGGL(k_hex8_strict<LAW_PLASTIC>, grid, block, 0, m->stream, A);
+    break;
+  case AKX_LAW_USER:
+    hipLaunchKernelGGL(k_hex8_strict<LAW_USER>, grid, block, 0, m->stream, A);
+    break;
+  case AKX_LAW_USER_FD:
+    hipLaunchKernelGGL(k_hex8_strict<LAW_USER_FD>, grid, block, 0, m->stream, A);
     break;
   default:
     throw Error(AKX_ERR_UNSUPPORTED, "unknown law");
@@ -732,6 +762,46 @@ void launch_element_pass(akx_model *m, bool materialize) {
                                        m->stream));
       }
     });
+  }
+}
+
+// The computeStress plugin of the user materials (Material::computeStress,
+// material.cc:128-154): grad u at every Gauss point of the material (the
+// reference order, as the other laws form it), handed to the caller, whose
+// stress goes back to the device for the force pass.
+void user_stress_phase(akx_model *m) {
+  if (!m->has_user())
+    return;
+  require(m->stress_fn != nullptr,
+          "a user material needs a computeStress callback (akx_model_set_stress_callback)");
+  for (int k = 0; k < (int)m->mats.size(); ++k) {
+    Material &mat = *m->mats[k];
+    if (!mat.user() || mat.n == 0)
+      continue;
+    const int64_t nq = mat.n * m->nqp;
+    ElemArgs A = elem_args(m, mat, false);
+    A.grad_u = mat.grad.p;
+    A.stress = nullptr;
+    require(A.grad_u && A.conn && A.X && A.u, "user material gradient pass without buffers");
+    if (m->etype == AKX_HEX8) {
+      const int64_t nb = (mat.n + HEX_STRICT_EPB - 1) / HEX_STRICT_EPB;
+      A.nblocks_padded = (nb + 7) / 8 * 8;
+      hipLaunchKernelGGL(k_hex8_strict<LAW_USER_GRAD>, dim3((unsigned)A.nblocks_padded),
+                         dim3(64), 0, m->stream, A);
+    } else {
+      hipLaunchKernelGGL((k_element_forces<ET_TET4, LAW_USER_GRAD, double, double>),
+                         dim3((unsigned)A.nblocks_padded), dim3(TPB), 0, m->stream, A);
+    }
+    HIPCHECK(hipGetLastError());
+    m->user_grad.resize(9 * nq);
+    m->user_stress.assign(9 * nq, 0.);
+    download_soa(m, mat.grad.p, m->user_grad.data(), nq, 9);
+    const int rc = m->stress_fn(m->stress_user, k, nq, m->user_grad.data(),
+                                m->user_stress.data());
+    if (rc != 0)
+      throw Error(AKX_ERR_INVALID,
+                  "computeStress of material " + std::to_string(k) + " failed");
+    upload_soa(m, m->user_stress.data(), mat.stress.p, nq, 9);
   }
 }
 
@@ -1195,7 +1265,7 @@ int akx_model_add_material(akx_model *m, int law, const double *params,
   return guard([&] {
     require(m != nullptr && m->etype >= 0, "set the mesh first");
     require(!m->initialized, "materials must be added before initFull");
-    require(law >= AKX_LAW_ELASTIC && law <= AKX_LAW_PLASTIC_LIH, "unknown law");
+    require(law >= AKX_LAW_ELASTIC && law <= AKX_LAW_USER_FD, "unknown law");
     require(params && nparams >= 3, "params = {rho, E, nu[, sigma_y, h]}");
     m->set_device();
     auto mat = std::make_unique<Material>();
@@ -1292,7 +1362,14 @@ int akx_model_init_full(akx_model *m) {
     require(m != nullptr && m->etype >= 0, "set the mesh first");
     require(!m->initialized, "initFull called twice");
     require(!m->mats.empty(), "no material");
+    require(!m->has_user() || m->precision == AKX_PRECISION_FP64,
+            "user materials run on the plain FP64 path");
     m->set_device();
+    for (auto &mp : m->mats)
+      if (mp->user()) { // the internals the plugin reads and writes
+        mp->grad.alloc(9 * mp->n * m->nqp);
+        mp->stress.alloc(9 * mp->n * m->nqp);
+      }
     int64_t nslots = 0;
     for (auto &mp : m->mats)
       nslots += mp->n * m->nnpe;
@@ -1412,6 +1489,17 @@ int akx_model_solve_step(akx_model *m, int64_t nb_steps) {
     if (nb_steps == 0)
       return;
     m->set_device();
+    if (m->has_user()) { // the plugin runs on the host between the passes
+      launch_predictor(m);
+      for (int64_t s = 0; s < nb_steps; ++s) {
+        user_stress_phase(m);
+        launch_element_pass(m, false);
+        flip_plastic_state(m);
+        launch_gather(m, true, s + 1 < nb_steps);
+      }
+      HIPCHECK(hipStreamSynchronize(m->stream));
+      return;
+    }
     if (m->pipe_parts > 1 && !m->pipe_planned)
       plan_pipeline(m);
     if (m->pipe_parts > 1 && m->pipe_ok && !m->timing) {
@@ -1451,6 +1539,7 @@ int akx_model_assemble_internal_forces(akx_model *m) {
     for (auto &mp : m->mats)
       if (mp->law == AKX_LAW_PLASTIC_LIH)
         mp->cur = 1 - mp->cur;
+    user_stress_phase(m);
     launch_element_pass(m, false);
     flip_plastic_state(m);
     launch_gather(m, false, false);
@@ -1678,6 +1767,8 @@ int akx_model_compute_all_stresses(akx_model *m) {
     m->set_device();
     for (auto &mp : m->mats) {
       int64_t nq = mp->n * m->nqp;
+      if (mp->user())
+        continue; // allocated at initFull, filled by the plugin
       if (mp->law != AKX_LAW_PLASTIC_LIH) {
         mp->grad.alloc(9 * nq);
         mp->stress.alloc(9 * nq);
@@ -1685,9 +1776,18 @@ int akx_model_compute_all_stresses(akx_model *m) {
         mp->cur = 1 - mp->cur; // recompute the current state from the previous
       }
     }
+    user_stress_phase(m);
     launch_element_pass(m, true);
     flip_plastic_state(m);
     HIPCHECK(hipStreamSynchronize(m->stream));
+  });
+}
+
+int akx_model_set_stress_callback(akx_model *m, akx_compute_stress_fn fn, void *user) {
+  return guard([&] {
+    require(m != nullptr, "null model");
+    m->stress_fn = fn;
+    m->stress_user = user;
   });
 }
 
@@ -1810,13 +1910,20 @@ int akx_model_get_energy(akx_model *m, const char *energy_id, int material_id,
     }
     const bool plastic_energy = id == "plastic";
     if (!plastic_energy) {
+      for (int k = 0; k < (int)m->mats.size(); ++k)
+        if ((material_id == -1 || material_id == k) && m->mats[k]->user())
+          throw Error(AKX_ERR_UNSUPPORTED,
+                      "the potential energy of a user material is not computed here");
       // the potential energy reads the stress of the current displacement
       bool need = false;
       for (auto &mp : m->mats)
         need = need || mp->law != AKX_LAW_PLASTIC_LIH;
       if (need) {
+        user_stress_phase(m);
         for (auto &mp : m->mats)
-          if (mp->law != AKX_LAW_PLASTIC_LIH) {
+          if (mp->user())
+            continue;
+          else if (mp->law != AKX_LAW_PLASTIC_LIH) {
             mp->grad.alloc(9 * mp->n * m->nqp);
             mp->stress.alloc(9 * mp->n * m->nqp);
           } else {

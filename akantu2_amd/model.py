@@ -12,6 +12,7 @@ getter call refreshes the mirror in place.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -19,6 +20,9 @@ from . import _native as N
 from .mesh import BoxMesh
 
 _explicit_lumped_mass = "explicit_lumped_mass"
+_not_ghost, _ghost = "_not_ghost", "_ghost"
+# parameter flags of Material.registerParamReal (accepted, not interpreted)
+_pat_parsable, _pat_readable, _pat_writable, _pat_modifiable = 1, 2, 4, 6
 
 _FIELDS = {
     "displacement": N.FIELD_DISPLACEMENT,
@@ -40,24 +44,76 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-class Material:
-    """Handle on one material of the model (Material's accessors)."""
+class MaterialFactory:
+    """The reference's MaterialFactory (material.hh:610-621; from Python
+    py_material.cc:237-240): ``registerAllocator(name, allocator)`` makes
+    ``model.addMaterial(name, ...)`` create ``allocator(dim, unused, model,
+    id)``, a subclass of :class:`Material` whose ``computeStress`` the
+    model calls at every stress evaluation."""
 
-    def __init__(self, model, mid, law, params, nb_elements):
+    _instance = None
+
+    def __init__(self):
+        self._allocators = {}
+
+    @staticmethod
+    def getInstance():
+        if MaterialFactory._instance is None:
+            MaterialFactory._instance = MaterialFactory()
+        return MaterialFactory._instance
+
+    def registerAllocator(self, name, allocator):
+        if name in _LAWS:
+            raise ValueError(f"{name!r} is a built-in law")
+        self._allocators[name] = allocator
+
+    def allocator(self, name):
+        return self._allocators.get(name)
+
+
+class Material:
+    """A material of the model (Material's accessors).
+
+    Subclass it to write a constitutive law, as in the reference
+    (py_material.cc:45-80): override ``computeStress(el_type, ghost_type)``
+    and fill ``self.getStress(el_type, ghost_type)`` ([nb_qp][9], sigma, or
+    the second Piola-Kirchhoff stress for finite deformation) from
+    ``self.getGradU(el_type, ghost_type)`` ([nb_qp][9], column-major
+    du_i/dX_j at i + 3 j).  Register the subclass with
+    ``MaterialFactory.getInstance().registerAllocator``.  The model runs such
+    a law on the plain FP64 path; its push-wave speed (stable time step) is
+    the elastic one of its rho, E, nu."""
+
+    def __init__(self, model, mid, law=None, params=None, nb_elements=None):
         self._model, self.id, self.law = model, mid, law
-        self.params = dict(params)
+        self.params = dict(params or {})
         self.nb_elements = nb_elements
+        self._live = None  # (grad_u, stress) views while computeStress runs
+        self._param_doc = {}
 
     def getParam(self, name):
         return self.params[name]
 
     get = getParam
 
+    # parameters of a user law (Material::registerParam / getReal)
+    def registerParamReal(self, name, flags=0, description=""):
+        self._param_doc[name] = description
+
+    def getReal(self, name):
+        return float(self.params[name])
+
+    def initMaterial(self):
+        """Called once at initFull (Material::initMaterial)."""
+
+    def computeStress(self, el_type, ghost_type=_not_ghost):
+        raise NotImplementedError("computeStress of a user material")
+
     def _internal(self, which, comps=9):
         m = self._model
         m._push()
         if self._needs_compute():
-            N.check(N.lib().akx_model_compute_all_stresses(m._h))
+            m._run(N.lib().akx_model_compute_all_stresses)
             m._stresses_fresh = True
         out = np.empty((self.nb_elements * m.nqp, comps))
         N.check(N.lib().akx_model_get_internal(m._h, self.id, which, _p(out)))
@@ -69,12 +125,17 @@ class Material:
             return False  # the history buffers hold the current state
         return not m._stresses_fresh
 
-    def getGradU(self):
+    def getGradU(self, el_type=None, ghost_type=_not_ghost):
+        if self._live is not None:
+            return self._live[0]
         return self._internal(N.INTERNAL_GRAD_U)
 
-    def getStress(self):
+    def getStress(self, el_type=None, ghost_type=_not_ghost):
         """sigma for small strain laws, the second Piola-Kirchhoff stress for
-        finite deformation (as Material::getStress / piola_kirchhoff_2)."""
+        finite deformation (as Material::getStress / piola_kirchhoff_2).
+        Inside computeStress: the array to fill."""
+        if self._live is not None:
+            return self._live[1]
         return self._internal(N.INTERNAL_STRESS)
 
     def getInelasticStrain(self):
@@ -112,6 +173,8 @@ class SolidMechanicsModel:
         self._initialized = False
         self._stresses_fresh = False
         self.time_step = 0.0
+        self._cb = None
+        self._cb_error = None
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -125,10 +188,18 @@ class SolidMechanicsModel:
     # ------------------------------------------------------------ materials
     def addMaterial(self, law="elastic", rho=1.0, E=1.0, nu=0.3, sigma_y=0.0, h=0.0,
                     finite_deformation=False, elements=None, E_el=None, nu_el=None,
-                    sigma_y_el=None, h_el=None):
+                    sigma_y_el=None, h_el=None, **user_params):
         """Register a material (the ``material <law> [...]`` section of the
         reference's input file).  ``elements`` is the element filter in
-        assembly order (default: all elements)."""
+        assembly order (default: all elements).  A name registered with
+        ``MaterialFactory`` creates that user material (``user_params`` are
+        its further parameters, read with ``getReal``)."""
+        alloc = MaterialFactory.getInstance().allocator(law) if isinstance(law, str) else None
+        if alloc is not None:
+            return self._add_user_material(alloc, rho, E, nu, finite_deformation,
+                                           elements, user_params)
+        if user_params:
+            raise TypeError(f"unknown parameters {sorted(user_params)} for {law!r}")
         if law == "elastic" and finite_deformation:
             law = "elastic_fd"
         lid = _LAWS[law] if isinstance(law, str) else int(law)
@@ -153,6 +224,62 @@ class SolidMechanicsModel:
         self.materials.append(mat)
         return mat
 
+    def _add_user_material(self, alloc, rho, E, nu, finite_deformation, elements, extra):
+        lid = N.LAW_USER_FD if finite_deformation else N.LAW_USER
+        params = np.array([rho, E, nu], np.float64)
+        mid = C.c_int(-1)
+        filt = None if elements is None else np.ascontiguousarray(elements, np.int32)
+        n = self.mesh.nb_elements if filt is None else filt.size
+        N.check(N.lib().akx_model_add_material(
+            self._h, lid, params.ctypes.data_as(C.POINTER(C.c_double)), 3,
+            _p(filt), 0 if filt is None else filt.size, C.byref(mid)))
+        mat = alloc(3, None, self, mid.value)
+        if not isinstance(mat, Material):
+            raise TypeError("a material allocator must return an akantu2_amd.Material")
+        mat._model, mat.id, mat.law, mat.nb_elements = self, mid.value, lid, n
+        mat.params.update(dict(rho=rho, E=E, nu=nu, finite_deformation=finite_deformation),
+                          **extra)
+        mat.element_ids = None if filt is None else filt.copy()
+        self.materials.append(mat)
+        self._install_stress_callback()
+        return mat
+
+    def _install_stress_callback(self):
+        if self._cb is not None:
+            return
+        ref = weakref.ref(self)
+        el_type = "_tetrahedron_4" if self.mesh.etype == 0 else "_hexahedron_8"
+
+        def compute_stress(user, mid, nq, grad_p, stress_p):
+            model = ref()
+            if model is None:
+                return 1
+            mat = model.materials[mid]
+            try:
+                grad = np.ctypeslib.as_array(grad_p, shape=(nq, 9))
+                stress = np.ctypeslib.as_array(stress_p, shape=(nq, 9))
+                mat._live = (grad, stress)
+                mat.computeStress(el_type, _not_ghost)
+                return 0
+            except BaseException as e:  # re-raised by the call that ran the law
+                model._cb_error = e
+                return 1
+            finally:
+                mat._live = None
+
+        self._cb = N.COMPUTE_STRESS_FN(compute_stress)
+        N.check(N.lib().akx_model_set_stress_callback(self._h, self._cb, None))
+
+    def _run(self, fn, *args):
+        """A library call that may run user laws: an exception raised in a
+        computeStress propagates from here."""
+        self._cb_error = None
+        status = fn(self._h, *args)
+        if self._cb_error is not None:
+            e, self._cb_error = self._cb_error, None
+            raise e
+        N.check(status)
+
     def getMaterial(self, i):
         return self.materials[i]
 
@@ -168,6 +295,9 @@ class SolidMechanicsModel:
         if not self.materials:
             self.addMaterial(**(material or {}))
         N.check(N.lib().akx_model_init_full(self._h))
+        for mat in self.materials:
+            if mat.law in (N.LAW_USER, N.LAW_USER_FD):
+                mat.initMaterial()
         self._initialized = True
 
     def getStableTimeStep(self) -> float:
@@ -187,12 +317,12 @@ class SolidMechanicsModel:
     def solveStep(self, nb_steps: int = 1):
         """``nb_steps`` consecutive ModelSolver::solveStep() calls."""
         self._push()
-        N.check(N.lib().akx_model_solve_step(self._h, int(nb_steps)))
+        self._run(N.lib().akx_model_solve_step, int(nb_steps))
         self._invalidate()
 
     def assembleInternalForces(self):
         self._push()
-        N.check(N.lib().akx_model_assemble_internal_forces(self._h))
+        self._run(N.lib().akx_model_assemble_internal_forces)
         self._invalidate()
 
     def synchronize(self):
@@ -292,8 +422,8 @@ class SolidMechanicsModel:
     def _energy(self, energy_id, material_id):
         self._push()
         out = C.c_double()
-        N.check(N.lib().akx_model_get_energy(self._h, energy_id.encode(), int(material_id),
-                                             C.byref(out)))
+        self._run(N.lib().akx_model_get_energy, energy_id.encode(), int(material_id),
+                  C.byref(out))
         if energy_id == "potential":
             self._stresses_fresh = True
         return out.value

@@ -60,6 +60,14 @@ extern "C" {
 #define AKX_LAW_ELASTIC_FD 1  /* "elastic" with finite_deformation = true */
 #define AKX_LAW_NEOHOOKEAN 2
 #define AKX_LAW_PLASTIC_LIH 3
+/* A law supplied by the caller: the Material::computeStress plugin
+ * (material.hh:78-81, a subclass registered with
+ * MaterialFactory::registerAllocator, material.hh:610-621; from Python
+ * py_material.cc:45-80,237-240).  params = {rho, E, nu}: E and nu give the
+ * push-wave speed of the stable time step and rho the lumped mass.  The plain
+ * FP64 path runs it (AKX_PRECISION_FP64). */
+#define AKX_LAW_USER 4     /* small strain: the callback writes sigma */
+#define AKX_LAW_USER_FD 5  /* finite deformation: the callback writes PK2 */
 
 /* Arithmetic of the element pass */
 #define AKX_PRECISION_FP64 0 /* plain path (default): FP64 in the reference's
@@ -127,6 +135,19 @@ int akx_material_law_by_name(const char *name, int finite_deformation,
                              int *law);
 
 int akx_model_set_precision(akx_model *m, int precision);
+
+/* computeStress of the AKX_LAW_USER / AKX_LAW_USER_FD materials.  Called on
+ * the calling thread, once per user material and per stress evaluation
+ * (every solveStep, assembleInternalForces, computeAllStresses), with the
+ * material's Gauss-point gradients grad_u [nb_qp][9] (column-major,
+ * du_i/dX_j at i + 3 j) and its stress [nb_qp][9] to fill (sigma, or PK2
+ * for AKX_LAW_USER_FD), as Material::getGradU / getStress give them to
+ * computeStress.  Gauss points are in the material's element order, nqp
+ * per element.  A non-zero return aborts the call with AKX_ERR_INVALID.
+ * The arrays are valid during the call only. */
+typedef int (*akx_compute_stress_fn)(void *user, int material_id, int64_t nb_qp,
+                                     const double *grad_u, double *stress);
+int akx_model_set_stress_callback(akx_model *m, akx_compute_stress_fn fn, void *user);
 
 /* initFull(_analysis_method = _explicit_lumped_mass): checks the jacobians
  * (AKX_ERR_JACOBIAN, like IntegratorGauss::checkJacobians), builds the

@@ -22,6 +22,8 @@
 #include <array>
 #include <cstdint>
 #include <deque>
+#include <exception>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -79,6 +81,18 @@ struct MaterialParameters {
 
 class SolidMechanicsModel;
 
+// A constitutive law written by the caller: the reference's Material
+// subclass overriding computeStress (material.hh:78-81).  grad_u and stress
+// are the material's Gauss-point arrays [nb_qp][9] (column-major, du_i/dX_j
+// at i + 3 j), as Material::getGradU / getStress; stress is sigma, or the
+// second Piola-Kirchhoff stress for a finite-deformation law.  It runs on
+// the host at every stress evaluation, beside the device passes.
+class ConstitutiveLaw {
+public:
+  virtual ~ConstitutiveLaw() = default;
+  virtual void computeStress(const Array<double> &grad_u, Array<double> &stress) = 0;
+};
+
 // Handle on one material (Material's accessors).
 class Material {
 public:
@@ -126,6 +140,22 @@ public:
                            filter.empty() ? nb_elements : (std::int64_t)filter.size());
     return materials.back();
   }
+  // a material whose computeStress is `law` (kept by reference; the caller
+  // keeps it alive); rho, E, nu give the lumped mass and the push-wave speed
+  Material &addMaterial(ConstitutiveLaw &law, const MaterialParameters &p,
+                        bool finite_deformation = false,
+                        const std::vector<std::int32_t> &filter = {}) {
+    int id = -1;
+    double params[3] = {p.rho, p.E, p.nu};
+    check(akx_model_add_material(h, finite_deformation ? AKX_LAW_USER_FD : AKX_LAW_USER,
+                                 params, 3, filter.empty() ? nullptr : filter.data(),
+                                 (std::int64_t)filter.size(), &id));
+    user_laws[id] = &law;
+    check(akx_model_set_stress_callback(h, &SolidMechanicsModel::stress_trampoline, this));
+    materials.emplace_back(*this, id,
+                           filter.empty() ? nb_elements : (std::int64_t)filter.size());
+    return materials.back();
+  }
   Material &getMaterial(int i) { return materials.at(i); }
 
   void initFull(AnalysisMethod = _explicit_lumped_mass) {
@@ -142,12 +172,12 @@ public:
   // ModelSolver::solveStep (nb_steps consecutive calls)
   void solveStep(std::int64_t nb_steps = 1) {
     push();
-    check(akx_model_solve_step(h, nb_steps));
+    run(akx_model_solve_step(h, nb_steps));
     stale = true;
   }
   void assembleInternalForces() {
     push();
-    check(akx_model_assemble_internal_forces(h));
+    run(akx_model_assemble_internal_forces(h));
     stale = true;
   }
 
@@ -199,7 +229,34 @@ private:
     }
   }
 
+  // computeStress of the user materials; an exception a law throws is
+  // rethrown from the call that ran it
+  static int stress_trampoline(void *self, int material_id, std::int64_t nb_qp,
+                               const double *grad_u, double *stress) {
+    auto *m = static_cast<SolidMechanicsModel *>(self);
+    try {
+      Array<double> g(nb_qp, 9), s(nb_qp, 9);
+      std::copy(grad_u, grad_u + 9 * nb_qp, g.data());
+      m->user_laws.at(material_id)->computeStress(g, s);
+      std::copy(s.data(), s.data() + 9 * nb_qp, stress);
+      return 0;
+    } catch (...) {
+      m->law_error = std::current_exception();
+      return 1;
+    }
+  }
+  void run(int status) {
+    if (law_error) {
+      std::exception_ptr e = law_error;
+      law_error = nullptr;
+      std::rethrow_exception(e);
+    }
+    check(status);
+  }
+
   akx_model *h = nullptr;
+  std::map<int, ConstitutiveLaw *> user_laws;
+  std::exception_ptr law_error;
   std::int64_t nb_nodes;
   int nqp;
   std::int64_t nb_elements;
@@ -273,7 +330,7 @@ private:
 
 inline Array<double> Material::getGradU() const {
   model->push();
-  check(akx_model_compute_all_stresses(model->h));
+  model->run(akx_model_compute_all_stresses(model->h));
   Array<double> out(n * model->nqp, 9);
   check(akx_model_get_internal(model->h, id, AKX_INTERNAL_GRAD_U, out.data()));
   return out;
@@ -281,7 +338,7 @@ inline Array<double> Material::getGradU() const {
 
 inline Array<double> Material::getStress() const {
   model->push();
-  check(akx_model_compute_all_stresses(model->h));
+  model->run(akx_model_compute_all_stresses(model->h));
   Array<double> out(n * model->nqp, 9);
   check(akx_model_get_internal(model->h, id, AKX_INTERNAL_STRESS, out.data()));
   return out;

@@ -6,6 +6,10 @@
 //
 //   drive_explicit <dir> <etype 0|1> <law name> <finite_deformation 0|1> <steps> [batch]
 //
+// <law name> "local_elastic" is a ConstitutiveLaw written here (the
+// reference's Material subclass overriding computeStress), the small-strain
+// elastic law in the reference's operation order.
+//
 // With "batch" the same simulation runs as one instance of an FP64 Batch
 // (include/akantu_mi355x.hpp), beside a second instance (the same mesh at
 // rest) that must not change its result.
@@ -23,6 +27,25 @@
 #include <vector>
 
 using namespace akantu_mi355x;
+
+// sigma = lambda tr(grad u) I + 2 mu sym(grad u) (material_elastic_inline_impl.hh:33-49)
+class LocalElastic : public ConstitutiveLaw {
+public:
+  LocalElastic(double E, double nu)
+      : lambda(nu * E / ((1. + nu) * (1. - 2. * nu))), mu(E / (2. * (1. + nu))) {}
+  void computeStress(const Array<double> &grad_u, Array<double> &stress) override {
+    for (std::int64_t q = 0; q < grad_u.size(); ++q) {
+      auto g = [&](int i, int j) { return grad_u(q, i + 3 * j); };
+      const double tr = g(0, 0) + g(1, 1) + g(2, 2), two_mu = 2. * mu;
+      for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+          const double eps = .5 * (g(j, i) + g(i, j));
+          stress(q, i + 3 * j) = i == j ? two_mu * eps + lambda * tr : two_mu * eps;
+        }
+    }
+  }
+  double lambda, mu;
+};
 
 template <class T> static std::vector<T> load(const std::string &path) {
   std::ifstream f(path, std::ios::binary | std::ios::ate);
@@ -91,7 +114,11 @@ int main(int argc, char **argv) {
       return 0;
     }
     SolidMechanicsModel model(mesh);
-    model.addMaterial(law, {P[0], P[1], P[2], P[3], P[4]}, fd);
+    LocalElastic local(P[1], P[2]);
+    if (law == "local_elastic")
+      model.addMaterial(local, {P[0], P[1], P[2]}, fd);
+    else
+      model.addMaterial(law, {P[0], P[1], P[2], P[3], P[4]}, fd);
     model.initFull(_explicit_lumped_mass);
     auto &blocked = model.getBlockedDOFs();
     auto &velocity = model.getVelocity();
