@@ -63,6 +63,12 @@ def _declare(lib):
         "akx_material_law_by_name": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
         "akx_model_set_precision": (C.c_int, [_vp, C.c_int]),
         "akx_model_set_stress_callback": (C.c_int, [_vp, COMPUTE_STRESS_FN, _vp]),
+        "akx_fe_gradient_on_integration_points": (C.c_int, [_vp, _vp, C.c_int, _vp, _i64,
+                                                            _vp]),
+        "akx_fe_compute_btd": (C.c_int, [_vp, _vp, C.c_int, _vp, _i64, _vp]),
+        "akx_fe_integrate": (C.c_int, [_vp, _vp, C.c_int, _vp, _i64, _vp]),
+        "akx_dof_assemble_elemental_array": (C.c_int, [_vp, _vp, C.c_int, _vp, _i64,
+                                                       C.c_double, _vp]),
         "akx_model_init_full": (C.c_int, [_vp]),
         "akx_model_get_stable_time_step": (C.c_int, [_vp, _dp]),
         "akx_model_set_time_step": (C.c_int, [_vp, C.c_double]),

@@ -1975,6 +1975,269 @@ int akx_model_get_internal(akx_model *m, int material_id, int internal,
   });
 }
 
+} // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// FEEngine / DOFManager calls on arbitrary fields, in the operation order of
+// the element passes (the reference's):
+//   gradientOnIntegrationPoints  g(k, j) = sum_n u(k, n) B(j, n)
+//                                (shape_functions_inline_impl.hh:296-310)
+//   computeBtD                   BtD(k, n) = sum_j D(k, j) B(j, n)
+//                                (shape_lagrange_inline_impl.hh:373-410)
+//   integrate                    I(k) = sum_q f(k, q) wJ_q
+//                                (integrator_gauss_inline_impl.hh:401-441)
+//   assembleElementalArrayLocalArray
+//                                out(conn(e, n), k) += s * r_e(k + d n), the
+//                                elements in filter order
+//                                (dof_manager.cc:65-120)
+// Per-point arrays are [point][comps] with the points of element i at rows
+// i * nqp ..; matrices per point are column-major (rows k, columns j).
+// ---------------------------------------------------------------------------
+struct FeArgs {
+  int64_t n;             // elements of the call (filter size)
+  const int32_t *filter; // null: element i is i
+  const int32_t *conn;
+  const double *X; // [3][nn]
+  int64_t nn;
+  const double *in;
+  double *out;
+  int d; // rows of the per-point matrix (degrees of freedom)
+};
+
+template <int ET>
+__device__ __forceinline__ void fe_point(const FeArgs &A, int64_t i, int q, int32_t (&nodes)[Elem<ET>::nnpe],
+                                         double (&b)[Elem<ET>::nnpe][3], double &wJ) {
+  constexpr int NN = Elem<ET>::nnpe;
+  const int64_t e = A.filter ? (int64_t)A.filter[i] : i;
+  double X[NN][3];
+#pragma unroll
+  for (int n = 0; n < NN; ++n) {
+    nodes[n] = A.conn[e * NN + n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      X[n][c] = A.X[(int64_t)c * A.nn + nodes[n]];
+  }
+  double Ji[3][3], detJ;
+  wJ = Shape<ET, double>::jacobian_at(X, q, Ji, detJ);
+#pragma unroll
+  for (int n = 0; n < NN; ++n)
+    Shape<ET, double>::bcol(Ji, q, n, b[n]);
+}
+
+// one thread per (element, Gauss point); in = field [nn][d]
+template <int ET> __global__ void k_fe_gradient(FeArgs A) {
+  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= A.n * NQ)
+    return;
+  int32_t nodes[NN];
+  double b[NN][3], wJ;
+  fe_point<ET>(A, t / NQ, (int)(t % NQ), nodes, b, wJ);
+  double *o = A.out + t * 3 * A.d;
+  for (int k = 0; k < A.d; ++k)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = A.in[(int64_t)nodes[0] * A.d + k] * b[0][j];
+#pragma unroll
+      for (int n = 1; n < NN; ++n)
+        s = s + A.in[(int64_t)nodes[n] * A.d + k] * b[n][j];
+      o[k + A.d * j] = s;
+    }
+}
+
+// one thread per (element, Gauss point); in = D [point][d * 3]
+template <int ET> __global__ void k_fe_btd(FeArgs A) {
+  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= A.n * NQ)
+    return;
+  int32_t nodes[NN];
+  double b[NN][3], wJ;
+  fe_point<ET>(A, t / NQ, (int)(t % NQ), nodes, b, wJ);
+  const double *D = A.in + t * 3 * A.d;
+  double *o = A.out + t * NN * A.d;
+  for (int k = 0; k < A.d; ++k)
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+      double s = D[k] * b[n][0];
+      s += D[k + A.d] * b[n][1];
+      s += D[k + 2 * A.d] * b[n][2];
+      o[k + A.d * n] = s;
+    }
+}
+
+// one thread per element; in = f [point][d]
+template <int ET> __global__ void k_fe_integrate(FeArgs A) {
+  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n)
+    return;
+  double wJ[NQ];
+  for (int q = 0; q < NQ; ++q) {
+    int32_t nodes[NN];
+    double b[NN][3];
+    fe_point<ET>(A, i, q, nodes, b, wJ[q]);
+  }
+  for (int k = 0; k < A.d; ++k) {
+    double s = A.in[(i * NQ) * A.d + k] * wJ[0];
+    for (int q = 1; q < NQ; ++q)
+      s = s + A.in[(i * NQ + q) * A.d + k] * wJ[q];
+    A.out[i * A.d + k] = s;
+  }
+}
+
+// one thread per node: its contributions (offsets into the elemental
+// array, in filter order) added in that order
+__global__ void k_dof_assemble(int64_t nn, const int64_t *rowptr, const int64_t *entry,
+                               const double *elem, int d, double scale, double *out) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nn)
+    return;
+  for (int k = 0; k < d; ++k) {
+    double acc = out[node * d + k];
+    for (int64_t r = rowptr[node]; r < rowptr[node + 1]; ++r)
+      acc += scale * elem[entry[r] + k];
+    out[node * d + k] = acc;
+  }
+}
+
+struct FeCall {
+  akx_model *m;
+  FeArgs A{};
+  DevBuf<int32_t> filter;
+  DevBuf<double> in, out;
+  FeCall(akx_model *mm, const int32_t *filt, int64_t nb_filter, int d) : m(mm) {
+    check_ready(m);
+    require(d >= 1 && d <= 64, "nb_degree_of_freedom must be in [1, 64]");
+    m->set_device();
+    A.n = filt ? nb_filter : m->nel;
+    require(A.n >= 0, "negative filter size");
+    if (filt && nb_filter > 0) {
+      for (int64_t i = 0; i < nb_filter; ++i)
+        require(filt[i] >= 0 && filt[i] < m->nel, "filter refers to a missing element");
+      filter.alloc(nb_filter);
+      HIPCHECK(hipMemcpy(filter.p, filt, nb_filter * 4, hipMemcpyHostToDevice));
+    }
+    A.filter = filter.p;
+    A.conn = m->conn.p;
+    A.X = m->X.p;
+    A.nn = m->nn;
+    A.d = d;
+  }
+  void put(const double *host, int64_t count) {
+    in.alloc(count > 0 ? count : 1);
+    if (count > 0)
+      HIPCHECK(hipMemcpy(in.p, host, count * 8, hipMemcpyHostToDevice));
+    A.in = in.p;
+  }
+  void make_out(int64_t count) {
+    out.alloc(count > 0 ? count : 1);
+    A.out = out.p;
+  }
+  void get(double *host, int64_t count) {
+    HIPCHECK(hipStreamSynchronize(m->stream));
+    if (count > 0)
+      HIPCHECK(hipMemcpy(host, out.p, count * 8, hipMemcpyDeviceToHost));
+  }
+};
+
+template <template <int> class K> void fe_launch(akx_model *m, const FeArgs &A, int64_t threads) {
+  if (threads == 0)
+    return;
+  const dim3 grid(grid_for(threads)), block(TPB);
+  if (m->etype == AKX_HEX8)
+    hipLaunchKernelGGL(K<ET_HEX8>::fn, grid, block, 0, m->stream, A);
+  else
+    hipLaunchKernelGGL(K<ET_TET4>::fn, grid, block, 0, m->stream, A);
+  HIPCHECK(hipGetLastError());
+}
+
+template <int ET> struct KGrad { static constexpr auto fn = k_fe_gradient<ET>; };
+template <int ET> struct KBtd { static constexpr auto fn = k_fe_btd<ET>; };
+template <int ET> struct KInt { static constexpr auto fn = k_fe_integrate<ET>; };
+
+} // namespace
+
+extern "C" {
+
+int akx_fe_gradient_on_integration_points(akx_model *m, const double *field, int nb_dof,
+                                          const int32_t *filter, int64_t nb_filter,
+                                          double *out) {
+  return guard([&] {
+    require(field && out, "null array");
+    FeCall c(m, filter, nb_filter, nb_dof);
+    const int64_t npt = c.A.n * m->nqp;
+    c.put(field, m->nn * nb_dof);
+    c.make_out(npt * 3 * nb_dof);
+    fe_launch<KGrad>(m, c.A, npt);
+    c.get(out, npt * 3 * nb_dof);
+  });
+}
+
+int akx_fe_compute_btd(akx_model *m, const double *Ds, int nb_comp, const int32_t *filter,
+                       int64_t nb_filter, double *BtDs) {
+  return guard([&] {
+    require(Ds && BtDs, "null array");
+    require(nb_comp % 3 == 0, "D must have 3 columns per point (nb_component = 3 d)");
+    FeCall c(m, filter, nb_filter, nb_comp / 3);
+    const int64_t npt = c.A.n * m->nqp;
+    c.put(Ds, npt * nb_comp);
+    c.make_out(npt * m->nnpe * c.A.d);
+    fe_launch<KBtd>(m, c.A, npt);
+    c.get(BtDs, npt * m->nnpe * c.A.d);
+  });
+}
+
+int akx_fe_integrate(akx_model *m, const double *f, int nb_dof, const int32_t *filter,
+                     int64_t nb_filter, double *intf) {
+  return guard([&] {
+    require(f && intf, "null array");
+    FeCall c(m, filter, nb_filter, nb_dof);
+    c.put(f, c.A.n * m->nqp * nb_dof);
+    c.make_out(c.A.n * nb_dof);
+    fe_launch<KInt>(m, c.A, c.A.n);
+    c.get(intf, c.A.n * nb_dof);
+  });
+}
+
+int akx_dof_assemble_elemental_array(akx_model *m, const double *elementary, int nb_dof,
+                                     const int32_t *filter, int64_t nb_filter,
+                                     double scale, double *assembled) {
+  return guard([&] {
+    require(elementary && assembled, "null array");
+    FeCall c(m, filter, nb_filter, nb_dof);
+    const int nnpe = m->nnpe;
+    // node -> contributions, in filter order (a counting sort)
+    std::vector<int64_t> rowptr(m->nn + 1, 0), entry(c.A.n * nnpe);
+    auto elem_of = [&](int64_t i) { return filter ? (int64_t)filter[i] : i; };
+    for (int64_t i = 0; i < c.A.n; ++i)
+      for (int a = 0; a < nnpe; ++a)
+        rowptr[m->conn_host[elem_of(i) * nnpe + a] + 1]++;
+    for (int64_t k = 0; k < m->nn; ++k)
+      rowptr[k + 1] += rowptr[k];
+    std::vector<int64_t> fill(rowptr.begin(), rowptr.end() - 1);
+    for (int64_t i = 0; i < c.A.n; ++i)
+      for (int a = 0; a < nnpe; ++a)
+        entry[fill[m->conn_host[elem_of(i) * nnpe + a]]++] = (i * nnpe + a) * nb_dof;
+    DevBuf<int64_t> drow, dent;
+    drow.alloc(rowptr.size());
+    dent.alloc(entry.empty() ? 1 : entry.size());
+    HIPCHECK(hipMemcpy(drow.p, rowptr.data(), rowptr.size() * 8, hipMemcpyHostToDevice));
+    if (!entry.empty())
+      HIPCHECK(hipMemcpy(dent.p, entry.data(), entry.size() * 8, hipMemcpyHostToDevice));
+    c.put(elementary, c.A.n * nnpe * nb_dof);
+    c.make_out(m->nn * nb_dof);
+    HIPCHECK(hipMemcpy(c.out.p, assembled, m->nn * nb_dof * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_dof_assemble, dim3(grid_for(m->nn)), dim3(TPB), 0, m->stream, m->nn,
+                       (const int64_t *)drow.p, (const int64_t *)dent.p, (const double *)c.in.p,
+                       nb_dof, scale, c.out.p);
+    HIPCHECK(hipGetLastError());
+    c.get(assembled, m->nn * nb_dof);
+  });
+}
+
 int akx_calibration_copy(int device, int64_t bytes, int repeats, double *ms) {
   return guard([&] {
     require(bytes >= 32 && repeats > 0 && ms, "bad calibration arguments");

@@ -149,6 +149,77 @@ class Material:
         return self._model._energy(energy_id, self.id)
 
 
+class FEEngine:
+    """The model's FEEngine calls on caller fields (fe_engine.hh:85-158),
+    computed on the device in the reference's operation order.  Per-point
+    arrays have the points of the i-th filtered element at rows i * nqp ..,
+    matrices per point column-major; ``filter`` lists elements (None: all)."""
+
+    def __init__(self, model):
+        self._m = model
+
+    def _filter(self, filter_elements):
+        if filter_elements is None:
+            return None, self._m.mesh.nb_elements
+        f = np.ascontiguousarray(filter_elements, np.int32)
+        return f, f.size
+
+    def gradientOnIntegrationPoints(self, u, nb_degree_of_freedom=None, el_type=None,
+                                    ghost_type=_not_ghost, filter_elements=None):
+        """[nb_nodes][d] -> [n * nqp][d * 3], du_k/dX_j at k + d j."""
+        u = np.ascontiguousarray(u, np.float64).reshape(self._m.nb_nodes, -1)
+        d = u.shape[1] if nb_degree_of_freedom is None else int(nb_degree_of_freedom)
+        f, n = self._filter(filter_elements)
+        out = np.empty((n * self._m.nqp, 3 * d))
+        N.check(N.lib().akx_fe_gradient_on_integration_points(self._m._h, _p(u), d, _p(f),
+                                                              n, _p(out)))
+        return out
+
+    def computeBtD(self, Ds, el_type=None, ghost_type=_not_ghost, filter_elements=None):
+        """[n * nqp][3 d] (d x 3 per point) -> [n * nqp][d * nnpe], D B."""
+        f, n = self._filter(filter_elements)
+        Ds = np.ascontiguousarray(Ds, np.float64).reshape(n * self._m.nqp, -1)
+        nnpe = self._m.mesh.conn.shape[1]
+        out = np.empty((Ds.shape[0], Ds.shape[1] // 3 * nnpe))
+        N.check(N.lib().akx_fe_compute_btd(self._m._h, _p(Ds), Ds.shape[1], _p(f), n,
+                                           _p(out)))
+        return out
+
+    def integrate(self, f_q, nb_degree_of_freedom=None, el_type=None, ghost_type=_not_ghost,
+                  filter_elements=None):
+        """[n * nqp][d] -> [n][d], sum over the Gauss points of f w detJ."""
+        f, n = self._filter(filter_elements)
+        f_q = np.ascontiguousarray(f_q, np.float64).reshape(n * self._m.nqp, -1)
+        d = f_q.shape[1] if nb_degree_of_freedom is None else int(nb_degree_of_freedom)
+        out = np.empty((n, d))
+        N.check(N.lib().akx_fe_integrate(self._m._h, _p(f_q), d, _p(f), n, _p(out)))
+        return out
+
+
+class DOFManager:
+    """DOFManager::assembleElementalArrayLocalArray (dof_manager.cc:65-120)
+    on the device: ``out`` [nb_nodes][d] += scale * elementary [n][d * nnpe],
+    each node's contributions in filter order."""
+
+    def __init__(self, model):
+        self._m = model
+
+    def assembleElementalArrayLocalArray(self, elementary, out, el_type=None,
+                                         ghost_type=_not_ghost, scale_factor=1.0,
+                                         filter_elements=None):
+        if not (isinstance(out, np.ndarray) and out.dtype == np.float64
+                and out.flags.c_contiguous):
+            raise TypeError("out must be a C-contiguous float64 array (updated in place)")
+        f = None if filter_elements is None else np.ascontiguousarray(filter_elements,
+                                                                      np.int32)
+        n = self._m.mesh.nb_elements if f is None else f.size
+        elementary = np.ascontiguousarray(elementary, np.float64).reshape(n, -1)
+        d = out.size // self._m.nb_nodes
+        N.check(N.lib().akx_dof_assemble_elemental_array(
+            self._m._h, _p(elementary), d, _p(f), n, float(scale_factor), _p(out)))
+        return out
+
+
 class SolidMechanicsModel:
     """One mesh of one element type on one gfx950 device."""
 
@@ -282,6 +353,12 @@ class SolidMechanicsModel:
 
     def getMaterial(self, i):
         return self.materials[i]
+
+    def getFEEngine(self, name=None):
+        return FEEngine(self)
+
+    def getDOFManager(self):
+        return DOFManager(self)
 
     def setPrecision(self, precision):
         p = {"fp64": N.PRECISION_FP64, "fp32": N.PRECISION_FP32,

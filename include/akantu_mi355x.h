@@ -149,6 +149,38 @@ typedef int (*akx_compute_stress_fn)(void *user, int material_id, int64_t nb_qp,
                                      const double *grad_u, double *stress);
 int akx_model_set_stress_callback(akx_model *m, akx_compute_stress_fn fn, void *user);
 
+/* FEEngine / DOFManager calls on caller fields, run on the device in the
+ * reference's operation order (the one of the element passes).  filter
+ * lists elements in order (NULL: all, ascending).  Per-point arrays are
+ * [nb_filter * nqp][comps], points of the i-th filtered element at rows
+ * i * nqp ..; a matrix per point is column-major.
+ *
+ * FEEngine::gradientOnIntegrationPoints(u, nablauq, nb_dof, type, ghost,
+ * filter) (fe_engine.hh:136-139, shape_functions_inline_impl.hh:263-315):
+ * field [nb_nodes][nb_dof] -> out [nb_filter * nqp][nb_dof * 3], the
+ * nb_dof x 3 gradient du_k/dX_j at k + nb_dof j. */
+int akx_fe_gradient_on_integration_points(akx_model *m, const double *field, int nb_dof,
+                                          const int32_t *filter, int64_t nb_filter,
+                                          double *out);
+/* FEEngine::computeBtD(Ds, BtDs, type, ghost, filter) (fe_engine.hh:155-158,
+ * shape_lagrange_inline_impl.hh:373-410): Ds [nb_filter * nqp][3 d] (a d x 3
+ * matrix per point) -> BtDs [nb_filter * nqp][d * nnpe], D B at k + d n. */
+int akx_fe_compute_btd(akx_model *m, const double *Ds, int nb_comp, const int32_t *filter,
+                       int64_t nb_filter, double *BtDs);
+/* FEEngine::integrate(f, intf, nb_dof, type, ghost, filter) (fe_engine.hh:85-94,
+ * integrator_gauss_inline_impl.hh:401-441): f [nb_filter * nqp][nb_dof] ->
+ * intf [nb_filter][nb_dof], sum over the Gauss points of f w detJ. */
+int akx_fe_integrate(akx_model *m, const double *f, int nb_dof, const int32_t *filter,
+                     int64_t nb_filter, double *intf);
+/* DOFManager::assembleElementalArrayLocalArray(elementary, assembled, type,
+ * ghost, scale, filter) (dof_manager.hh:85, dof_manager.cc:65-120):
+ * assembled [nb_nodes][nb_dof] += scale * elementary [nb_filter][nb_dof *
+ * nnpe] (node-major, dof k of local node n at k + nb_dof n), each node's
+ * contributions added in filter order. */
+int akx_dof_assemble_elemental_array(akx_model *m, const double *elementary, int nb_dof,
+                                     const int32_t *filter, int64_t nb_filter,
+                                     double scale, double *assembled);
+
 /* initFull(_analysis_method = _explicit_lumped_mass): checks the jacobians
  * (AKX_ERR_JACOBIAN, like IntegratorGauss::checkJacobians), builds the
  * node->element gather table and assembles the lumped mass. */
