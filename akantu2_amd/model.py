@@ -8,6 +8,11 @@ arrays returned by the getters are host mirrors of device-resident fields:
 edits made to them are uploaded before the next ``solveStep`` (the reference
 lets users write through ``getDisplacement()``), and after a step the next
 getter call refreshes the mirror in place.
+
+User constitutive laws subclass :class:`Material` and register with
+:class:`MaterialFactory`, as in the reference; ``getFEEngine()`` and
+``getDOFManager()`` give the FE-engine gradient / BtD / integrate and the
+elemental assembly on caller fields.
 """
 from __future__ import annotations
 
