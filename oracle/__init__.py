@@ -181,6 +181,10 @@ class OracleModel:
     def set_time_step(self, dt: float):
         self._s.dt = float(dt)
 
+    @property
+    def dt(self) -> float:
+        return self._s.dt
+
     def internal_forces(self):
         lib().orc_internal_forces(C.byref(self._s))
 

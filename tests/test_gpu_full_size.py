@@ -1,0 +1,124 @@
+"""Parity at the full workload sizes, a few steps each, the oracle running on
+the host's cores (OpenMP; its results do not depend on the thread count).
+
+- 200 x 100 x 100 jittered Hex8, neo-Hookean (2,000,000 elements,
+  16,000,000 Gauss points): the plain FP64 path against the oracle after 3
+  steps (bar 1e-12) and the FP32 form within 1e-5;
+- 1000 x 100 x 100 x 6 jittered Tet4 with per-element E, nu (60,000,000
+  elements): the plain FP64 path bit for bit after 2 steps, the FP32 form
+  within 1e-5;
+- size-independent properties at those sizes: the internal forces sum to
+  zero over the nodes (sum_n dN_n/dX = 0 in every element, so translation
+  invariance holds up to rounding), and two identical runs give the same
+  bits.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from akantu2_amd import mesh as M
+from tests.helpers import make_gpu, relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+TOL_FP32 = 1e-5
+
+
+def _threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _oracle(mesh, blocked, v0, law, dt=None, **mat):
+    oracle.set_threads(_threads())
+    o = oracle.OracleModel(mesh.etype, mesh.nodes, mesh.conn, law, **mat)
+    o.init_full()
+    o.blocked[:] = blocked
+    o.v[:] = v0
+    o.set_time_step(dt if dt is not None else 0.5 * o.stable_dt())
+    return o
+
+
+def _translation_residual(f):
+    """|sum_n f_n| / sum_n |f_n| per component, the largest."""
+    return float((np.abs(f.sum(axis=0)) / np.abs(f).sum(axis=0)).max())
+
+
+@pytest.fixture(scope="module")
+def hex8_full():
+    t = time.perf_counter()
+    mesh = M.box_mesh("_hexahedron_8", 200, 100, 100, jitter=0.1, seed=1)
+    blocked = M.clamped_face_x0(mesh)
+    v0 = M.initial_velocity(mesh, 1e-3, seed=1)
+    o = _oracle(mesh, blocked, v0, "neohookean")
+    o.solve_steps(3)
+    print(f"hex8 full: mesh + oracle 3 steps {time.perf_counter() - t:.1f} s", flush=True)
+    yield mesh, blocked, v0, o
+    oracle.set_threads(1)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL), ("fp32", TOL_FP32)])
+def test_hex8_neohookean_200x100x100(hex8_full, precision, tol):
+    mesh, blocked, v0, o = hex8_full
+    assert mesh.nb_elements == 2_000_000
+    g, dt = make_gpu(mesh, blocked, v0, "neohookean", precision=precision)
+    assert dt == o.dt
+    g.solveStep(3)
+    errs = {f: relerr(getattr(g, f)(), ref) for f, ref in
+            (("getDisplacement", o.u), ("getVelocity", o.v),
+             ("getAcceleration", o.a), ("getInternalForce", o.f_int))}
+    tr = _translation_residual(g.getInternalForce())
+    print(precision, {k: f"{v:.3e}" for k, v in errs.items()}, f"sum f_int {tr:.2e}",
+          flush=True)
+    assert errs["getDisplacement"] <= tol and errs["getVelocity"] <= tol
+    if precision == "fp64":  # log() of the device and of glibc may differ in the last bit
+        assert max(errs.values()) <= tol
+        print("bit-identical f_int:", np.array_equal(g.getInternalForce(), o.f_int), flush=True)
+    assert tr <= 1e-9
+
+
+def test_hex8_full_size_runs_are_bit_identical(hex8_full):
+    mesh, blocked, v0, o = hex8_full
+    out = []
+    for _ in range(2):
+        g, _ = make_gpu(mesh, blocked, v0, "neohookean", dt=o.dt)
+        g.solveStep(5)
+        out.append(g.getDisplacement().tobytes())
+        del g
+    assert out[0] == out[1]
+
+
+def test_tet4_per_element_1000x100x100():
+    t = time.perf_counter()
+    mesh = M.box_mesh("_tetrahedron_4", 1000, 100, 100, jitter=0.1, seed=3)
+    assert mesh.nb_elements == 60_000_000
+    rng = np.random.default_rng(3)
+    E = np.exp(rng.uniform(np.log(0.5), np.log(2.0), mesh.nb_elements))
+    nu = rng.uniform(0.2, 0.35, mesh.nb_elements)
+    blocked = M.clamped_face_x0(mesh)
+    v0 = M.initial_velocity(mesh, 1e-3, 3)
+    o = _oracle(mesh, blocked, v0, "elastic", E_el=E, nu_el=nu)
+    o.solve_steps(2)
+    oracle.set_threads(1)
+    print(f"tet4 full: mesh + oracle 2 steps {time.perf_counter() - t:.1f} s", flush=True)
+    for precision, tol in (("fp64", TOL), ("fp32", TOL_FP32)):
+        g, dt = make_gpu(mesh, blocked, v0, "elastic", E_el=E, nu_el=nu,
+                         precision=precision)
+        assert dt == o.dt
+        g.solveStep(2)
+        eu = relerr(g.getDisplacement(), o.u)
+        ef = relerr(g.getInternalForce(), o.f_int)
+        tr = _translation_residual(g.getInternalForce())
+        print(f"tet4 {precision}: u {eu:.3e} f_int {ef:.3e} sum f_int {tr:.2e}", flush=True)
+        assert eu <= tol and tr <= 1e-9
+        if precision == "fp64":
+            assert np.array_equal(g.getDisplacement(), o.u)
+            assert np.array_equal(g.getInternalForce(), o.f_int)
+        del g
