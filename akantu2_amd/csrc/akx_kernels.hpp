@@ -513,7 +513,7 @@ k_element_forces(ElemArgs A) {
 // the dN/dX columns, grad u, the law and the 24 force terms of its Gauss
 // point times w*detJ; the per-node sums over the Gauss points are then taken
 // in ascending Gauss-point order (the oracle's and Eigen's GEMV order), two
-// nodes at a time, by the lanes that own the nodes.  The force terms reuse
+// nodes at a time, one component per lane.  The force terms reuse
 // the LDS of the nodal values, which are dead once grad u is formed: 4.6 KB
 // of LDS per 8 elements, so registers, not LDS, bound the waves per SIMD.
 //
@@ -605,7 +605,17 @@ __device__ __forceinline__ void hex8_strict_group(
       }
     }
     __syncthreads();
-    if (active && q < 2) {
+    if constexpr (W == 3) {
+      // lane q < 6 sums component q % 3 of node 2 hh2 + q / 3
+      if (active && q < 6) {
+        const int c = q >= 3 ? 1 : 0;
+        double s = sR[el][0][q];
+#pragma unroll
+        for (int qq = 1; qq < 8; ++qq)
+          s += sR[el][qq][q];
+        fe[(int64_t)pos_e[2 * hh2 + c] * 3 + (q - 3 * c)] = s;
+      }
+    } else if (active && q < 2) {
       const int n = 2 * hh2 + q;
       double r[3];
 #pragma unroll
