@@ -23,6 +23,10 @@ struct ElemArgs {
   const double *lam_el, *mu_el, *sy_el, *h_el; // per launch element, or null
   const int32_t *pos;     // [(pos_offset + i) * nnpe + a] -> entry of the
                           // node-major elemental-force table
+  // Hex8, plain FP64: J^-1 and w det J of every Gauss point of the mesh,
+  // [5][nel * 8] double2 {Ji00, Ji01} {Ji02, Ji10} {Ji11, Ji12} {Ji20, Ji21}
+  // {Ji22, wJ}, or null
+  const double2 *jw;
   void *fe;               // [entry][3] of FT
   // materialised internals [9][n*nqp] (or null), column-major 3x3 index
   double *grad_u, *stress;
@@ -71,6 +75,11 @@ struct NodesSoA {
       x[c] = X[(int64_t)c * stride + n];
       d[c] = u[(int64_t)c * stride + n];
     }
+  }
+  __device__ __forceinline__ void load_u(int32_t n, double (&d)[3]) const {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      d[c] = u[(int64_t)c * stride + n];
   }
 };
 

@@ -523,38 +523,62 @@ k_element_forces(ElemArgs A) {
 // ---------------------------------------------------------------------------
 constexpr int HEX_STRICT_EPB = 8;
 
-template <int LAW, int EPB, class HT, class NA, int W = 3>
+template <int LAW, int EPB, class HT, class NA, int W = 3, bool PRE = false>
 __device__ __forceinline__ void hex8_strict_group(
     double (&sN)[EPB][8][6], const double (&sD)[8][3][8],
     double wq, int el, int q, bool active, const int32_t *conn_e, const NA &nodes_of,
     double lambda, double mu, double sy, double hh,
-    const QpViewT<HT> &V, int64_t iq, const int32_t *pos_e, double *fe) {
+    const QpViewT<HT> &V, int64_t iq, const int32_t *pos_e, double *fe,
+    const double2 *jw = nullptr, int64_t jw_stride = 0) {
+  double Ji[3][3], g[3][3], S[3][3], wJ = 0.;
   if (active) {
-    double x[3], d[3];
-    nodes_of.load(conn_e[q], x, d);
+    if constexpr (PRE) {
+      // J^-1 and w det J from the table (jw: this Gauss point's column 0)
+      double d[3];
+      nodes_of.load_u(conn_e[q], d);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      sN[el][q][c] = x[c];
-      sN[el][q][3 + c] = d[c];
+      for (int c = 0; c < 3; ++c)
+        sN[el][q][3 + c] = d[c];
+      const double2 j0 = jw[0], j1 = jw[jw_stride], j2 = jw[2 * jw_stride],
+                    j3 = jw[3 * jw_stride], j4 = jw[4 * jw_stride];
+      Ji[0][0] = j0.x;
+      Ji[0][1] = j0.y;
+      Ji[0][2] = j1.x;
+      Ji[1][0] = j1.y;
+      Ji[1][1] = j2.x;
+      Ji[1][2] = j2.y;
+      Ji[2][0] = j3.x;
+      Ji[2][1] = j3.y;
+      Ji[2][2] = j4.x;
+      wJ = j4.y;
+    } else {
+      double x[3], d[3];
+      nodes_of.load(conn_e[q], x, d);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sN[el][q][c] = x[c];
+        sN[el][q][3 + c] = d[c];
+      }
     }
   }
   __syncthreads();
-  double Ji[3][3], g[3][3], S[3][3], wJ = 0.;
   if (active) {
-    // J(i, j) = sum_n dnds(i, n) X(n, j)
-    double J[3][3];
+    if constexpr (!PRE) {
+      // J(i, j) = sum_n dnds(i, n) X(n, j)
+      double J[3][3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+      for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        double s = sD[q][a][0] * sN[el][0][j];
+        for (int j = 0; j < 3; ++j) {
+          double s = sD[q][a][0] * sN[el][0][j];
 #pragma unroll
-        for (int n = 1; n < 8; ++n)
-          s += sD[q][a][n] * sN[el][n][j];
-        J[a][j] = s;
-      }
-    double detJ = jacobian_det_inv(J, Ji);
-    wJ = detJ * wq;
+          for (int n = 1; n < 8; ++n)
+            s += sD[q][a][n] * sN[el][n][j];
+          J[a][j] = s;
+        }
+      double detJ = jacobian_det_inv(J, Ji);
+      wJ = detJ * wq;
+    }
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
       double b[3];
@@ -632,6 +656,10 @@ __device__ __forceinline__ void hex8_strict_group(
   }
 }
 
+// The single-model launch reads J^-1 and w det J from the table that init
+// fills (k_hex8_jacobian_table): they depend on the reference coordinates
+// only, as the reference's FEEngine precompute does (fe_engine_template_tmpl
+// .hh:98-120), so the pass loads u but not X.
 template <int LAW>
 __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
   __shared__ double sN[HEX_STRICT_EPB][8][6];
@@ -654,10 +682,10 @@ __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
     if (A.h_el)
       hh = A.h_el[i];
   }
-  hex8_strict_group<LAW, HEX_STRICT_EPB, double>(
+  hex8_strict_group<LAW, HEX_STRICT_EPB, double, NodesSoA, 3, true>(
       sN, sD, c_hex8.w[q], el, q, active, A.conn + e * 8, NodesSoA{A.X, A.u, A.nn},
       lambda, mu, sy, hh, qp_view(A, 8), i * 8 + q, A.pos + (A.pos_offset + (active ? i : 0)) * 8,
-      reinterpret_cast<double *>(A.fe));
+      reinterpret_cast<double *>(A.fe), A.jw + e * 8 + q, A.nel * 8);
 }
 
 // ---------------------------------------------------------------------------

@@ -167,6 +167,35 @@ template <int ET> __global__ void k_check_jacobians(InitArgs A) {
   }
 }
 
+// J^-1 and w det J of every Hex8 Gauss point of the mesh, one thread per
+// Gauss point, in the arithmetic of the element pass (Shape::jacobian_at:
+// J = dN/dxi X summed over the nodes in order, the LU inverse, det J * w).
+__global__ void __launch_bounds__(256) k_hex8_jacobian_table(const int32_t *conn,
+                                                             const double *X, int64_t nn,
+                                                             int64_t nel, double2 *jw) {
+  const int64_t iq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nq = nel * 8;
+  if (iq >= nq)
+    return;
+  const int64_t e = iq >> 3;
+  const int q = (int)(iq & 7);
+  double Xe[8][3];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const int32_t nd = conn[e * 8 + n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      Xe[n][c] = X[(int64_t)c * nn + nd];
+  }
+  double Ji[3][3], detJ;
+  const double wJ = Shape<ET_HEX8, double>::jacobian_at(Xe, q, Ji, detJ);
+  jw[iq] = make_double2(Ji[0][0], Ji[0][1]);
+  jw[nq + iq] = make_double2(Ji[0][2], Ji[1][0]);
+  jw[2 * nq + iq] = make_double2(Ji[1][1], Ji[1][2]);
+  jw[3 * nq + iq] = make_double2(Ji[2][0], Ji[2][1]);
+  jw[4 * nq + iq] = make_double2(Ji[2][2], wJ);
+}
+
 // Lumped mass contributions m_e(a) = sum_q (rho N_a(q)) wJ_q
 // (fe_engine_template_tmpl_field.hh:112-138)
 template <int ET> __global__ void k_mass_elements(InitArgs A) {
@@ -445,6 +474,7 @@ struct akx_model {
   DevBuf<uint8_t> blocked;
   DevBuf<float> cu, cv;
   DevBuf<int32_t> conn, pos;
+  DevBuf<double2> jw;      // Hex8 plain FP64: J^-1, w det J [5][nel * 8]
   DevBuf<int64_t> ell_off; // [nn/256 + 1]
   DevBuf<uint8_t> deg;     // [nn]
   int64_t ell_total = 0;   // entries of the node-major table
@@ -620,6 +650,7 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
   A.h_el = mat.h_el.p;
   A.pos = m->pos.p;
   A.fe = m->fe.p;
+  A.jw = m->jw.p;
   if (mat.user()) {
     A.stress = mat.stress.p; // read by the force pass
   } else if (materialize && mat.law != AKX_LAW_PLASTIC_LIH) {
@@ -705,6 +736,7 @@ void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
 }
 
 void launch_hex8_strict(akx_model *m, Material &mat, const ElemArgs &A) {
+  require(A.jw != nullptr, "Hex8 element pass without its jacobian table");
   dim3 grid((unsigned)A.nblocks_padded), block(64);
   switch (mat.law) {
   case AKX_LAW_ELASTIC:
@@ -784,6 +816,7 @@ void user_stress_phase(akx_model *m) {
     A.stress = nullptr;
     require(A.grad_u && A.conn && A.X && A.u, "user material gradient pass without buffers");
     if (m->etype == AKX_HEX8) {
+      require(A.jw != nullptr, "Hex8 gradient pass without its jacobian table");
       const int64_t nb = (mat.n + HEX_STRICT_EPB - 1) / HEX_STRICT_EPB;
       A.nblocks_padded = (nb + 7) / 8 * 8;
       hipLaunchKernelGGL(k_hex8_strict<LAW_USER_GRAD>, dim3((unsigned)A.nblocks_padded),
@@ -1398,6 +1431,16 @@ int akx_model_init_full(akx_model *m) {
                       bad % m->nqp, bad / m->nqp);
         throw Error(AKX_ERR_JACOBIAN, msg);
       }
+    }
+    // FEEngine precompute of the plain FP64 Hex8 pass: J^-1 and w det J per
+    // Gauss point (fe_engine_template_tmpl.hh:98-120)
+    if (m->etype == AKX_HEX8 && m->precision == AKX_PRECISION_FP64 && m->nel > 0) {
+      const int64_t nq = m->nel * 8;
+      m->jw.alloc(5 * nq);
+      hipLaunchKernelGGL(k_hex8_jacobian_table, dim3(grid_for(nq)), dim3(TPB), 0,
+                         m->stream, (const int32_t *)m->conn.p, (const double *)m->X.p,
+                         m->nn, m->nel, m->jw.p);
+      HIPCHECK(hipGetLastError());
     }
     build_gather_table(m);
     m->fe.alloc(std::max<int64_t>(m->ell_total, 1) * 3 * m->fe_elem_size());
@@ -2327,9 +2370,14 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
     (void)deg;
     switch (kernel_class) {
     case 0: // element pass: connectivity + table entries, X and u once per
-            // node, r_e write
-      *bytes = nel_all * (8.0 * m->nnpe + 3.0 * s_f * m->nnpe) +
-               m->nn * (3.0 * s_node + 3.0 * s_node) + extra;
+            // node, r_e write; with the Hex8 jacobian table, the table (80 B
+            // per Gauss point) instead of X
+      if (m->jw.p)
+        *bytes = nel_all * (8.0 * m->nnpe + 3.0 * s_f * m->nnpe + 80.0 * m->nqp) +
+                 m->nn * 3.0 * s_node + extra;
+      else
+        *bytes = nel_all * (8.0 * m->nnpe + 3.0 * s_f * m->nnpe) +
+                 m->nn * (3.0 * s_node + 3.0 * s_node) + extra;
       break;
     case 1: { // gather + update: degree, r_e read, f_int write,
               // mass, blocked, a/v read+write, u read+write (fused predictor)
