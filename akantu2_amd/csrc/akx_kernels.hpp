@@ -530,7 +530,12 @@ __device__ __forceinline__ void hex8_strict_group(
     double lambda, double mu, double sy, double hh,
     const QpViewT<HT> &V, int64_t iq, const int32_t *pos_e, double *fe,
     const double2 *jw = nullptr, int64_t jw_stride = 0) {
+  // With the table, the dN/dX columns of the gradient are kept for the
+  // force terms (24 registers instead of J^-1's 9 and 120 operations); the
+  // plastic law keeps recomputing them, its history already holds registers.
+  constexpr bool KEEP_B = PRE && LAW != LAW_PLASTIC;
   double Ji[3][3], g[3][3], S[3][3], wJ = 0.;
+  double Bk[KEEP_B ? 8 : 1][3];
   if (active) {
     if constexpr (PRE) {
       // J^-1 and w det J from the table (jw: this Gauss point's column 0)
@@ -588,6 +593,8 @@ __device__ __forceinline__ void hex8_strict_group(
         s += Ji[a][1] * sD[q][1][n];
         s += Ji[a][2] * sD[q][2][n];
         b[a] = s;
+        if constexpr (KEEP_B)
+          Bk[n][a] = s;
       }
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -614,10 +621,14 @@ __device__ __forceinline__ void hex8_strict_group(
         double b[3], r[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          double s = Ji[a][0] * sD[q][0][n];
-          s += Ji[a][1] * sD[q][1][n];
-          s += Ji[a][2] * sD[q][2][n];
-          b[a] = s;
+          if constexpr (KEEP_B) {
+            b[a] = Bk[n][a];
+          } else {
+            double s = Ji[a][0] * sD[q][0][n];
+            s += Ji[a][1] * sD[q][1][n];
+            s += Ji[a][2] * sD[q][2][n];
+            b[a] = s;
+          }
         }
         if constexpr (finite_law<LAW>())
           btd_finite(g, S, b, r);
