@@ -718,7 +718,7 @@ struct GatherArgs {
   const int64_t *ell_off; // [nn/256 + 1] first entry of each 256-node slice
   const uint8_t *deg;      // [nn] elements around each node
   const void *fe;          // [entry][3]
-  double *fint;          // [3][nn]
+  double *fint;          // [3][nn], or null (not stored: steps follow)
   const double *fext;    // [3][nn] or null (zero external force)
   const double *mass;    // [nn]
   const uint8_t *blocked;
@@ -749,9 +749,11 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
     f1 = f1 + -1. * double(p[1]);
     f2 = f2 + -1. * double(p[2]);
   }
-  A.fint[n] = f0;
-  A.fint[A.nn + n] = f1;
-  A.fint[2 * A.nn + n] = f2;
+  if (A.fint) { // null while more steps follow in the same batch
+    A.fint[n] = f0;
+    A.fint[A.nn + n] = f1;
+    A.fint[2 * A.nn + n] = f2;
+  }
   if constexpr (!UPDATE)
     return;
   const double M = A.mass[n];

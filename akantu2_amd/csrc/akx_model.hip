@@ -677,7 +677,8 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
 // Host-side checks that every buffer a kernel variant dereferences exists
 // (a kernel must never be launched with a null pointer it will use).
 void check_gather_args(const GatherArgs &G, bool update, bool comp) {
-  require(G.ell_off && G.deg && G.fe && G.fint, "gather launched without its table");
+  require(G.ell_off && G.deg && G.fe, "gather launched without its table");
+  require(G.fint || update, "a gather without update must store f_int");
   if (update)
     require(G.mass && G.blocked && G.u && G.v && G.a,
             "nodal update launched without nodal state");
@@ -866,8 +867,13 @@ GatherArgs gather_args(akx_model *m) {
   return G;
 }
 
+// f_int is stored by the last step of a solve_step call (and by the gathers
+// without update): the steps in between use it only inside the gather, and
+// nothing reads the stored array before the call returns.
 void launch_gather(akx_model *m, bool update, bool predict) {
   GatherArgs G = gather_args(m);
+  if (update && predict)
+    G.fint = nullptr;
   dim3 grid((unsigned)G.nblocks_padded), block(TPB);
   const bool f64 = m->precision != AKX_PRECISION_FP32;
   check_gather_args(G, update, update && !f64);
@@ -1030,6 +1036,8 @@ void launch_element_part(akx_model *m, int p) {
 
 void launch_gather_group(akx_model *m, int p, bool predict) {
   GatherArgs G = gather_args(m);
+  if (predict)
+    G.fint = nullptr; // stored by the last step only (launch_gather)
   G.node0 = m->pipe_node[p];
   G.node_end = m->pipe_node[p + 1];
   if (G.node_end <= G.node0)
@@ -2379,9 +2387,10 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
         *bytes = nel_all * (8.0 * m->nnpe + 3.0 * s_f * m->nnpe) +
                  m->nn * (3.0 * s_node + 3.0 * s_node) + extra;
       break;
-    case 1: { // gather + update: degree, r_e read, f_int write,
-              // mass, blocked, a/v read+write, u read+write (fused predictor)
-      double per_node = 4.0 + 3 * s_node /*f_int*/ + s_node /*M*/ + 1.0 +
+    case 1: { // gather + update of a step followed by another: degree, r_e
+              // read, mass, blocked, a/v read+write, u read+write (fused
+              // predictor); the last step of a call also writes f_int
+      double per_node = 4.0 + s_node /*M*/ + 1.0 +
                         2 * 3 * s_node /*a*/ + 2 * 3 * s_node /*v*/ +
                         2 * 3 * s_node /*u*/ + (m->has_fext ? 3 * s_node : 0.);
       if (m->precision == AKX_PRECISION_FP32)

@@ -145,6 +145,8 @@ def test_step_batching_is_invisible():
         g2.solveStep()
     assert np.array_equal(g1.getDisplacement(), g2.getDisplacement())
     assert np.array_equal(g1.getVelocity(), g2.getVelocity())
+    # f_int is stored by the last step of a call only; both hold the same
+    assert np.array_equal(g1.getInternalForce(), g2.getInternalForce())
 
 
 def test_user_writes_through_getters():
