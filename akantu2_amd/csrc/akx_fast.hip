@@ -61,7 +61,7 @@ __device__ __forceinline__ R inv_det(const R (&m)[3][3], R (&r)[3][3]) {
   R c10 = m[2][1] * m[0][2] - m[2][2] * m[0][1];
   R c20 = m[0][1] * m[1][2] - m[0][2] * m[1][1];
   R det = c00 * m[0][0] + c10 * m[1][0] + c20 * m[2][0];
-  R id = R(1) / det;
+  R id = akx_rcp(det);
   r[0][0] = c00 * id;
   r[0][1] = c10 * id;
   r[0][2] = c20 * id;

@@ -244,9 +244,35 @@ AKX_HD void neohookean_pk2(R lambda, R mu, const R (&g)[3][3], R (&S)[3][3]) {
 // J - 1 = tr g + ((tr g)^2 - tr(g g)) / 2 + det g.
 // At strains of 1e-3 the reference's form keeps only ~4 significant digits
 // in FP32; this one keeps full FP32 accuracy.
+// log1p in FP32: for |x| < 1/16 the Taylor series to x^7 (truncation below
+// (1/16)^7 / 8 = 5e-10 relative, under the FP32 rounding of the result), in
+// Horner form with FMAs; larger |x| (strains beyond a few percent) take the
+// device library's log1pf, which costs about 120 instructions.
 template <class R> AKX_HD R akx_log1p(R x);
 template <> AKX_HD double akx_log1p<double>(double x) { return log1p(x); }
-template <> AKX_HD float akx_log1p<float>(float x) { return log1pf(x); }
+template <> AKX_HD float akx_log1p<float>(float x) {
+  if (fabsf(x) < 0.0625f) {
+    float p = fmaf(1.f / 7.f, x, -1.f / 6.f);
+    p = fmaf(p, x, 0.2f);
+    p = fmaf(p, x, -0.25f);
+    p = fmaf(p, x, 1.f / 3.f);
+    p = fmaf(p, x, -0.5f);
+    return fmaf(p * x, x, x);
+  }
+  return log1pf(x);
+}
+
+// 1 / x: the hardware reciprocal (1 ulp) for FP32 on the device, where the
+// correctly rounded division sequence costs ten instructions; exact
+// division elsewhere.
+template <class R> AKX_HD R akx_rcp(R x) { return R(1) / x; }
+template <> AKX_HD float akx_rcp<float>(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.f / x;
+#endif
+}
 
 // E, C^{-1} and S are symmetric (C^{-1} is a function of E, so C^{-1} E is
 // too): each is formed once per pair (i <= j) and mirrored.
@@ -278,7 +304,7 @@ AKX_HD void neohookean_pk2_stable(R lambda, R mu, const R (&g)[3][3],
   const R c01 = R(2) * E[3], c02 = R(2) * E[4], c12 = R(2) * E[5];
   R k00 = c11 * c22 - c12 * c12, k11 = c00 * c22 - c02 * c02, k22 = c00 * c11 - c01 * c01;
   R k01 = c02 * c12 - c01 * c22, k02 = c01 * c12 - c02 * c11, k12 = c01 * c02 - c00 * c12;
-  const R id = R(1) / (c00 * k00 + c01 * k01 + c02 * k02);
+  const R id = akx_rcp(c00 * k00 + c01 * k01 + c02 * k02);
   const R Ci[3][3] = {{k00 * id, k01 * id, k02 * id},
                       {k01 * id, k11 * id, k12 * id},
                       {k02 * id, k12 * id, k22 * id}};
