@@ -151,7 +151,8 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t o = (int64_t)c * G.ncap + n;
-      G.fint[o] = f[c];
+      if (G.fint) // null while more steps follow in the same call
+        G.fint[o] = f[c];
       if ((blk >> c) & 1)
         continue;
       double a = G.a[o], v = G.v[o];
@@ -672,10 +673,14 @@ BatchGatherArgs gather_args(akx_batch *b) {
   return G;
 }
 
+// f_int is stored by the last step of a call only: the steps before it use
+// it inside the gather, and nothing reads the array before the call returns.
 void gather_pass(akx_batch *b, bool predict) {
   if (b->nslices_live == 0)
     return;
-  const BatchGatherArgs G = gather_args(b);
+  BatchGatherArgs G = gather_args(b);
+  if (predict)
+    G.fint = nullptr;
   dim3 grid((unsigned)G.nblocks), block(256);
   if (b->fp32()) {
     if (predict)
@@ -1387,7 +1392,8 @@ int akx_batch_get_kernel_bytes(akx_batch *b, int kernel_class, double *bytes) {
         if (I.law == AKX_LAW_PLASTIC_LIH) // history read and written
           total += (double)I.nel * nqp * 2 * (b->fp32() ? 7 : 28) * hs;
       } else {
-        total += corners * 3 * fs + nn * (1 + 1 + 8 + 3 * 8 + 3 * 3 * 2 * 8 +
+        // a step followed by another (f_int is written by the last one only)
+        total += corners * 3 * fs + nn * (1 + 1 + 8 + 3 * 3 * 2 * 8 +
                                           (b->fp32() ? 2 * 3 * 2 * 4 : 0));
       }
     }
