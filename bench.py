@@ -184,11 +184,11 @@ def run_plain(args):
             if tr.get("shape") == list(shape):
                 traffic = tr.get("element_pass_bytes")
         # The dominant kernel (the element pass, ~85 % of the step) is bound
-        # by FP64 arithmetic, not by HBM: it recomputes dN/dX and w detJ from
-        # the node coordinates every step instead of streaming them.  MI355X's
-        # FP64 peak is the same 78.6 TFLOP/s on the matrix cores and the
-        # vector ALUs; the reference operation order is FMA-free, so its own
-        # ceiling is half.
+        # by FP64 arithmetic, not by HBM: it forms the dN/dX columns from the
+        # stored J^-1 every step instead of streaming them, and its table
+        # reads run at a third of the HBM bandwidth.  MI355X's FP64 peak is
+        # the same 78.6 TFLOP/s on the matrix cores and the vector ALUs; the
+        # reference operation order is FMA-free, so its own ceiling is half.
         roofline = {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
                     "traffic": traffic,
