@@ -27,6 +27,11 @@ struct ElemArgs {
   // [5][nel * 8] double2 {Ji00, Ji01} {Ji02, Ji10} {Ji11, Ji12} {Ji20, Ji21}
   // {Ji22, wJ}, or null
   const double2 *jw;
+  // Tet4: one 48-byte record {X, u} per node, [nn][6], kept equal to X and u
+  // by every kernel that writes u; the element pass reads it instead of the
+  // six [3][nn] arrays (three 16-byte loads per node instead of six 8-byte
+  // loads from six cache lines)
+  const double *xu;
   void *fe;               // [entry][3] of FT
   // materialised internals [9][n*nqp] (or null), column-major 3x3 index
   double *grad_u, *stress;

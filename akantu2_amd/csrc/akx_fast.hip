@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
   const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
   CT lambda, mu, sy, hh;
   element_params(A, i, lambda, mu, sy, hh);
-  tet4_one<LAW, CT, FT, QpViewT<double>>(A.conn + e * 4, NodesSoA{A.X, A.u, A.nn}, lambda,
+  tet4_one<LAW, CT, FT, QpViewT<double>, akx::NodesXU>(A.conn + e * 4, akx::NodesXU{A.xu}, lambda,
                                          mu, sy, hh,
                                 qp_view(A, 1), i, A.pos + (A.pos_offset + i) * 4,
                                 reinterpret_cast<FT *>(A.fe));

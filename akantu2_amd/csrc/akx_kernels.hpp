@@ -500,9 +500,14 @@ k_element_forces(ElemArgs A) {
     sy = CT(A.sy_el[i]);
   if (A.h_el)
     hh = CT(A.h_el[i]);
-  element_forces_one<ET, LAW, CT, FT, double>(A.conn + e * NN, NodesSoA{A.X, A.u, A.nn},
-                                              lambda, mu, sy, hh, qp_view(A, NQ), i * NQ,
-                                      A.pos + (A.pos_offset + i) * NN, A.fe);
+  if constexpr (ET == ET_TET4)
+    element_forces_one<ET, LAW, CT, FT, double>(A.conn + e * NN, NodesXU{A.xu}, lambda, mu,
+                                                sy, hh, qp_view(A, NQ), i * NQ,
+                                                A.pos + (A.pos_offset + i) * NN, A.fe);
+  else
+    element_forces_one<ET, LAW, CT, FT, double>(A.conn + e * NN, NodesSoA{A.X, A.u, A.nn},
+                                                lambda, mu, sy, hh, qp_view(A, NQ), i * NQ,
+                                                A.pos + (A.pos_offset + i) * NN, A.fe);
 }
 
 
@@ -723,6 +728,7 @@ struct GatherArgs {
   const double *mass;    // [nn]
   const uint8_t *blocked;
   double *u, *v, *a;     // [3][nn]
+  double *xu;            // Tet4: [nn][6] {X, u} node records (u kept equal), or null
   float *cu, *cv;        // compensation terms [3][nn] (FP32 form) or null
   double dt;
   int64_t nblocks_padded;
@@ -760,6 +766,15 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
   const uint8_t blk = A.blocked[n];
   const double dd = .5 * A.dt; // beta * dt
   double f[3] = {f0, f1, f2};
+  // u after the predictor (blocked components keep theirs), for the record
+  double un[3] = {0., 0., 0.};
+  if constexpr (PREDICT) {
+    if (A.xu)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if ((blk >> c) & 1)
+          un[c] = A.u[(int64_t)c * A.nn + n];
+  }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     if ((blk >> c) & 1)
@@ -785,6 +800,7 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
         kahan_add(u, cu, A.dt * v + (.5 * A.dt) * dt_a);
         kahan_add(v, cv, dt_a);
         A.u[o] = u;
+        un[c] = u;
         A.cu[o] = cu;
       }
       A.cv[o] = cv;
@@ -792,12 +808,20 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
       v += dd * x;
       if constexpr (PREDICT) {
         const double dt_a = A.dt * a;
-        A.u[o] = A.u[o] + (A.dt * v + (.5 * A.dt) * dt_a);
+        const double u = A.u[o] + (A.dt * v + (.5 * A.dt) * dt_a);
+        A.u[o] = u;
+        un[c] = u;
         v = v + dt_a;
       }
     }
     A.a[o] = a;
     A.v[o] = v;
+  }
+  if constexpr (PREDICT) {
+    if (A.xu) { // u0 at 48 n + 24 bytes, then an aligned pair
+      A.xu[6 * n + 3] = un[0];
+      *reinterpret_cast<double2 *>(A.xu + 6 * n + 4) = make_double2(un[1], un[2]);
+    }
   }
 }
 
@@ -808,6 +832,12 @@ __global__ void __launch_bounds__(256) k_predictor(GatherArgs A) {
   if (n >= A.node_end)
     return;
   const uint8_t blk = A.blocked[n];
+  double un[3] = {0., 0., 0.}; // u after the predictor, for the record
+  if (A.xu)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if ((blk >> c) & 1)
+        un[c] = A.u[(int64_t)c * A.nn + n];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     if ((blk >> c) & 1)
@@ -827,7 +857,12 @@ __global__ void __launch_bounds__(256) k_predictor(GatherArgs A) {
       v = v + dt_a;
     }
     A.u[o] = u;
+    un[c] = u;
     A.v[o] = v;
+  }
+  if (A.xu) {
+    A.xu[6 * n + 3] = un[0];
+    *reinterpret_cast<double2 *>(A.xu + 6 * n + 4) = make_double2(un[1], un[2]);
   }
 }
 

@@ -196,6 +196,21 @@ __global__ void __launch_bounds__(256) k_hex8_jacobian_table(const int32_t *conn
   jw[4 * nq + iq] = make_double2(Ji[2][2], wJ);
 }
 
+// {X, u} node records of the Tet4 element pass from the [3][nn] arrays:
+// u only (refresh after an upload of u) or X and u (init)
+__global__ void __launch_bounds__(256) k_fill_xu(const double *X, const double *u, int64_t nn,
+                                                 double *xu, bool with_x) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn)
+    return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (with_x)
+      xu[6 * n + c] = X[(int64_t)c * nn + n];
+    xu[6 * n + 3 + c] = u[(int64_t)c * nn + n];
+  }
+}
+
 // Lumped mass contributions m_e(a) = sum_q (rho N_a(q)) wJ_q
 // (fe_engine_template_tmpl_field.hh:112-138)
 template <int ET> __global__ void k_mass_elements(InitArgs A) {
@@ -475,6 +490,7 @@ struct akx_model {
   DevBuf<float> cu, cv;
   DevBuf<int32_t> conn, pos;
   DevBuf<double2> jw;      // Hex8 plain FP64: J^-1, w det J [5][nel * 8]
+  DevBuf<double> xu;       // Tet4: [nn][6] {X, u} node records of the element pass
   DevBuf<int64_t> ell_off; // [nn/256 + 1]
   DevBuf<uint8_t> deg;     // [nn]
   int64_t ell_total = 0;   // entries of the node-major table
@@ -621,6 +637,15 @@ void download_soa(akx_model *m, const T *dev_soa, T *host_aos, int64_t n, int co
   HIPCHECK(hipStreamSynchronize(m->stream));
 }
 
+// Tet4 node records: X and u (init) or u alone (after an upload of u)
+void refresh_xu(akx_model *m, bool with_x) {
+  if (!m->xu.p || m->nn == 0)
+    return;
+  hipLaunchKernelGGL(k_fill_xu, dim3(grid_for(m->nn)), dim3(TPB), 0, m->stream,
+                     (const double *)m->X.p, (const double *)m->u.p, m->nn, m->xu.p, with_x);
+  HIPCHECK(hipGetLastError());
+}
+
 void zero_async(akx_model *m, double *p, int64_t n) {
   if (n <= 0)
     return;
@@ -651,6 +676,7 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
   A.pos = m->pos.p;
   A.fe = m->fe.p;
   A.jw = m->jw.p;
+  A.xu = m->xu.p;
   if (mat.user()) {
     A.stress = mat.stress.p; // read by the force pass
   } else if (materialize && mat.law != AKX_LAW_PLASTIC_LIH) {
@@ -772,6 +798,7 @@ void launch_element_pass(akx_model *m, bool materialize) {
       continue;
     ElemArgs A = elem_args(m, mat, materialize);
     check_elem_args(A, mat.law);
+    require(m->etype != AKX_TET4 || A.xu != nullptr, "Tet4 element pass without its node records");
     require(m->fe.n >= (size_t)(m->ell_total * 3 * m->fe_elem_size()) &&
                 m->pos.n >= (size_t)((mat.pos_offset + mat.n) * m->nnpe),
             "element force table too small for this material");
@@ -816,6 +843,7 @@ void user_stress_phase(akx_model *m) {
     A.grad_u = mat.grad.p;
     A.stress = nullptr;
     require(A.grad_u && A.conn && A.X && A.u, "user material gradient pass without buffers");
+    require(m->etype != AKX_TET4 || A.xu != nullptr, "Tet4 gradient pass without its node records");
     if (m->etype == AKX_HEX8) {
       require(A.jw != nullptr, "Hex8 gradient pass without its jacobian table");
       const int64_t nb = (mat.n + HEX_STRICT_EPB - 1) / HEX_STRICT_EPB;
@@ -858,6 +886,7 @@ GatherArgs gather_args(akx_model *m) {
   G.mass = m->mass.p;
   G.blocked = m->blocked.p;
   G.u = m->u.p;
+  G.xu = m->xu.p;
   G.v = m->v.p;
   G.a = m->a.p;
   G.cu = m->cu.p;
@@ -1023,6 +1052,7 @@ void launch_element_part(akx_model *m, int p) {
                        : akxf::elements_per_block(m->etype);
   A.nblocks_padded = ((cnt + epb - 1) / epb + 7) / 8 * 8;
   check_elem_args(A, mat.law);
+  require(m->etype != AKX_TET4 || A.xu != nullptr, "Tet4 element pass without its node records");
   if (strict) {
     if (m->etype == AKX_TET4)
       launch_elem_law<ET_TET4, double, double>(m, mat, A);
@@ -1450,6 +1480,10 @@ int akx_model_init_full(akx_model *m) {
                          m->nn, m->nel, m->jw.p);
       HIPCHECK(hipGetLastError());
     }
+    if (m->etype == AKX_TET4 && m->nn > 0) {
+      m->xu.alloc(6 * m->nn);
+      refresh_xu(m, true);
+    }
     build_gather_table(m);
     m->fe.alloc(std::max<int64_t>(m->ell_total, 1) * 3 * m->fe_elem_size());
     // lumped mass
@@ -1809,6 +1843,8 @@ int akx_model_set_field(akx_model *m, int field, const void *host) {
       throw Error(AKX_ERR_INVALID, "field is read only or unknown");
     }
     upload_soa(m, (const double *)host, dst, m->nn, 3);
+    if (field == AKX_FIELD_DISPLACEMENT)
+      refresh_xu(m, false);
   });
 }
 
@@ -2390,7 +2426,7 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
     case 1: { // gather + update of a step followed by another: degree, r_e
               // read, mass, blocked, a/v read+write, u read+write (fused
               // predictor); the last step of a call also writes f_int
-      double per_node = 4.0 + s_node /*M*/ + 1.0 +
+      double per_node = 4.0 + s_node /*M*/ + 1.0 + (m->xu.p ? 3 * s_node : 0.) /*u records*/ +
                         2 * 3 * s_node /*a*/ + 2 * 3 * s_node /*v*/ +
                         2 * 3 * s_node /*u*/ + (m->has_fext ? 3 * s_node : 0.);
       if (m->precision == AKX_PRECISION_FP32)
