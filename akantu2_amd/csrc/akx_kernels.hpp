@@ -535,10 +535,10 @@ __device__ __forceinline__ void hex8_strict_group(
     double lambda, double mu, double sy, double hh,
     const QpViewT<HT> &V, int64_t iq, const int32_t *pos_e, double *fe,
     const double2 *jw = nullptr, int64_t jw_stride = 0) {
-  // With the table, the dN/dX columns of the gradient are kept for the
-  // force terms (24 registers instead of J^-1's 9 and 120 operations); the
-  // plastic law keeps recomputing them, its history already holds registers.
-  constexpr bool KEEP_B = PRE && LAW != LAW_PLASTIC;
+  // The dN/dX columns of the gradient are kept for the force terms (24
+  // registers instead of J^-1's 9 and 120 operations); the plastic law keeps
+  // recomputing them, its history already holds registers.
+  constexpr bool KEEP_B = LAW != LAW_PLASTIC;
   double Ji[3][3], g[3][3], S[3][3], wJ = 0.;
   double Bk[KEEP_B ? 8 : 1][3];
   if (active) {
