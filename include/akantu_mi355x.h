@@ -242,7 +242,9 @@ int akx_model_synchronize(akx_model *m);
 int akx_model_enable_kernel_timing(akx_model *m, int enable);
 int akx_model_get_kernel_timing(akx_model *m, int kernel_class,
                                 double *total_ms, int64_t *launches);
-/* Bytes each kernel class moves per launch in the unique-bytes model. */
+/* Bytes each kernel class moves per launch in the unique-bytes model (each
+ * array read or written once, node data once per node); for class 1 the
+ * gather of a step followed by another, which does not store f_int. */
 int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes);
 /* Streaming device copy of `bytes` (read + write), repeated; average time
  * per copy in ms.  No counterpart in the reference: it calibrates the HBM
