@@ -148,6 +148,15 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
     const double M = G.mass[n];
     const uint8_t blk = G.blocked[n];
     const double dd = .5 * dt; // beta * dt
+    // u of the node's record (u0 at 48 n + 24 bytes, then an aligned pair),
+    // read and written whole; blocked components keep their value
+    double us[3] = {0., 0., 0.};
+    if constexpr (PREDICT) {
+      const double2 u12 = *reinterpret_cast<const double2 *>(G.xu + 6 * n + 4);
+      us[0] = G.xu[6 * n + 3];
+      us[1] = u12.x;
+      us[2] = u12.y;
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t o = (int64_t)c * G.ncap + n;
@@ -168,27 +177,27 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
         float cv = G.cv[o];
         kahan_add(v, cv, dd * x);
         if constexpr (PREDICT) {
-          double *up = G.xu + 6 * n + 3 + c;
-          double u = *up;
           float cu = G.cu[o];
           const double dt_a = dt * a;
-          kahan_add(u, cu, dt * v + (.5 * dt) * dt_a);
+          kahan_add(us[c], cu, dt * v + (.5 * dt) * dt_a);
           kahan_add(v, cv, dt_a);
-          *up = u;
           G.cu[o] = cu;
         }
         G.cv[o] = cv;
       } else {
         v += dd * x;
         if constexpr (PREDICT) {
-          double *up = G.xu + 6 * n + 3 + c;
           const double dt_a = dt * a;
-          *up = *up + (dt * v + (.5 * dt) * dt_a);
+          us[c] = us[c] + (dt * v + (.5 * dt) * dt_a);
           v = v + dt_a;
         }
       }
       G.a[o] = a;
       G.v[o] = v;
+    }
+    if constexpr (PREDICT) {
+      G.xu[6 * n + 3] = us[0];
+      *reinterpret_cast<double2 *>(G.xu + 6 * n + 4) = make_double2(us[1], us[2]);
     }
   }
 }
@@ -205,14 +214,15 @@ __global__ void __launch_bounds__(256) k_batch_predictor(BatchGatherArgs G) {
     const double dt = G.inst[SL.inst].dt;
     const int64_t n = SL.node0 + threadIdx.x;
     const uint8_t blk = G.blocked[n];
+    const double2 u12 = *reinterpret_cast<const double2 *>(G.xu + 6 * n + 4);
+    double us[3] = {G.xu[6 * n + 3], u12.x, u12.y};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       if ((blk >> c) & 1)
         continue;
       const int64_t o = (int64_t)c * G.ncap + n;
       const double a = G.a[o];
-      double *up = G.xu + 6 * n + 3 + c;
-      double v = G.v[o], u = *up;
+      double v = G.v[o], u = us[c];
       const double dt_a = dt * a;
       if constexpr (COMP) {
         float cu = G.cu[o], cv = G.cv[o];
@@ -224,9 +234,11 @@ __global__ void __launch_bounds__(256) k_batch_predictor(BatchGatherArgs G) {
         u = u + (dt * v + (.5 * dt) * dt_a);
         v = v + dt_a;
       }
-      *up = u;
+      us[c] = u;
       G.v[o] = v;
     }
+    G.xu[6 * n + 3] = us[0];
+    *reinterpret_cast<double2 *>(G.xu + 6 * n + 4) = make_double2(us[1], us[2]);
   }
 }
 
