@@ -750,10 +750,12 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
   double f0 = 0., f1 = 0., f2 = 0.;
   const int deg = A.deg[n];
   const FT *p = fe + (A.ell_off[n >> 8] + (n & 255)) * 3;
+  // the table is read once per step: non-temporal loads keep it from
+  // displacing the nodal state in the caches
   for (int d = 0; d < deg; ++d, p += 256 * 3) {
-    f0 = f0 + -1. * double(p[0]);
-    f1 = f1 + -1. * double(p[1]);
-    f2 = f2 + -1. * double(p[2]);
+    f0 = f0 + -1. * double(__builtin_nontemporal_load(p));
+    f1 = f1 + -1. * double(__builtin_nontemporal_load(p + 1));
+    f2 = f2 + -1. * double(__builtin_nontemporal_load(p + 2));
   }
   if (A.fint) { // null while more steps follow in the same batch
     A.fint[n] = f0;
