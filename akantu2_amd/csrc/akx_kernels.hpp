@@ -543,14 +543,21 @@ __device__ __forceinline__ void hex8_strict_group(
   double Bk[KEEP_B ? 8 : 1][3];
   if (active) {
     if constexpr (PRE) {
-      // J^-1 and w det J from the table (jw: this Gauss point's column 0)
+      // J^-1 and w det J from the table (jw: this Gauss point's column 0),
+      // streamed once per step: non-temporal loads leave the caches to the
+      // nodal data
       double d[3];
       nodes_of.load_u(conn_e[q], d);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
         sN[el][q][3 + c] = d[c];
-      const double2 j0 = jw[0], j1 = jw[jw_stride], j2 = jw[2 * jw_stride],
-                    j3 = jw[3 * jw_stride], j4 = jw[4 * jw_stride];
+      typedef double dv2 __attribute__((ext_vector_type(2)));
+      const dv2 *jv = reinterpret_cast<const dv2 *>(jw);
+      const dv2 j0 = __builtin_nontemporal_load(jv),
+                j1 = __builtin_nontemporal_load(jv + jw_stride),
+                j2 = __builtin_nontemporal_load(jv + 2 * jw_stride),
+                j3 = __builtin_nontemporal_load(jv + 3 * jw_stride),
+                j4 = __builtin_nontemporal_load(jv + 4 * jw_stride);
       Ji[0][0] = j0.x;
       Ji[0][1] = j0.y;
       Ji[0][2] = j1.x;
