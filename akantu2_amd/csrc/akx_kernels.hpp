@@ -513,18 +513,21 @@ k_element_forces(ElemArgs A) {
 
 // ---------------------------------------------------------------------------
 // Hex8 element pass in the reference's operation order, one lane per Gauss
-// point (8 lanes per element, EPB elements per block).  Nodal coordinates
-// and displacements are staged in LDS; each lane forms J, its LU inverse,
-// the dN/dX columns, grad u, the law and the 24 force terms of its Gauss
-// point times w*detJ; the per-node sums over the Gauss points are then taken
-// in ascending Gauss-point order (the oracle's and Eigen's GEMV order), two
-// nodes at a time, one component per lane.  The force terms reuse
-// the LDS of the nodal values, which are dead once grad u is formed: 4.6 KB
-// of LDS per 8 elements, so registers, not LDS, bound the waves per SIMD.
+// point (8 lanes per element, EPB elements per block).  Nodal values are
+// staged in LDS.  Each lane forms J and its LU inverse from the staged
+// coordinates (PRE = false, batches) or reads J^-1 and w*detJ from the
+// table init fills (PRE = true, single model; only u is staged), then the
+// dN/dX columns, grad u, the law and the 24 force terms of its Gauss point
+// times w*detJ; the per-node sums over the Gauss points are then taken in
+// ascending Gauss-point order (the oracle's and Eigen's GEMV order), two
+// nodes at a time, one component per lane.  The force terms reuse the LDS
+// of the nodal values, which are dead once grad u is formed: 4.6 KB of LDS
+// per 8 elements, so registers, not LDS, bound the waves per SIMD.
 //
 // hex8_strict_group handles one group of EPB elements; every lane of the
 // block calls it (it holds barriers), inactive lanes with active = false.
-// sD holds dN_n/dxi_m at the Gauss points, [q][m][n]; wq is the Gauss weight.
+// sD holds dN_n/dxi_m at the Gauss points, [q][m][n]; wq is the Gauss weight;
+// with PRE, jw points at the lane's Gauss point in the [5][jw_stride] table.
 // ---------------------------------------------------------------------------
 constexpr int HEX_STRICT_EPB = 8;
 
