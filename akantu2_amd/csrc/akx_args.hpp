@@ -21,6 +21,7 @@ struct ElemArgs {
   int64_t nn;
   double lambda, mu, sigma_y, h;
   const double *lam_el, *mu_el, *sy_el, *h_el; // per launch element, or null
+  const float2 *lm_f; // FP32 form: (float(lambda), float(mu)) per element, or null
   const int32_t *pos;     // [(pos_offset + i) * nnpe + a] -> entry of the
                           // node-major elemental-force table
   // Hex8, plain FP64: J^-1 and w det J of every Gauss point of the mesh,

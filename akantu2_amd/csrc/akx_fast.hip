@@ -183,8 +183,14 @@ template <class CT>
 __device__ __forceinline__ void element_params(const ElemArgs &A, int64_t i,
                                                CT &lambda, CT &mu, CT &sy,
                                                CT &hh) {
-  lambda = CT(A.lam_el ? A.lam_el[i] : A.lambda);
-  mu = CT(A.mu_el ? A.mu_el[i] : A.mu);
+  if (sizeof(CT) == 4 && A.lm_f) { // the same FP32 values, one 8-byte load
+    const float2 v = A.lm_f[i];
+    lambda = CT(v.x);
+    mu = CT(v.y);
+  } else {
+    lambda = CT(A.lam_el ? A.lam_el[i] : A.lambda);
+    mu = CT(A.mu_el ? A.mu_el[i] : A.mu);
+  }
   sy = CT(A.sy_el ? A.sy_el[i] : A.sigma_y);
   hh = CT(A.h_el ? A.h_el[i] : A.h);
 }

@@ -106,6 +106,13 @@ __global__ void k_fill_f64(double *p, int64_t n, double v) {
     p[i] = v;
 }
 
+// FP32 copies of the per-element Lame constants (the FP32 form's reads)
+__global__ void k_lame_f32(const double *lam, const double *mu, float2 *lm, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    lm[i] = make_float2(float(lam[i]), float(mu[i]));
+}
+
 // Lame constants per element from E, nu (material_elastic.cc:53-62)
 __global__ void k_lame(const double *E, const double *nu, double E0, double nu0,
                        double *lam, double *mu, int64_t n) {
@@ -465,6 +472,7 @@ struct Material {
   std::vector<int32_t> filter_host;
   DevBuf<int32_t> filter;
   DevBuf<double> lam_el, mu_el, sy_el, h_el;
+  DevBuf<float2> lm_f; // FP32 form: (float(lambda), float(mu)) per element
   bool per_element = false;
   // materialised internals and plastic history [2][9][n*nqp] / [2][n*nqp]
   DevBuf<double> grad, stress; // materialised by compute_all_stresses
@@ -646,6 +654,16 @@ void refresh_xu(akx_model *m, bool with_x) {
   HIPCHECK(hipGetLastError());
 }
 
+// FP32 form: (float(lambda), float(mu)) pairs of a material with per-element
+// constants, the values its element pass rounds the FP64 pair to
+void fill_lame_f32(akx_model *m, Material &mat) {
+  mat.lm_f.alloc(mat.n);
+  hipLaunchKernelGGL(k_lame_f32, dim3(grid_for(mat.n)), dim3(TPB), 0, m->stream,
+                     (const double *)mat.lam_el.p, (const double *)mat.mu_el.p,
+                     mat.lm_f.p, mat.n);
+  HIPCHECK(hipGetLastError());
+}
+
 void zero_async(akx_model *m, double *p, int64_t n) {
   if (n <= 0)
     return;
@@ -671,6 +689,7 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
   A.h = mat.h;
   A.lam_el = mat.per_element ? mat.lam_el.p : nullptr;
   A.mu_el = mat.per_element ? mat.mu_el.p : nullptr;
+  A.lm_f = mat.per_element ? mat.lm_f.p : nullptr;
   A.sy_el = mat.sy_el.p;
   A.h_el = mat.h_el.p;
   A.pos = m->pos.p;
@@ -1040,6 +1059,8 @@ void launch_element_part(akx_model *m, int p) {
   A.n = cnt;
   A.first = mat.first + e0;
   A.pos_offset = mat.pos_offset + e0;
+  if (A.lm_f)
+    A.lm_f += e0;
   if (A.lam_el) {
     A.lam_el += e0;
     A.mu_el += e0;
@@ -1407,8 +1428,10 @@ int akx_model_set_element_parameters(akx_model *m, int material_id,
                          (const double *)dE.p, (const double *)dnu.p, mat.E,
                          mat.nu, mat.lam_el.p, mat.mu_el.p, n);
       HIPCHECK(hipGetLastError());
-      HIPCHECK(hipStreamSynchronize(m->stream));
       mat.per_element = true;
+      if (m->precision == AKX_PRECISION_FP32)
+        fill_lame_f32(m, mat);
+      HIPCHECK(hipStreamSynchronize(m->stream));
     }
     if (sigma_y)
       up(mat.sy_el, sigma_y);
@@ -1514,6 +1537,9 @@ int akx_model_init_full(akx_model *m) {
     for (auto &mp : m->mats)
       alloc_internals(m, *mp);
     if (m->precision == AKX_PRECISION_FP32) {
+      for (auto &mp : m->mats)
+        if (mp->per_element && mp->n > 0)
+          fill_lame_f32(m, *mp);
       m->cu.alloc(3 * m->nn);
       m->cv.alloc(3 * m->nn);
       hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(3 * m->nn)), dim3(TPB), 0,
@@ -2402,8 +2428,8 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
     double extra = 0.;
     for (auto &mp : m->mats) {
       nel_all += mp->n;
-      if (mp->per_element)
-        extra += mp->n * 2.0 * 8.0; // lambda, mu per element (stored FP64)
+      if (mp->per_element) // lambda, mu per element: FP64, or the FP32 pair
+        extra += mp->n * (mp->lm_f.p ? 2.0 * 4.0 : 2.0 * 8.0);
       if (mp->law == AKX_LAW_PLASTIC_LIH)
         extra += (double)mp->n * m->nqp * (2 * (9 + 9 + 9 + 1) * 8.0); // read + write history
       if (!mp->contiguous)
