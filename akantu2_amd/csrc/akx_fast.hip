@@ -491,16 +491,26 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
 // ---------------------------------------------------------------------------
 // Tet4: one thread per element
 // ---------------------------------------------------------------------------
-template <int LAW, class CT, class FT, class VT, class NA, int W = 3>
+// EARLY_POS: load the table positions beside the connectivity and keep them
+// there (the scheduler otherwise places the load at its use, after the element
+// math, a third memory round trip per element).  Faster for the one-element-
+// per-thread FP32 pass; slower inside the persistent batch loop, and in FP64
+// it costs the pass a wave per SIMD.
+template <int LAW, class CT, class FT, class VT, class NA, int W = 3, bool EARLY_POS = false>
 __device__ __forceinline__ void tet4_one(const int32_t *conn_e, const NA &nodes_of,
                                          CT lambda, CT mu, CT sy, CT hh, const VT &V,
                                          int64_t iq, const int32_t *pos_e, FT *fe) {
   const int4 nd = *reinterpret_cast<const int4 *>(conn_e);
+  int4 pv;
+  if constexpr (EARLY_POS)
+    pv = *reinterpret_cast<const int4 *>(pos_e);
   const int32_t nodes[4] = {nd.x, nd.y, nd.z, nd.w};
   double X[4][3], U[4][3];
 #pragma unroll
   for (int n = 0; n < 4; ++n)
     nodes_of.load(nodes[n], X[n], U[n]);
+  if constexpr (EARLY_POS)
+    __builtin_amdgcn_sched_barrier(0);
   // natural directions: xi -> node 3, eta -> node 1, zeta -> node 2
   const int nm[3] = {3, 1, 2};
   CT J[3][3], H[3][3];
@@ -538,7 +548,8 @@ __device__ __forceinline__ void tet4_one(const int32_t *conn_e, const NA &nodes_
         s += P[k][J2] * Ji[J2][m];
       Q[k][m] = wJ * s;
     }
-  const int4 pv = *reinterpret_cast<const int4 *>(pos_e);
+  if constexpr (!EARLY_POS)
+    pv = *reinterpret_cast<const int4 *>(pos_e);
   akx::put_entry<W>(fe, (int64_t)pv.x, FT(-(Q[0][0] + Q[0][1] + Q[0][2])),
                     FT(-(Q[1][0] + Q[1][1] + Q[1][2])), FT(-(Q[2][0] + Q[2][1] + Q[2][2])));
   akx::put_entry<W>(fe, (int64_t)pv.y, FT(Q[0][1]), FT(Q[1][1]), FT(Q[2][1]));
@@ -554,7 +565,7 @@ __global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
   const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
   CT lambda, mu, sy, hh;
   element_params(A, i, lambda, mu, sy, hh);
-  tet4_one<LAW, CT, FT, QpViewT<double>, akx::NodesXU>(A.conn + e * 4, akx::NodesXU{A.xu}, lambda,
+  tet4_one<LAW, CT, FT, QpViewT<double>, akx::NodesXU, 3, sizeof(CT) == 4>(A.conn + e * 4, akx::NodesXU{A.xu}, lambda,
                                          mu, sy, hh,
                                 qp_view(A, 1), i, A.pos + (A.pos_offset + i) * 4,
                                 reinterpret_cast<FT *>(A.fe));
