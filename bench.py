@@ -88,7 +88,7 @@ def rel_errors(a, b):
 # plain form (headline)
 # --------------------------------------------------------------------------
 def host_threads():
-    """Host cores the CPU baseline may use (the box's share for one GPU)."""
+    """Host cores the CPU baseline may use (at most 16: a one-GPU host share)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -598,8 +598,8 @@ def main():
                     help="serve form: device memory the server may use")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is designed, not built)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 "
+                          "(sharding over several GPUs is designed, not implemented)"}))
         sys.exit(2)
     run = {"plain": run_plain, "precision": run_precision, "batch": run_batch,
            "serve": run_serve}[args.form]
