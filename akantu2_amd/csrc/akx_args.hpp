@@ -22,6 +22,12 @@ struct ElemArgs {
   double lambda, mu, sigma_y, h;
   const double *lam_el, *mu_el, *sy_el, *h_el; // per launch element, or null
   const float2 *lm_f; // FP32 form: (float(lambda), float(mu)) per element, or null
+  // FP32 Tet4: per 256-element block, the block's distinct nodes (bl_nodes
+  // [b * bl_maxu, (b + 1) * bl_maxu), padded with -1) and each element's
+  // corners as indices into them (lconn, [n][4]); null when not built
+  const int32_t *bl_nodes;
+  const uint16_t *lconn;
+  int32_t bl_maxu; // most distinct nodes of one block
   const int32_t *pos;     // [(pos_offset + i) * nnpe + a] -> entry of the
                           // node-major elemental-force table
   // Hex8, plain FP64: J^-1 and w det J of every Gauss point of the mesh,
@@ -106,6 +112,21 @@ __device__ __forceinline__ void put_entry(FT *fe, int64_t entry, FT a, FT b, FT 
     o[1] = make_double2(c, 0.);
   }
 }
+
+// {X, u} records staged in LDS, indexed by a block's distinct-node number
+struct NodesLDS {
+  const double *s; // [distinct nodes][6]
+  __device__ __forceinline__ void load(int32_t n, double (&x)[3], double (&d)[3]) const {
+    const double2 *p = reinterpret_cast<const double2 *>(s + 6 * n);
+    const double2 a = p[0], b = p[1], c = p[2];
+    x[0] = a.x;
+    x[1] = a.y;
+    x[2] = b.x;
+    d[0] = b.y;
+    d[1] = c.x;
+    d[2] = c.y;
+  }
+};
 
 struct NodesXU {
   const double *xu; // [n][6]

@@ -497,10 +497,9 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
 // per-thread FP32 pass; slower inside the persistent batch loop, and in FP64
 // it costs the pass a wave per SIMD.
 template <int LAW, class CT, class FT, class VT, class NA, int W = 3, bool EARLY_POS = false>
-__device__ __forceinline__ void tet4_one(const int32_t *conn_e, const NA &nodes_of,
+__device__ __forceinline__ void tet4_one(const int4 nd, const NA &nodes_of,
                                          CT lambda, CT mu, CT sy, CT hh, const VT &V,
                                          int64_t iq, const int32_t *pos_e, FT *fe) {
-  const int4 nd = *reinterpret_cast<const int4 *>(conn_e);
   int4 pv;
   if constexpr (EARLY_POS)
     pv = *reinterpret_cast<const int4 *>(pos_e);
@@ -565,10 +564,51 @@ __global__ void __launch_bounds__(256) k_tet4_fast(ElemArgs A) {
   const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
   CT lambda, mu, sy, hh;
   element_params(A, i, lambda, mu, sy, hh);
-  tet4_one<LAW, CT, FT, QpViewT<double>, akx::NodesXU, 3, sizeof(CT) == 4>(A.conn + e * 4, akx::NodesXU{A.xu}, lambda,
+  tet4_one<LAW, CT, FT, QpViewT<double>, akx::NodesXU, 3, sizeof(CT) == 4>(
+      *reinterpret_cast<const int4 *>(A.conn + e * 4), akx::NodesXU{A.xu}, lambda,
                                          mu, sy, hh,
                                 qp_view(A, 1), i, A.pos + (A.pos_offset + i) * 4,
                                 reinterpret_cast<FT *>(A.fe));
+}
+
+// FP32 Tet4 with the block's node records staged in LDS: the 256 elements of
+// a block share about 0.7 distinct nodes each, so each record is read from
+// global memory once per block instead of once per element corner.
+template <int LAW>
+__global__ void __launch_bounds__(256) k_tet4_lds(ElemArgs A) {
+  extern __shared__ double2 s_xu[]; // [bl_maxu][3]
+  const int64_t blk = xcd_block(A.nblocks_padded);
+  if (blk * 256 >= A.n) // padding block: no elements, no node list
+    return;
+  const int64_t i = blk * 256 + threadIdx.x;
+  const bool active = i < A.n;
+  // the element's own streams are loaded before the staging, so that their
+  // round trip overlaps it
+  ushort4 lc = make_ushort4(0, 0, 0, 0);
+  int4 pv = make_int4(0, 0, 0, 0);
+  float lambda = 0.f, mu = 0.f, sy = 0.f, hh = 0.f;
+  if (active) {
+    lc = *reinterpret_cast<const ushort4 *>(A.lconn + i * 4);
+    pv = *reinterpret_cast<const int4 *>(A.pos + (A.pos_offset + i) * 4);
+    element_params(A, i, lambda, mu, sy, hh);
+  }
+  const int32_t *list = A.bl_nodes + blk * A.bl_maxu;
+  for (int k = threadIdx.x; k < A.bl_maxu; k += 256) {
+    const int32_t node = list[k];
+    if (node >= 0) {
+      const double2 *p = reinterpret_cast<const double2 *>(A.xu + 6 * (int64_t)node);
+      s_xu[3 * k] = p[0];
+      s_xu[3 * k + 1] = p[1];
+      s_xu[3 * k + 2] = p[2];
+    }
+  }
+  __syncthreads();
+  if (!active)
+    return;
+  tet4_one<LAW, float, float, QpViewT<double>, akx::NodesLDS, 3, true>(
+      make_int4(lc.x, lc.y, lc.z, lc.w), akx::NodesLDS{reinterpret_cast<const double *>(s_xu)},
+      lambda, mu, sy, hh, qp_view(A, 1), i, reinterpret_cast<const int32_t *>(&pv),
+      reinterpret_cast<float *>(A.fe));
 }
 
 // ---------------------------------------------------------------------------
@@ -586,7 +626,8 @@ __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
     const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
     for (int j = threadIdx.x; j < T.count; j += blockDim.x) {
       const int64_t e = T.first + j;
-      tet4_one<LAW, float, float, QpCompact, NodesXU, FE_W>(A.conn + e * 4, NodesXU{A.xu},
+      tet4_one<LAW, float, float, QpCompact, NodesXU, FE_W>(
+          *reinterpret_cast<const int4 *>(A.conn + e * 4), NodesXU{A.xu},
                                          float(P.lambda), float(P.mu), float(P.sigma_y),
                                          float(P.h), V, A.hoff + e, A.pos + e * 4,
                                          reinterpret_cast<float *>(A.fe));
@@ -626,8 +667,14 @@ int64_t elements_per_block(int etype) { return etype == 1 ? HEX_EPB : 256; }
 template <int L, class CT, class FT>
 static void launch_one(int etype, const ElemArgs &A, hipStream_t s) {
   dim3 grid((unsigned)A.nblocks_padded), block(256);
-  if (etype != 1)
+  if (etype != 1) {
+    if constexpr (sizeof(CT) == 4)
+      if (A.lconn) {
+        hipLaunchKernelGGL((k_tet4_lds<L>), grid, block, (size_t)A.bl_maxu * 48, s, A);
+        return;
+      }
     hipLaunchKernelGGL((k_tet4_fast<L, CT, FT>), grid, block, 0, s, A);
+  }
   else if constexpr (sizeof(CT) == 4) // FP32: sum-factorised Hex8
     hipLaunchKernelGGL((k_hex8_sf<L, FT>), grid, block, 0, s, A);
   else

@@ -473,6 +473,10 @@ struct Material {
   DevBuf<int32_t> filter;
   DevBuf<double> lam_el, mu_el, sy_el, h_el;
   DevBuf<float2> lm_f; // FP32 form: (float(lambda), float(mu)) per element
+  // FP32 Tet4: distinct nodes of each 256-element block and local corners
+  DevBuf<int32_t> bl_nodes;
+  DevBuf<uint16_t> lconn;
+  int32_t bl_maxu = 0;
   bool per_element = false;
   // materialised internals and plastic history [2][9][n*nqp] / [2][n*nqp]
   DevBuf<double> grad, stress; // materialised by compute_all_stresses
@@ -654,6 +658,47 @@ void refresh_xu(akx_model *m, bool with_x) {
   HIPCHECK(hipGetLastError());
 }
 
+// FP32 Tet4 element pass: for each block of 256 consecutive elements of a
+// contiguous material, its distinct nodes in order of first use (at most
+// 1,024; every block's list padded with -1 to the longest) and each
+// element's corners as indices into them
+void build_block_nodes(akx_model *m, Material &mat) {
+  const int64_t n = mat.n, nb = (n + 255) / 256;
+  std::vector<int32_t> off(nb + 1, 0), nodes;
+  std::vector<uint16_t> lc(n * 4);
+  std::vector<int64_t> stamp(m->nn, -1);
+  std::vector<int32_t> local(m->nn, 0);
+  nodes.reserve(n);
+  int32_t maxu = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t start = (int64_t)nodes.size();
+    for (int64_t i = b * 256; i < std::min<int64_t>(n, b * 256 + 256); ++i)
+      for (int a = 0; a < 4; ++a) {
+        const int32_t node = m->conn_host[(mat.first + i) * 4 + a];
+        if (stamp[node] != b) {
+          stamp[node] = b;
+          local[node] = (int32_t)((int64_t)nodes.size() - start);
+          nodes.push_back(node);
+        }
+        lc[i * 4 + a] = (uint16_t)local[node];
+      }
+    require((int64_t)nodes.size() < (int64_t)INT32_MAX, "block node lists too large");
+    off[b + 1] = (int32_t)nodes.size();
+    maxu = std::max<int32_t>(maxu, off[b + 1] - off[b]);
+  }
+  std::vector<int32_t> padded((size_t)nb * maxu, -1);
+  for (int64_t b = 0; b < nb; ++b)
+    std::copy(nodes.begin() + off[b], nodes.begin() + off[b + 1],
+              padded.begin() + b * maxu);
+  mat.bl_nodes.alloc(padded.size());
+  mat.lconn.alloc(n * 4);
+  HIPCHECK(hipMemcpy(mat.bl_nodes.p, padded.data(), padded.size() * sizeof(int32_t),
+                     hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(mat.lconn.p, lc.data(), lc.size() * sizeof(uint16_t),
+                     hipMemcpyHostToDevice));
+  mat.bl_maxu = maxu;
+}
+
 // FP32 form: (float(lambda), float(mu)) pairs of a material with per-element
 // constants, the values its element pass rounds the FP64 pair to
 void fill_lame_f32(akx_model *m, Material &mat) {
@@ -690,6 +735,9 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
   A.lam_el = mat.per_element ? mat.lam_el.p : nullptr;
   A.mu_el = mat.per_element ? mat.mu_el.p : nullptr;
   A.lm_f = mat.per_element ? mat.lm_f.p : nullptr;
+  A.bl_nodes = mat.bl_nodes.p;
+  A.lconn = mat.lconn.p;
+  A.bl_maxu = mat.bl_maxu;
   A.sy_el = mat.sy_el.p;
   A.h_el = mat.h_el.p;
   A.pos = m->pos.p;
@@ -1061,6 +1109,7 @@ void launch_element_part(akx_model *m, int p) {
   A.pos_offset = mat.pos_offset + e0;
   if (A.lm_f)
     A.lm_f += e0;
+  A.lconn = nullptr; // the block node lists follow the whole range's blocks
   if (A.lam_el) {
     A.lam_el += e0;
     A.mu_el += e0;
@@ -1537,9 +1586,12 @@ int akx_model_init_full(akx_model *m) {
     for (auto &mp : m->mats)
       alloc_internals(m, *mp);
     if (m->precision == AKX_PRECISION_FP32) {
-      for (auto &mp : m->mats)
+      for (auto &mp : m->mats) {
         if (mp->per_element && mp->n > 0)
           fill_lame_f32(m, *mp);
+        if (m->etype == AKX_TET4 && mp->contiguous && mp->n > 0)
+          build_block_nodes(m, *mp);
+      }
       m->cu.alloc(3 * m->nn);
       m->cv.alloc(3 * m->nn);
       hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(3 * m->nn)), dim3(TPB), 0,
@@ -2430,6 +2482,9 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
       nel_all += mp->n;
       if (mp->per_element) // lambda, mu per element: FP64, or the FP32 pair
         extra += mp->n * (mp->lm_f.p ? 2.0 * 4.0 : 2.0 * 8.0);
+      if (mp->lconn.p) // 2-byte local corners instead of 4-byte node ids, plus
+                       // the blocks' node lists
+        extra += -2.0 * m->nnpe * mp->n + 4.0 * (double)mp->bl_nodes.n;
       if (mp->law == AKX_LAW_PLASTIC_LIH)
         extra += (double)mp->n * m->nqp * (2 * (9 + 9 + 9 + 1) * 8.0); // read + write history
       if (!mp->contiguous)

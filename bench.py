@@ -324,7 +324,7 @@ def run_precision(args):
                    "elements": mesh.nb_elements, "nodes": mesh.nb_nodes},
         "roofline": {"bound": "hbm", "achieved": el["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": el["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_tet4_fast<elastic, f32>",
+                     "traffic": None, "kernel": "k_tet4_lds<elastic> (FP32, node records staged in LDS)",
                      "algorithmic_bytes_per_launch": el["bytes"],
                      "avg_kernel_ms": el["avg_ms"]},
         "forms": res,

@@ -90,3 +90,23 @@ def test_fp32_form_every_law(etype, law, amp, mat):
     if law == PL:
         assert o.iso.max() > 0
 
+
+
+def test_fp32_tet4_staged_pass_equals_per_element_pass():
+    """The FP32 Tet4 element pass of a whole contiguous material stages each
+    256-element block's node records in LDS; the pipelined schedule's partial
+    launches read the records per element instead.  Same arithmetic, so the
+    two give the same bits (per-element E and nu, several step counts)."""
+    mesh, blocked, v0 = workload("_tetrahedron_4", (30, 7, 5), jitter=0.1, seed=4)
+    rng = np.random.default_rng(4)
+    E = np.exp(rng.uniform(np.log(0.5), np.log(2.0), mesh.nb_elements))
+    nu = rng.uniform(0.2, 0.35, mesh.nb_elements)
+    staged = make_gpu(mesh, blocked, v0, "elastic", precision="fp32", E_el=E, nu_el=nu)[0]
+    parts = make_gpu(mesh, blocked, v0, "elastic", precision="fp32", E_el=E, nu_el=nu,
+                     dt=staged.getTimeStep())[0]
+    assert parts.setPipeline(3)
+    for n in (1, 2, 40):
+        staged.solveStep(n)
+        parts.solveStep(n)
+        for get in ("getDisplacement", "getVelocity", "getAcceleration", "getInternalForce"):
+            assert np.array_equal(getattr(staged, get)(), getattr(parts, get)()), (get, n)
