@@ -113,6 +113,23 @@ __device__ __forceinline__ void put_entry(FT *fe, int64_t entry, FT a, FT b, FT 
   }
 }
 
+// Tet4 passes with staged records: copies the {X, u} records of block blk's
+// distinct nodes into LDS (three 16-byte loads and stores per node).  The
+// caller synchronises the block before reading them.
+__device__ __forceinline__ void stage_block_records(const ElemArgs &A, int64_t blk,
+                                                    double2 *s) {
+  const int32_t *list = A.bl_nodes + blk * A.bl_maxu;
+  for (int k = threadIdx.x; k < A.bl_maxu; k += blockDim.x) {
+    const int32_t node = list[k];
+    if (node >= 0) {
+      const double2 *p = reinterpret_cast<const double2 *>(A.xu + 6 * (int64_t)node);
+      s[3 * k] = p[0];
+      s[3 * k + 1] = p[1];
+      s[3 * k + 2] = p[2];
+    }
+  }
+}
+
 // {X, u} records staged in LDS, indexed by a block's distinct-node number
 struct NodesLDS {
   const double *s; // [distinct nodes][6]

@@ -583,7 +583,8 @@ __global__ void __launch_bounds__(256) k_tet4_lds(ElemArgs A) {
   const int64_t i = blk * 256 + threadIdx.x;
   const bool active = i < A.n;
   // the element's own streams are loaded before the staging, so that their
-  // round trip overlaps it
+  // round trip overlaps it (loading the last element's for the lanes past
+  // the end instead, without the branch, measured 2.5% slower here)
   ushort4 lc = make_ushort4(0, 0, 0, 0);
   int4 pv = make_int4(0, 0, 0, 0);
   float lambda = 0.f, mu = 0.f, sy = 0.f, hh = 0.f;
@@ -592,16 +593,7 @@ __global__ void __launch_bounds__(256) k_tet4_lds(ElemArgs A) {
     pv = *reinterpret_cast<const int4 *>(A.pos + (A.pos_offset + i) * 4);
     element_params(A, i, lambda, mu, sy, hh);
   }
-  const int32_t *list = A.bl_nodes + blk * A.bl_maxu;
-  for (int k = threadIdx.x; k < A.bl_maxu; k += 256) {
-    const int32_t node = list[k];
-    if (node >= 0) {
-      const double2 *p = reinterpret_cast<const double2 *>(A.xu + 6 * (int64_t)node);
-      s_xu[3 * k] = p[0];
-      s_xu[3 * k + 1] = p[1];
-      s_xu[3 * k + 2] = p[2];
-    }
-  }
+  akx::stage_block_records(A, blk, s_xu);
   __syncthreads();
   if (!active)
     return;

@@ -511,6 +511,43 @@ k_element_forces(ElemArgs A) {
 }
 
 
+// Tet4 in the reference's operation order with each 256-element block's node
+// records staged in LDS (the block lists of initFull): the same arithmetic
+// as k_element_forces on the same values, read from LDS instead of once per
+// element corner from global memory.
+template <int LAW, class CT, class FT>
+__global__ void __launch_bounds__(256) k_tet4_strict_lds(ElemArgs A) {
+  extern __shared__ double2 s_rec[]; // [bl_maxu][3]
+  const int64_t blk = xcd_block(A.nblocks_padded);
+  if (blk * 256 >= A.n) // padding block: no elements, no node list
+    return;
+  const int64_t i = blk * 256 + threadIdx.x;
+  const bool active = i < A.n;
+  // The element's own streams are loaded first, their round trip overlapping
+  // the staging; lanes past the end read the last element (no branch, so no
+  // load is awaited before the staging's loads are issued).
+  const int64_t ic = active ? i : A.n - 1;
+  const ushort4 l = *reinterpret_cast<const ushort4 *>(A.lconn + ic * 4);
+  const int4 pv = *reinterpret_cast<const int4 *>(A.pos + (A.pos_offset + ic) * 4);
+  CT lambda = CT(A.lambda), mu = CT(A.mu), sy = CT(A.sigma_y), hh = CT(A.h);
+  if (A.lam_el) {
+    lambda = CT(A.lam_el[ic]);
+    mu = CT(A.mu_el[ic]);
+  }
+  if (A.sy_el)
+    sy = CT(A.sy_el[ic]);
+  if (A.h_el)
+    hh = CT(A.h_el[ic]);
+  stage_block_records(A, blk, s_rec);
+  __syncthreads();
+  if (!active)
+    return;
+  const int4 lc = make_int4(l.x, l.y, l.z, l.w);
+  element_forces_one<ET_TET4, LAW, CT, FT, double>(
+      reinterpret_cast<const int32_t *>(&lc), NodesLDS{reinterpret_cast<const double *>(s_rec)},
+      lambda, mu, sy, hh, qp_view(A, 1), i, reinterpret_cast<const int32_t *>(&pv), A.fe);
+}
+
 // ---------------------------------------------------------------------------
 // Hex8 element pass in the reference's operation order, one lane per Gauss
 // point (8 lanes per element, EPB elements per block).  Nodal values are

@@ -798,6 +798,35 @@ void check_elem_args(const ElemArgs &A, int law) {
 template <int ET, class CT, class FT>
 void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
   dim3 grid((unsigned)A.nblocks_padded), block(TPB);
+  if constexpr (ET == ET_TET4) {
+    if (A.lconn && TPB == 256) { // block node lists of a whole contiguous material
+      const size_t lds = (size_t)A.bl_maxu * 48;
+      switch (mat.law) {
+      case AKX_LAW_ELASTIC:
+        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_ELASTIC, CT, FT>), grid, block, lds,
+                           m->stream, A);
+        HIPCHECK(hipGetLastError());
+        return;
+      case AKX_LAW_ELASTIC_FD:
+        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_ELASTIC_FD, CT, FT>), grid, block, lds,
+                           m->stream, A);
+        HIPCHECK(hipGetLastError());
+        return;
+      case AKX_LAW_NEOHOOKEAN:
+        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_NEOHOOKEAN, CT, FT>), grid, block, lds,
+                           m->stream, A);
+        HIPCHECK(hipGetLastError());
+        return;
+      case AKX_LAW_PLASTIC_LIH:
+        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_PLASTIC, CT, FT>), grid, block, lds,
+                           m->stream, A);
+        HIPCHECK(hipGetLastError());
+        return;
+      default: // user laws: the per-element pass
+        break;
+      }
+    }
+  }
   switch (mat.law) {
   case AKX_LAW_ELASTIC:
     hipLaunchKernelGGL((k_element_forces<ET, LAW_ELASTIC, CT, FT>), grid, block,
@@ -1555,6 +1584,9 @@ int akx_model_init_full(akx_model *m) {
     if (m->etype == AKX_TET4 && m->nn > 0) {
       m->xu.alloc(6 * m->nn);
       refresh_xu(m, true);
+      for (auto &mp : m->mats) // Tet4 passes with staged node records
+        if (mp->contiguous && mp->n > 0 && m->precision != AKX_PRECISION_FP64_FAST)
+          build_block_nodes(m, *mp);
     }
     build_gather_table(m);
     m->fe.alloc(std::max<int64_t>(m->ell_total, 1) * 3 * m->fe_elem_size());
@@ -1586,12 +1618,9 @@ int akx_model_init_full(akx_model *m) {
     for (auto &mp : m->mats)
       alloc_internals(m, *mp);
     if (m->precision == AKX_PRECISION_FP32) {
-      for (auto &mp : m->mats) {
+      for (auto &mp : m->mats)
         if (mp->per_element && mp->n > 0)
           fill_lame_f32(m, *mp);
-        if (m->etype == AKX_TET4 && mp->contiguous && mp->n > 0)
-          build_block_nodes(m, *mp);
-      }
       m->cu.alloc(3 * m->nn);
       m->cv.alloc(3 * m->nn);
       hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(3 * m->nn)), dim3(TPB), 0,
