@@ -72,7 +72,8 @@ def _declare(lib):
         "akx_model_init_full": (C.c_int, [_vp]),
         "akx_model_get_stable_time_step": (C.c_int, [_vp, _dp]),
         "akx_model_set_time_step": (C.c_int, [_vp, C.c_double]),
-        "akx_model_solve_step": (C.c_int, [_vp, _i64]),
+        "akx_model_solve_step": (C.c_int, [_vp, C.c_char_p]),
+        "akx_model_solve_steps": (C.c_int, [_vp, _i64]),
         "akx_model_assemble_internal_forces": (C.c_int, [_vp]),
         "akx_model_get_nb_nodes": (C.c_int, [_vp, C.POINTER(_i64)]),
         "akx_model_get_field": (C.c_int, [_vp, C.c_int, _vp]),

@@ -1422,9 +1422,16 @@ int akx_material_law_by_name(const char *name, int finite_deformation, int *law)
       *law = finite_deformation ? AKX_LAW_ELASTIC_FD : AKX_LAW_ELASTIC;
     else if (s == "neohookean")
       *law = AKX_LAW_NEOHOOKEAN;
-    else if (s == "plastic_linear_isotropic_hardening")
+    else if (s == "plastic_linear_isotropic_hardening") {
+      // The reference's finite-deformation branch (material_linear_isotropic
+      // _hardening.cc:36-52, _inline_impl.hh:88-186) is not provided: refuse
+      // it rather than run the small-strain law in its place.
+      if (finite_deformation)
+        throw Error(AKX_ERR_UNSUPPORTED,
+                    "plastic_linear_isotropic_hardening with finite_deformation = true "
+                    "is not provided (small-strain plasticity only)");
       *law = AKX_LAW_PLASTIC_LIH;
-    else
+    } else
       throw Error(AKX_ERR_UNSUPPORTED, "unknown material law: " + s);
   });
 }
@@ -1673,7 +1680,7 @@ int akx_model_set_time_step(akx_model *m, double dt) {
   });
 }
 
-int akx_model_solve_step(akx_model *m, int64_t nb_steps) {
+int akx_model_solve_steps(akx_model *m, int64_t nb_steps) {
   return guard([&] {
     check_ready(m);
     require(m->dt > 0, "setTimeStep has not been called");
@@ -1705,6 +1712,21 @@ int akx_model_solve_step(akx_model *m, int64_t nb_steps) {
       launch_gather(m, true, s + 1 < nb_steps);
     }
   });
+}
+
+// ModelSolver::solveStep(solver_id) (model_solver.cc:267-280): "" names the
+// default solver, which initFull(_explicit_lumped_mass) registers as
+// "explicit_lumped" (solid_mechanics_model.cc:165-169); any other id is not a
+// solver of this model and fails as DOFManager::getTimeStepSolver does
+// (dof_manager.cc:585-596).
+int akx_model_solve_step(akx_model *m, const char *solver_id) {
+  const int st = guard([&] {
+    const std::string id = solver_id ? solver_id : "";
+    if (!id.empty() && id != "explicit_lumped")
+      throw Error(AKX_ERR_INVALID, "The non linear solver dof_manager:tss:" + id +
+                                       " does not exists in dof_manager");
+  });
+  return st != AKX_OK ? st : akx_model_solve_steps(m, 1);
 }
 
 int akx_model_set_pipeline(akx_model *m, int parts, int *active) {

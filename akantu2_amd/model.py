@@ -3,11 +3,15 @@ Python API (python/py_solid_mechanics_model.cc:57-173, py_model.cc:131-139,
 py_material.cc:45-80): ``SolidMechanicsModel(mesh)``, ``initFull``,
 ``getStableTimeStep``, ``setTimeStep``, ``solveStep``, ``getDisplacement`` ...
 
-Nodal arrays are numpy arrays in the reference's [nb_nodes][3] layout.  The
-arrays returned by the getters are host mirrors of device-resident fields:
-edits made to them are uploaded before the next ``solveStep`` (the reference
-lets users write through ``getDisplacement()``), and after a step the next
-getter call refreshes the mirror in place.
+Nodal arrays are numpy arrays in the reference's [nb_nodes][3] layout.  A
+getter returns a view of a host copy of the device-resident field, and the
+copy behaves like the reference's live ``Array`` (solid_mechanics_model.hh
+:440-468): while any array referring to it is alive, every call that changes
+the device field refreshes the copy in place, and edits made through it are
+uploaded before the next call that reads the field (the reference lets users
+write through ``getDisplacement()``).  A copy nothing refers to any more is
+dropped, so a ``solveStep`` with no array held costs O(1) on the host; every
+array held across steps costs one device-to-host copy per step.
 
 User constitutive laws subclass :class:`Material` and register with
 :class:`MaterialFactory`, as in the reference; ``getFEEngine()`` and
@@ -17,6 +21,7 @@ elemental assembly on caller fields.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import weakref
 
 import numpy as np
@@ -47,6 +52,47 @@ _LAWS = {"elastic": N.LAW_ELASTIC, "elastic_fd": N.LAW_ELASTIC_FD,
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+_libc = C.CDLL(None)
+_libc.memcmp.restype = C.c_int
+_libc.memcmp.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+_SOLVER_IDS = ("", "explicit_lumped")
+
+
+def _check_solver_id(solver_id):
+    """ModelSolver::getSolver (model_solver.cc:201-209): "" is the default
+    solver, which initFull(_explicit_lumped_mass) names "explicit_lumped"
+    (solid_mechanics_model.cc:165-169)."""
+    if not isinstance(solver_id, str):
+        raise TypeError(f"solver_id must be a str, not {type(solver_id).__name__} "
+                        "(solveSteps(n) runs n steps)")
+    if solver_id not in _SOLVER_IDS:
+        raise RuntimeError(f"The non linear solver dof_manager:tss:{solver_id} does not "
+                           "exists in dof_manager")
+
+
+class _Mirror:
+    """Host copy of one device field.  Every array a getter hands out is a
+    view of ``base`` (numpy points a view's ``.base`` at the owner of the
+    memory, also for views of views), so ``base`` has references beyond this
+    object exactly while some handed-out array is alive."""
+
+    __slots__ = ("base", "snap", "view", "writable")
+
+    def __init__(self, base, writable):
+        self.base = base
+        self.writable = writable
+        self.snap = base.copy() if writable else None
+        self.view = None  # weak reference to the last view handed out
+
+    def held(self):
+        # this object's reference + getrefcount's argument
+        return sys.getrefcount(self.base) > 2
+
+    def edited(self):
+        return self.writable and _libc.memcmp(self.base.ctypes.data, self.snap.ctypes.data,
+                                              self.base.nbytes) != 0
 
 
 class MaterialFactory:
@@ -244,8 +290,6 @@ class SolidMechanicsModel:
         self.nb_nodes = nodes.shape[0]
         self.materials = []
         self._mirrors = {}
-        self._snap = {}
-        self._stale = set()
         self._initialized = False
         self._stresses_fresh = False
         self.time_step = 0.0
@@ -276,9 +320,17 @@ class SolidMechanicsModel:
                                            elements, user_params)
         if user_params:
             raise TypeError(f"unknown parameters {sorted(user_params)} for {law!r}")
-        if law == "elastic" and finite_deformation:
-            law = "elastic_fd"
-        lid = _LAWS[law] if isinstance(law, str) else int(law)
+        if law == "elastic_fd":
+            law, finite_deformation = "elastic", True
+        if isinstance(law, str):
+            # the factory lookup of the C ABI: unknown names and the finite-
+            # deformation plastic branch are refused there
+            lid_c = C.c_int(-1)
+            N.check(N.lib().akx_material_law_by_name(law.encode(), int(bool(finite_deformation)),
+                                                     C.byref(lid_c)))
+            lid = lid_c.value
+        else:
+            lid = int(law)
         params = np.array([rho, E, nu, sigma_y, h], np.float64)
         mid = C.c_int(-1)
         filt = None if elements is None else np.ascontiguousarray(elements, np.int32)
@@ -388,56 +440,87 @@ class SolidMechanicsModel:
         N.check(N.lib().akx_model_get_stable_time_step(self._h, C.byref(dt)))
         return dt.value
 
-    def setTimeStep(self, dt: float):
+    def setTimeStep(self, dt: float, solver_id: str = ""):
+        """ModelSolver::setTimeStep(time_step, solver_id) (model_solver.cc:323)."""
+        _check_solver_id(solver_id)
         N.check(N.lib().akx_model_set_time_step(self._h, float(dt)))
         self.time_step = float(dt)
 
-    def getTimeStep(self) -> float:
+    def getTimeStep(self, solver_id: str = "") -> float:
+        _check_solver_id(solver_id)
         return self.time_step
 
     # ---------------------------------------------------------------- steps
-    def solveStep(self, nb_steps: int = 1):
-        """``nb_steps`` consecutive ModelSolver::solveStep() calls."""
+    def solveStep(self, solver_id: str = ""):
+        """ModelSolver::solveStep(solver_id="") (model_solver.cc:267-280,
+        py_model.cc:130-135): one explicit step of the default solver, which
+        ``"explicit_lumped"`` also names."""
+        _check_solver_id(solver_id)
         self._push()
-        self._run(N.lib().akx_model_solve_step, int(nb_steps))
-        self._invalidate()
+        self._run(N.lib().akx_model_solve_step, solver_id.encode())
+        self._refresh()
+
+    def solveSteps(self, nb_steps: int):
+        """``nb_steps`` consecutive ``solveStep()`` calls in one library call
+        (no counterpart in the reference): the same results bit for bit, with
+        the next step's predictor fused into each step's nodal gather."""
+        self._push()
+        self._run(N.lib().akx_model_solve_steps, int(nb_steps))
+        self._refresh()
 
     def assembleInternalForces(self):
         self._push()
         self._run(N.lib().akx_model_assemble_internal_forces)
-        self._invalidate()
+        self._refresh()
 
     def synchronize(self):
         N.check(N.lib().akx_model_synchronize(self._h))
 
     # --------------------------------------------------------------- fields
-    def _invalidate(self):
-        self._stale = set(self._mirrors)
-        self._stresses_fresh = False
-
     def _push(self):
-        for name, arr in self._mirrors.items():
-            if name in _WRITABLE and not np.array_equal(arr, self._snap[name]):
-                a = np.ascontiguousarray(arr)
-                N.check(N.lib().akx_model_set_field(self._h, _FIELDS[name], _p(a)))
-                self._snap[name] = arr.copy()
+        """Upload the edits made through the arrays handed out, and forget
+        the copies nothing refers to any more.  A copy is current with the
+        device whenever an edit is made through it (``_refresh`` runs after
+        every call that changes a field), so uploading it whole is exact."""
+        for name in list(self._mirrors):
+            mir = self._mirrors[name]
+            if mir.edited():
+                N.check(N.lib().akx_model_set_field(self._h, _FIELDS[name], _p(mir.base)))
+                np.copyto(mir.snap, mir.base)
                 self._stresses_fresh = False
+            if not mir.held():
+                del self._mirrors[name]
+
+    def _refresh(self, names=None):
+        """After a call that changed device fields: bring the held copies up
+        to date in place (all of them, or those in ``names``)."""
+        self._stresses_fresh = False
+        for name in list(self._mirrors):
+            if names is not None and name not in names:
+                continue
+            mir = self._mirrors[name]
+            if not mir.held():
+                del self._mirrors[name]
+                continue
+            N.check(N.lib().akx_model_get_field(self._h, _FIELDS[name], _p(mir.base)))
+            if mir.writable:
+                np.copyto(mir.snap, mir.base)
 
     def _get(self, name):
-        self._push()
-        dtype = np.uint8 if name == "blocked_dofs" else np.float64
-        arr = self._mirrors.get(name)
-        if arr is None or name in self._stale:
-            tmp = np.empty((self.nb_nodes, 3), dtype)
-            N.check(N.lib().akx_model_get_field(self._h, _FIELDS[name], _p(tmp)))
-            if arr is None:
-                arr = tmp
-                self._mirrors[name] = arr
-            else:
-                arr[...] = tmp
-            self._snap[name] = arr.copy()
-            self._stale.discard(name)
-        return arr
+        mir = self._mirrors.get(name)
+        if mir is None:
+            dtype = np.uint8 if name == "blocked_dofs" else np.float64
+            base = np.empty((self.nb_nodes, 3), dtype)
+            N.check(N.lib().akx_model_get_field(self._h, _FIELDS[name], _p(base)))
+            mir = _Mirror(base, name in _WRITABLE)
+            self._mirrors[name] = mir
+        view = mir.view() if mir.view is not None else None
+        if view is None:
+            view = mir.base.view()
+            if not mir.writable:
+                view.flags.writeable = False
+            mir.view = weakref.ref(view)
+        return view
 
     def getDisplacement(self):
         return self._get("displacement")
@@ -494,8 +577,7 @@ class SolidMechanicsModel:
             N.check(N.lib().akx_model_apply_traction(
                 self._h, int(group["facet_type"]), f.shape[0], _p(f),
                 t.ctypes.data_as(C.POINTER(C.c_double))))
-            self._stale.add("external_force")
-            self._mirrors.pop("external_force", None)
+            self._refresh({"external_force"})
             return
         nodes = np.asarray(group["nodes"])
         bc.apply(self.getBlockedDOFs(), self.getDisplacement(), nodes)

@@ -150,10 +150,10 @@ def run_plain(args):
     schedule = "plain"
     if args.pipeline > 1 and model.setPipeline(args.pipeline):
         schedule = f"pipelined, {args.pipeline} parts"
-    model.solveStep(args.warmup)
+    model.solveSteps(args.warmup)
     model.synchronize()
     t0 = time.perf_counter()
-    model.solveStep(steps)
+    model.solveSteps(steps)
     model.synchronize()
     elapsed = time.perf_counter() - t0
     nqp = mesh.nb_elements * 8
@@ -162,7 +162,7 @@ def run_plain(args):
     # per-kernel times: a separate pass with HIP events around every launch
     # (plain schedule), so the timed loop above carries no event overhead
     model.enableKernelTiming(True)
-    model.solveStep(min(steps, 100))
+    model.solveSteps(min(steps, 100))
     model.synchronize()
 
     kern = {}
@@ -280,14 +280,14 @@ def _timed_model(mesh, blocked, v0, E, nu, precision, warmup, steps):
     m.getBlockedDOFs()[:] = blocked
     m.getVelocity()[:] = v0
     m.setTimeStep(0.5 * m.getStableTimeStep())
-    m.solveStep(warmup)
+    m.solveSteps(warmup)
     m.synchronize()
     t0 = time.perf_counter()
-    m.solveStep(steps)
+    m.solveSteps(steps)
     m.synchronize()
     el = time.perf_counter() - t0
     m.enableKernelTiming(True)
-    m.solveStep(10)
+    m.solveSteps(10)
     m.synchronize()
     kern = {}
     for k, name in {0: "element_pass", 1: "gather_update"}.items():
@@ -357,7 +357,7 @@ def cpu_baseline_precision(shape=(60, 10, 10), steps=200):
     g.getBlockedDOFs()[:] = blocked
     g.getVelocity()[:] = v0
     g.setTimeStep(dt)
-    g.solveStep(steps)
+    g.solveSteps(steps)
     err = {"u": rel_errors(g.getDisplacement(), o.u), "v": rel_errors(g.getVelocity(), o.v),
            "f_int": rel_errors(g.getInternalForce(), o.f_int)}
     return {"value": mesh.nb_elements * steps / el, "unit": "elem*qp/s", "cores": 1,

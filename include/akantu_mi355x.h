@@ -130,7 +130,9 @@ int akx_model_set_element_parameters(akx_model *m, int material_id,
                                      const double *sigma_y, const double *h);
 /* Look a law up by the reference's material name ("elastic", "neohookean",
  * "plastic_linear_isotropic_hardening"); finite_deformation selects the total
- * Lagrangian branch of "elastic". */
+ * Lagrangian branch of "elastic".  "plastic_linear_isotropic_hardening" with
+ * finite_deformation != 0 (material_linear_isotropic_hardening.cc:36-52) is
+ * not provided and returns AKX_ERR_UNSUPPORTED; unknown names too. */
 int akx_material_law_by_name(const char *name, int finite_deformation,
                              int *law);
 
@@ -189,8 +191,18 @@ int akx_model_init_full(akx_model *m);
 /* Time stepping -------------------------------------------------------------*/
 int akx_model_get_stable_time_step(akx_model *m, double *dt);
 int akx_model_set_time_step(akx_model *m, double dt);
-/* nb_steps consecutive solveStep() calls. */
-int akx_model_solve_step(akx_model *m, int64_t nb_steps);
+/* ModelSolver::solveStep(const ID & solver_id = "") (model_solver.hh:78,
+ * model_solver.cc:267-280): one explicit step.  solver_id NULL or "" is the
+ * default solver, "explicit_lumped" names it (the id initFull(
+ * _explicit_lumped_mass) registers, solid_mechanics_model.cc:165-169); any
+ * other id fails with AKX_ERR_INVALID, as DOFManager::getTimeStepSolver
+ * throws for a solver that does not exist (dof_manager.cc:585-596).  The
+ * step is queued on the model's stream; a following field read waits for it. */
+int akx_model_solve_step(akx_model *m, const char *solver_id);
+/* nb_steps consecutive solveStep() calls in one call, no counterpart in the
+ * reference: bit-identical to nb_steps calls of akx_model_solve_step, with the
+ * next step's predictor fused into each step's nodal gather. */
+int akx_model_solve_steps(akx_model *m, int64_t nb_steps);
 /* SolidMechanicsModel::assembleInternalForces on the current displacement
  * (f_int only, no time integration). */
 int akx_model_assemble_internal_forces(akx_model *m);

@@ -9,12 +9,17 @@
 //   for (int s = 0; s < 1000; ++s) model.solveStep();
 //   const auto & u = model.getDisplacement();   // [nb_nodes][3]
 //
-// Arrays are host mirrors of the device fields in the reference's layout
-// (Array<Real>: one tuple of nb_component values per row).  Mutable getters
-// mark the field dirty; the next solveStep uploads it (the reference lets
-// users write through getDisplacement()).  Errors throw akantu_mi355x::Exception
-// (the reference throws debug::Exception; a negative jacobian throws
-// JacobianException like IntegratorGaussException).
+// The nodal Arrays the getters return are host copies of the device fields
+// in the reference's layout (Array<Real>: one tuple of nb_component values per
+// row) that behave like the reference's live Arrays (solid_mechanics_model.hh
+// :440-468) when held across steps: a call that changes the device fields only
+// marks the copies stale (O(1)); the next element access of a stale copy
+// downloads it, and a non-const access marks it edited, so that the next
+// call that reads the field uploads it (the reference lets users write through
+// getDisplacement()).  A raw pointer from data() sees the values of that
+// data() call: call data() again after a step.  Errors throw
+// akantu_mi355x::Exception (the reference throws debug::Exception; a negative
+// jacobian throws JacobianException like IntegratorGaussException).
 #pragma once
 
 #include "akantu_mi355x.h"
@@ -50,22 +55,78 @@ inline void check(int status) {
 enum AnalysisMethod { _explicit_lumped_mass };
 enum ElementType { _tetrahedron_4 = AKX_TET4, _hexahedron_8 = AKX_HEX8 };
 
+// Keeps a model's host copies of its device fields current: bit f of
+// `stale` says copy f is older than the device field, bit f of `dirty` that
+// it was accessed for writing since it was last uploaded.
+class FieldSync {
+public:
+  std::uint32_t stale = 0, dirty = 0;
+  virtual void pull(int field) = 0; // download copy `field`, clear its stale bit
+
+protected:
+  ~FieldSync() = default;
+};
+
 // Minimal Array<T>: contiguous rows of nb_component values (aka_array.hh).
+// An Array a model hands out is bound to the model's FieldSync; a copy of it
+// is a plain Array of the values it holds at the copy.
 template <class T> class Array {
 public:
   Array() = default;
   Array(std::int64_t size, int nb_component, T value = T())
       : nb_comp(nb_component), data_(size * nb_component, value) {}
+  Array(const Array &o) : nb_comp(o.nb_comp), data_((o.touch(), o.data_)) {}
+  Array(Array &&o) : nb_comp(o.nb_comp), data_((o.touch(), std::move(o.data_))) {}
+  Array &operator=(const Array &o) {
+    if (this != &o) {
+      o.touch();
+      touch_write();
+      nb_comp = o.nb_comp;
+      data_ = o.data_;
+    }
+    return *this;
+  }
   std::int64_t size() const { return (std::int64_t)data_.size() / nb_comp; }
   int getNbComponent() const { return nb_comp; }
-  T &operator()(std::int64_t i, int c = 0) { return data_[i * nb_comp + c]; }
-  const T &operator()(std::int64_t i, int c = 0) const { return data_[i * nb_comp + c]; }
-  T *data() { return data_.data(); }
-  const T *data() const { return data_.data(); }
+  T &operator()(std::int64_t i, int c = 0) {
+    touch_write();
+    return data_[i * nb_comp + c];
+  }
+  const T &operator()(std::int64_t i, int c = 0) const {
+    touch();
+    return data_[i * nb_comp + c];
+  }
+  T *data() {
+    touch_write();
+    return data_.data();
+  }
+  const T *data() const {
+    touch();
+    return data_.data();
+  }
 
 private:
+  friend class SolidMechanicsModel;
+  void touch() const {
+    if (sync && (sync->stale >> field & 1u))
+      sync->pull(field);
+  }
+  void touch_write() {
+    if (sync) {
+      touch();
+      sync->dirty |= 1u << field;
+    }
+  }
+  T *raw() { return data_.data(); } // no synchronisation (the owner's own I/O)
+  void bind(FieldSync *s, int f) {
+    sync = s;
+    field = f;
+  }
+
   int nb_comp = 1;
   std::vector<T> data_;
+  FieldSync *sync = nullptr;
+  int field = -1;
 };
 
 // Mesh given as arrays (Mesh::getNodes / Mesh::getConnectivity).
@@ -108,7 +169,7 @@ private:
   std::int64_t n;
 };
 
-class SolidMechanicsModel {
+class SolidMechanicsModel : private FieldSync {
 public:
   explicit SolidMechanicsModel(const Mesh &mesh, int device = 0)
       : nb_nodes(mesh.nodes.size()),
@@ -117,16 +178,21 @@ public:
     check(akx_model_create(device, &h));
     check(akx_model_set_mesh(h, mesh.type, nb_nodes, mesh.nodes.data(), nb_elements,
                              mesh.connectivity.data()));
-    for (auto &f : fields)
-      f = Array<double>(nb_nodes, 3);
+    for (int f = 0; f < NF; ++f) {
+      fields[f] = Array<double>(nb_nodes, 3);
+      fields[f].bind(this, f);
+    }
     blocked = Array<std::uint8_t>(nb_nodes, 3);
+    blocked.bind(this, AKX_FIELD_BLOCKED_DOFS);
+    stale = ALL;
   }
   ~SolidMechanicsModel() { akx_model_destroy(h); }
   SolidMechanicsModel(const SolidMechanicsModel &) = delete;
   SolidMechanicsModel &operator=(const SolidMechanicsModel &) = delete;
 
   // material <name> [ rho = .. E = .. nu = .. ] ; finite_deformation selects
-  // the total-Lagrangian branch of "elastic"
+  // the total-Lagrangian branch of "elastic" (with
+  // "plastic_linear_isotropic_hardening" it throws: not provided)
   Material &addMaterial(const std::string &name, const MaterialParameters &p,
                         bool finite_deformation = false,
                         const std::vector<std::int32_t> &filter = {}) {
@@ -159,8 +225,9 @@ public:
   Material &getMaterial(int i) { return materials.at(i); }
 
   void initFull(AnalysisMethod = _explicit_lumped_mass) {
+    push();
     check(akx_model_init_full(h));
-    pull_all();
+    stale = ALL;
   }
   double getStableTimeStep() {
     push();
@@ -168,31 +235,45 @@ public:
     check(akx_model_get_stable_time_step(h, &dt));
     return dt;
   }
-  void setTimeStep(double dt) { check(akx_model_set_time_step(h, dt)); }
-  // ModelSolver::solveStep (nb_steps consecutive calls)
-  void solveStep(std::int64_t nb_steps = 1) {
+  // ModelSolver::setTimeStep / getTimeStep (model_solver.cc:316-326)
+  void setTimeStep(double dt, const std::string &solver_id = "") {
+    check_solver(solver_id);
+    check(akx_model_set_time_step(h, dt));
+    time_step = dt;
+  }
+  double getTimeStep(const std::string &solver_id = "") const {
+    check_solver(solver_id);
+    return time_step;
+  }
+  // ModelSolver::solveStep(const ID & solver_id = "") (model_solver.hh:78):
+  // "" and "explicit_lumped" name the solver initFull(_explicit_lumped_mass)
+  // registers (solid_mechanics_model.cc:165-169)
+  void solveStep(const std::string &solver_id = "") {
     push();
-    run(akx_model_solve_step(h, nb_steps));
-    stale = true;
+    run(akx_model_solve_step(h, solver_id.c_str()));
+    stale = ALL;
+  }
+  // nb_steps solveStep() calls in one library call (no counterpart in the
+  // reference; the same results bit for bit)
+  void solveSteps(std::int64_t nb_steps) {
+    push();
+    run(akx_model_solve_steps(h, nb_steps));
+    stale = ALL;
   }
   void assembleInternalForces() {
     push();
     run(akx_model_assemble_internal_forces(h));
-    stale = true;
+    stale = ALL;
   }
 
   // nodal Arrays (solid_mechanics_model.hh:440-468)
-  Array<double> &getDisplacement() { return field(AKX_FIELD_DISPLACEMENT, true); }
-  Array<double> &getVelocity() { return field(AKX_FIELD_VELOCITY, true); }
-  Array<double> &getAcceleration() { return field(AKX_FIELD_ACCELERATION, true); }
-  Array<double> &getExternalForce() { return field(AKX_FIELD_EXTERNAL_FORCE, true); }
-  const Array<double> &getInternalForce() { return field(AKX_FIELD_INTERNAL_FORCE, false); }
-  const Array<double> &getMass() { return field(AKX_FIELD_MASS, false); }
-  Array<std::uint8_t> &getBlockedDOFs() {
-    refresh();
-    dirty_blocked = true;
-    return blocked;
-  }
+  Array<double> &getDisplacement() { return fields[AKX_FIELD_DISPLACEMENT]; }
+  Array<double> &getVelocity() { return fields[AKX_FIELD_VELOCITY]; }
+  Array<double> &getAcceleration() { return fields[AKX_FIELD_ACCELERATION]; }
+  Array<double> &getExternalForce() { return fields[AKX_FIELD_EXTERNAL_FORCE]; }
+  const Array<double> &getInternalForce() const { return fields[AKX_FIELD_INTERNAL_FORCE]; }
+  const Array<double> &getMass() const { return fields[AKX_FIELD_MASS]; }
+  Array<std::uint8_t> &getBlockedDOFs() { return blocked; }
 
   akx_model *handle() { return h; }
   int nbQuadraturePoints() const { return nqp; }
@@ -200,33 +281,28 @@ public:
 private:
   friend class Material;
   static constexpr int NF = 6;
-  Array<double> &field(int f, bool writable) {
-    refresh();
-    if (writable)
-      dirty[f] = true;
-    return fields[f];
+  static constexpr std::uint32_t ALL = (1u << NF) - 1 | 1u << AKX_FIELD_BLOCKED_DOFS;
+  static void check_solver(const std::string &id) {
+    if (!id.empty() && id != "explicit_lumped")
+      throw Exception(AKX_ERR_INVALID, "The non linear solver dof_manager:tss:" + id +
+                                           " does not exists in dof_manager");
   }
-  void refresh() {
-    if (stale)
-      pull_all();
+  void pull(int f) override {
+    void *dst = f == AKX_FIELD_BLOCKED_DOFS ? (void *)blocked.raw() : (void *)fields[f].raw();
+    check(akx_model_get_field(h, f, dst));
+    stale &= ~(1u << f);
   }
-  void pull_all() {
-    for (int f = 0; f < NF; ++f)
-      check(akx_model_get_field(h, f, fields[f].data()));
-    check(akx_model_get_field(h, AKX_FIELD_BLOCKED_DOFS, blocked.data()));
-    stale = false;
-  }
+  // upload the copies accessed for writing (current with the device when the
+  // access was made, so the upload is exact)
   void push() {
     for (int f : {AKX_FIELD_DISPLACEMENT, AKX_FIELD_VELOCITY, AKX_FIELD_ACCELERATION,
-                  AKX_FIELD_EXTERNAL_FORCE})
-      if (dirty[f]) {
-        check(akx_model_set_field(h, f, fields[f].data()));
-        dirty[f] = false;
+                  AKX_FIELD_EXTERNAL_FORCE, AKX_FIELD_BLOCKED_DOFS})
+      if (dirty >> f & 1u) {
+        const void *src =
+            f == AKX_FIELD_BLOCKED_DOFS ? (void *)blocked.raw() : (void *)fields[f].raw();
+        check(akx_model_set_field(h, f, src));
+        dirty &= ~(1u << f);
       }
-    if (dirty_blocked) {
-      check(akx_model_set_field(h, AKX_FIELD_BLOCKED_DOFS, blocked.data()));
-      dirty_blocked = false;
-    }
   }
 
   // computeStress of the user materials; an exception a law throws is
@@ -260,11 +336,9 @@ private:
   std::int64_t nb_nodes;
   int nqp;
   std::int64_t nb_elements;
+  double time_step = 0.;
   std::array<Array<double>, NF> fields;
-  std::array<bool, NF> dirty{};
   Array<std::uint8_t> blocked;
-  bool dirty_blocked = false;
-  bool stale = true;
   std::deque<Material> materials; // stable references
 };
 

@@ -4,7 +4,7 @@
 // loop, read the nodal arrays back.  Inputs and outputs are raw binary files
 // written and read by tests/test_gpu_cpp_api.py.
 //
-//   drive_explicit <dir> <etype 0|1> <law name> <finite_deformation 0|1> <steps> [batch]
+//   drive_explicit <dir> <etype 0|1> <law name> <finite_deformation 0|1> <steps> [batch|held]
 //
 // <law name> "local_elastic" is a ConstitutiveLaw written here (the
 // reference's Material subclass overriding computeStress), the small-strain
@@ -13,6 +13,11 @@
 // With "batch" the same simulation runs as one instance of an FP64 Batch
 // (include/akantu_mi355x.hpp), beside a second instance (the same mesh at
 // rest) that must not change its result.
+//
+// With "held" the driver takes `auto & u = model.getDisplacement()` before the
+// loop, runs solveStep("explicit_lumped") and, after steps / 2 steps, adds
+// (2e-5, -1e-5, 3e-5) to u(7) through the held reference, as a reference
+// driver writing through its Array would.
 //
 // <dir>/nodes.f64 [nn][3], conn.i32 [nel][nnpe], blocked.u8 [nn][3],
 // v0.f64 [nn][3], params.f64 {rho, E, nu, sigma_y, h, dt_factor}
@@ -66,10 +71,11 @@ template <class T> static void save(const std::string &path, const T *p, size_t 
 int main(int argc, char **argv) {
   if (argc != 6 && argc != 7) {
     std::cerr << "usage: drive_explicit <dir> <etype> <law> <finite_deformation> <steps> "
-                 "[batch]\n";
+                 "[batch|held]\n";
     return 2;
   }
   const bool batch = argc == 7 && std::string(argv[6]) == "batch";
+  const bool held = argc == 7 && std::string(argv[6]) == "held";
   const std::string dir = argv[1];
   const int etype = std::atoi(argv[2]);
   const std::string law = argv[3];
@@ -129,8 +135,20 @@ int main(int argc, char **argv) {
       }
     const double dt = P[5] * model.getStableTimeStep();
     model.setTimeStep(dt);
-    for (long s = 0; s < steps; ++s)
-      model.solveStep();
+    if (held) {
+      auto &u = model.getDisplacement();
+      for (long s = 0; s < steps; ++s) {
+        if (s == steps / 2) {
+          const double bump[3] = {2e-5, -1e-5, 3e-5};
+          for (int c = 0; c < 3; ++c)
+            u(7, c) += bump[c];
+        }
+        model.solveStep("explicit_lumped");
+      }
+    } else {
+      for (long s = 0; s < steps; ++s)
+        model.solveStep();
+    }
     save(dir + "/u.f64", model.getDisplacement().data(), 3 * nn);
     save(dir + "/v.f64", model.getVelocity().data(), 3 * nn);
     save(dir + "/fint.f64", model.getInternalForce().data(), 3 * nn);

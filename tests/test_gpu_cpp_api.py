@@ -73,3 +73,27 @@ def test_cpp_batch_matches_oracle(tmp_path, etype, law, fd, shape, steps):
     assert relerr(rd("u.f64"), o.u) <= 1e-12
     assert relerr(rd("v.f64"), o.v) <= 1e-12
     assert relerr(rd("fint.f64"), o.f_int) <= 1e-12
+
+
+def test_cpp_held_reference_write(tmp_path):
+    """`auto & u = model.getDisplacement()` held across the loop and edited
+    after half the steps: the edit lands on the current state (live Array)."""
+    mesh, blocked, v0 = workload("_tetrahedron_4", (8, 3, 3), jitter=0.1, seed=5)
+    d = str(tmp_path)
+    mesh.nodes.astype(np.float64).tofile(os.path.join(d, "nodes.f64"))
+    mesh.conn.astype(np.int32).tofile(os.path.join(d, "conn.i32"))
+    blocked.astype(np.uint8).tofile(os.path.join(d, "blocked.u8"))
+    v0.astype(np.float64).tofile(os.path.join(d, "v0.f64"))
+    np.array([1.0, 1.0, 0.3, 0.0, 0.0, 0.5]).tofile(os.path.join(d, "params.f64"))
+    steps = 40
+    r = subprocess.run([DRIVER, d, str(mesh.etype), "elastic", "0", str(steps), "held"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o, dt = make_oracle(mesh, blocked, v0, "elastic")
+    o.solve_steps(steps // 2)
+    o.u[7] += np.array([2e-5, -1e-5, 3e-5])
+    o.solve_steps(steps - steps // 2)
+    rd = lambda n: np.fromfile(os.path.join(d, n)).reshape(-1, 3)
+    assert relerr(rd("u.f64"), o.u) <= 1e-12
+    assert relerr(rd("v.f64"), o.v) <= 1e-12
+    assert relerr(rd("fint.f64"), o.f_int) <= 1e-12

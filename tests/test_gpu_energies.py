@@ -36,7 +36,7 @@ def test_energies_match_oracle(etype, law, amp, mat):
     g.getExternalForce()[:] = fext
     for k in range(3):
         o.solve_steps(70)
-        g.solveStep(70)
+        g.solveSteps(70)
         for eid in ("kinetic", "potential", "external work", "plastic", "dissipated"):
             eg, eo = g.getEnergy(eid), o.energy(eid)
             print(f"{etype} {law} step {(k + 1) * 70} {eid}: gpu {eg:.15e} oracle {eo:.15e}",
@@ -55,12 +55,12 @@ def test_energies_are_reproducible_and_do_not_change_the_state():
     kw = dict(sigma_y=2e-3, h=0.05)
     g, dt = make_gpu(mesh, blocked, v0, PL, **kw)
     h, _ = make_gpu(mesh, blocked, v0, PL, dt=dt, **kw)
-    g.solveStep(40)
-    h.solveStep(40)
+    g.solveSteps(40)
+    h.solveSteps(40)
     first = [g.getEnergy(e) for e in ("kinetic", "potential", "plastic")]
     again = [g.getEnergy(e) for e in ("kinetic", "potential", "plastic")]
     assert first == again
-    g.solveStep(40)
-    h.solveStep(40)
+    g.solveSteps(40)
+    h.solveSteps(40)
     assert np.array_equal(g.getDisplacement(), h.getDisplacement())
     assert g.getEnergy("plastic") == h.getEnergy("plastic")

@@ -44,7 +44,7 @@ def test_gradient_closed_forms_and_volume(etype, shape):
 def test_chain_reproduces_internal_force(etype, shape):
     mesh, blocked, v0 = workload(etype, shape, jitter=0.1, seed=42, amp=1e-2)
     g, dt = make_gpu(mesh, blocked, v0, "elastic", E=1.4, nu=0.28)
-    g.solveStep(60)
+    g.solveSteps(60)
     mat = g.getMaterial(0)
     fe, dof = g.getFEEngine(), g.getDOFManager()
     grad = fe.gradientOnIntegrationPoints(g.getDisplacement(), 3)

@@ -70,7 +70,7 @@ def test_hex8_neohookean_200x100x100(hex8_full, precision, tol):
     assert mesh.nb_elements == 2_000_000
     g, dt = make_gpu(mesh, blocked, v0, "neohookean", precision=precision)
     assert dt == o.dt
-    g.solveStep(3)
+    g.solveSteps(3)
     errs = {f: relerr(getattr(g, f)(), ref) for f, ref in
             (("getDisplacement", o.u), ("getVelocity", o.v),
              ("getAcceleration", o.a), ("getInternalForce", o.f_int))}
@@ -89,7 +89,7 @@ def test_hex8_full_size_runs_are_bit_identical(hex8_full):
     out = []
     for _ in range(2):
         g, _ = make_gpu(mesh, blocked, v0, "neohookean", dt=o.dt)
-        g.solveStep(5)
+        g.solveSteps(5)
         out.append(g.getDisplacement().tobytes())
         del g
     assert out[0] == out[1]
@@ -112,7 +112,7 @@ def test_tet4_per_element_1000x100x100():
         g, dt = make_gpu(mesh, blocked, v0, "elastic", E_el=E, nu_el=nu,
                          precision=precision)
         assert dt == o.dt
-        g.solveStep(2)
+        g.solveSteps(2)
         eu = relerr(g.getDisplacement(), o.u)
         ef = relerr(g.getInternalForce(), o.f_int)
         tr = _translation_residual(g.getInternalForce())

@@ -56,7 +56,7 @@ def test_even_odd_elastic_materials_equal_per_element_oracle(etype, shape):
     assert dtg == dt
     for k in range(3):
         o.solve_steps(70)
-        g.solveStep(70)
+        g.solveSteps(70)
         eu = relerr(g.getDisplacement(), o.u)
         ef = relerr(g.getInternalForce(), o.f_int)
         print(etype, k, f"u {eu:.2e} f_int {ef:.2e}", flush=True)
@@ -74,7 +74,7 @@ def test_two_plastic_materials_equal_per_element_oracle():
     g, _ = _two_materials(mesh, blocked, v0, PL, parts, dt)
     for k in range(3):
         o.solve_steps(100)
-        g.solveStep(100)
+        g.solveSteps(100)
         eu = relerr(g.getDisplacement(), o.u)
         print("plastic", k, f"u {eu:.2e}", flush=True)
         assert eu <= 1e-12
@@ -94,8 +94,8 @@ def test_contiguous_halves_give_the_single_material_bits(etype):
     one, dt = make_gpu(mesh, blocked, v0, "neohookean", **mat)
     two, _ = _two_materials(mesh, blocked, v0, "neohookean",
                             [(ids[:half], mat), (ids[half:], mat)], dt)
-    one.solveStep(150)
-    two.solveStep(150)
+    one.solveSteps(150)
+    two.solveSteps(150)
     assert np.array_equal(one.getDisplacement(), two.getDisplacement())
     assert np.array_equal(one.getInternalForce(), two.getInternalForce())
 
@@ -109,7 +109,7 @@ def test_fp32_form_with_two_materials():
                         nu_el=_per_element(mesh.nb_elements, parts, "nu"))
     g, _ = _two_materials(mesh, blocked, v0, "elastic", parts, dt, precision="fp32")
     o.solve_steps(300)
-    g.solveStep(300)
+    g.solveSteps(300)
     eu = relerr(g.getDisplacement(), o.u)
     print("fp32 two materials", f"u {eu:.2e}", flush=True)
     assert eu <= 1e-5

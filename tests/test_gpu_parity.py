@@ -41,7 +41,7 @@ def test_c1_plain_fp64_1000_steps(precision):
     np.testing.assert_array_equal(g.getMass(), o.mass)
     for k in range(10):
         o.solve_steps(100)
-        g.solveStep(100)
+        g.solveSteps(100)
         _compare(g, o, f"C1 step {(k + 1) * 100}")
     if precision == "fp64":  # bit for bit
         assert np.array_equal(g.getDisplacement(), o.u)
@@ -60,7 +60,7 @@ def test_laws_and_elements_jittered(etype, law, precision):
     assert dtg == dt
     for k in range(3):
         o.solve_steps(100)
-        g.solveStep(100)
+        g.solveSteps(100)
         _compare(g, o, f"{etype} {law} step {(k + 1) * 100}")
 
 
@@ -73,7 +73,7 @@ def test_c2_reduced_checked_every_100_steps(precision):
     g, _ = make_gpu(mesh, blocked, v0, "neohookean", dt=dt, precision=precision)
     for k in range(10):
         o.solve_steps(100)
-        g.solveStep(100)
+        g.solveSteps(100)
         _compare(g, o, f"C2-reduced step {(k + 1) * 100}")
 
 
@@ -87,7 +87,7 @@ def test_plastic_linear_isotropic_hardening(etype, precision):
                     precision=precision, **kw)
     for k in range(3):
         o.solve_steps(100)
-        g.solveStep(100)
+        g.solveSteps(100)
         _compare(g, o, f"plastic step {(k + 1) * 100}")
     assert o.iso.max() > 0  # the run does yield
     mat = g.getMaterial(0)
@@ -108,7 +108,7 @@ def test_per_element_parameters(precision):
                       precision=precision)
     assert dtg == dt
     o.solve_steps(200)
-    g.solveStep(200)
+    g.solveSteps(200)
     _compare(g, o, "per-element params")
 
 
@@ -118,7 +118,7 @@ def test_stresses_and_gradients_match(precision):
     o, dt = make_oracle(mesh, blocked, v0, "neohookean")
     g, _ = make_gpu(mesh, blocked, v0, "neohookean", dt=dt, precision=precision)
     o.solve_steps(50)
-    g.solveStep(50)
+    g.solveSteps(50)
     mat = g.getMaterial(0)
     assert relerr(mat.getGradU(), o.grad_u) <= TOL
     assert relerr(mat.getStress(), o.stress) <= TOL
@@ -129,7 +129,7 @@ def test_bit_reproducible_runs():
     out = []
     for _ in range(2):
         g, _ = make_gpu(mesh, blocked, v0, "neohookean")
-        g.solveStep(150)
+        g.solveSteps(150)
         out.append((g.getDisplacement().copy(), g.getInternalForce().copy()))
     assert out[0][0].tobytes() == out[1][0].tobytes()
     assert out[0][1].tobytes() == out[1][1].tobytes()
@@ -140,7 +140,7 @@ def test_step_batching_is_invisible():
     mesh, blocked, v0 = workload("_tetrahedron_4", (6, 3, 3), jitter=0.1, seed=9)
     g1, dt = make_gpu(mesh, blocked, v0, "elastic")
     g2, _ = make_gpu(mesh, blocked, v0, "elastic", dt=dt)
-    g1.solveStep(37)
+    g1.solveSteps(37)
     for _ in range(37):
         g2.solveStep()
     assert np.array_equal(g1.getDisplacement(), g2.getDisplacement())
@@ -156,13 +156,13 @@ def test_user_writes_through_getters():
     o, dt = make_oracle(mesh, blocked, v0, "elastic")
     g, _ = make_gpu(mesh, blocked, v0, "elastic", dt=dt)
     o.solve_steps(10)
-    g.solveStep(10)
+    g.solveSteps(10)
     bump = np.zeros_like(o.u)
     bump[5] = [1e-4, -2e-4, 3e-4]
     o.u += bump
     g.getDisplacement()[:] += bump
     o.solve_steps(10)
-    g.solveStep(10)
+    g.solveSteps(10)
     _compare(g, o, "after user write")
 
 
@@ -175,7 +175,7 @@ def test_external_force_and_assemble_internal_forces():
     o.f_ext[:] = fext
     g.getExternalForce()[:] = fext
     o.solve_steps(40)
-    g.solveStep(40)
+    g.solveSteps(40)
     _compare(g, o, "f_ext")
     rng = np.random.default_rng(0)
     o.u[:] = rng.uniform(-1e-3, 1e-3, o.u.shape)
@@ -222,8 +222,8 @@ def test_pipelined_schedule_is_invisible(etype, law, shape):
     piped, _ = make_gpu(mesh, blocked, v0, law, dt=dt)
     assert piped.setPipeline(4)
     for n in (1, 2, 57):
-        plain.solveStep(n)
-        piped.solveStep(n)
+        plain.solveSteps(n)
+        piped.solveSteps(n)
         o.solve_steps(n)
         for get in ("getDisplacement", "getVelocity", "getAcceleration",
                     "getInternalForce"):

@@ -47,7 +47,7 @@ def test_patch_linear_elastic_explicit(fixtures_dir, name):
     o.u[:] = lin
     assert 0.8 * o.stable_dt() == dt
     o.set_time_step(dt)
-    g.solveStep(100)
+    g.solveSteps(100)
     o.solve_steps(100)
     u, v = g.getDisplacement(), g.getVelocity()
     ekin = 0.5 * np.sum(g.getMass() * v * v)

@@ -29,7 +29,7 @@ def test_fp64_fast(etype, law, steps, tol):
     o, dt = make_oracle(mesh, blocked, v0, law)
     g, _ = make_gpu(mesh, blocked, v0, law, dt=dt, precision="fp64_fast")
     o.solve_steps(steps)
-    g.solveStep(steps)
+    g.solveSteps(steps)
     eu, ev, ef = _errs(g, o)
     print(f"fp64_fast {etype} {law} {steps} steps: u {eu:.2e} v {ev:.2e} f {ef:.2e}")
     assert max(eu, ev, ef) <= tol
@@ -50,7 +50,7 @@ def test_fp32_form_error_against_fp64_oracle(etype, law, tol_u):
     o, dt = make_oracle(mesh, blocked, v0, law, **kw)
     g, _ = make_gpu(mesh, blocked, v0, law, dt=dt, precision="fp32", **kw)
     o.solve_steps(2000)
-    g.solveStep(2000)
+    g.solveSteps(2000)
     for name, a, b in (("u", g.getDisplacement(), o.u), ("v", g.getVelocity(), o.v),
                        ("f_int", g.getInternalForce(), o.f_int)):
         l2 = np.linalg.norm(a - b) / np.linalg.norm(b)
@@ -80,7 +80,7 @@ def test_fp32_form_every_law(etype, law, amp, mat):
     o, dt = make_oracle(mesh, blocked, v0, law, **mat)
     g, _ = make_gpu(mesh, blocked, v0, law, dt=dt, precision="fp32", **mat)
     o.solve_steps(600)
-    g.solveStep(600)
+    g.solveSteps(600)
     eu = relerr(g.getDisplacement(), o.u)
     ev = relerr(g.getVelocity(), o.v)
     ef = relerr(g.getInternalForce(), o.f_int)
@@ -106,7 +106,7 @@ def test_fp32_tet4_staged_pass_equals_per_element_pass():
                      dt=staged.getTimeStep())[0]
     assert parts.setPipeline(3)
     for n in (1, 2, 40):
-        staged.solveStep(n)
-        parts.solveStep(n)
+        staged.solveSteps(n)
+        parts.solveSteps(n)
         for get in ("getDisplacement", "getVelocity", "getAcceleration", "getInternalForce"):
             assert np.array_equal(getattr(staged, get)(), getattr(parts, get)()), (get, n)

@@ -37,7 +37,7 @@ def test_traction_loaded_bar(etype, law):
     assert np.array_equal(g.getExternalForce(), o.f_ext)
     assert np.array_equal(g.getBlockedDOFs(), o.blocked)
     for _ in range(3):
-        g.solveStep(100)
+        g.solveSteps(100)
         o.solve_steps(100)
         eu = relerr(g.getDisplacement(), o.u)
         ef = relerr(g.getInternalForce(), o.f_int)
@@ -62,7 +62,7 @@ def test_paraview_dump_of_device_fields(tmp_path):
     g.setBaseName("bar", str(tmp_path))
     for f in ("displacement", "velocity", "internal_force", "stress"):
         g.addDumpField(f)
-    g.solveStep(20)
+    g.solveSteps(20)
     path = g.dump()
     root = ET.parse(path).getroot()
     arrays = {}

@@ -107,8 +107,8 @@ def test_user_elastic_equals_builtin(etype, shape):
     ref, dt = make_gpu(mesh, blocked, v0, "elastic", **mat)
     usr, law = _user_model(mesh, blocked, v0, "local_elastic", dt, **mat)
     assert usr.getStableTimeStep() == ref.getStableTimeStep()
-    ref.solveStep(200)
-    usr.solveStep(200)
+    ref.solveSteps(200)
+    usr.solveSteps(200)
     assert law.calls == 200
     for f in ("getDisplacement", "getVelocity", "getInternalForce"):
         a, b = getattr(usr, f)(), getattr(ref, f)()
@@ -121,8 +121,8 @@ def test_user_svk_equals_builtin_finite_strain():
     mat = dict(rho=1.0, E=0.9, nu=0.31)
     ref, dt = make_gpu(mesh, blocked, v0, "elastic_fd", **mat)
     usr, law = _user_model(mesh, blocked, v0, "local_svk", dt, finite_deformation=True, **mat)
-    ref.solveStep(150)
-    usr.solveStep(150)
+    ref.solveSteps(150)
+    usr.solveSteps(150)
     eu = relerr(usr.getDisplacement(), ref.getDisplacement())
     ef = relerr(usr.getInternalForce(), ref.getInternalForce())
     print("svk", eu, ef, flush=True)
@@ -134,8 +134,8 @@ def test_user_stress_drives_the_forces():
     ref, dt = make_gpu(mesh, blocked, v0, "elastic", rho=1.0, E=2.0, nu=0.3)
     usr, law = _user_model(mesh, blocked, v0, "local_elastic", dt, rho=1.0, E=1.0, nu=0.3)
     law.scale = 2.0
-    ref.solveStep(120)
-    usr.solveStep(120)
+    ref.solveSteps(120)
+    usr.solveSteps(120)
     eu = relerr(usr.getDisplacement(), ref.getDisplacement())
     print("scaled", eu, flush=True)
     assert eu <= 1e-12
@@ -146,12 +146,12 @@ def test_internals_and_errors():
     mat = dict(rho=1.0, E=1.1, nu=0.25)
     ref, dt = make_gpu(mesh, blocked, v0, "elastic", **mat)
     usr, law = _user_model(mesh, blocked, v0, "local_elastic", dt, **mat)
-    ref.solveStep(30)
-    usr.solveStep(30)
+    ref.solveSteps(30)
+    usr.solveSteps(30)
     calls = law.calls
     assert np.array_equal(usr.getMaterial(0).getGradU(), ref.getMaterial(0).getGradU())
     assert np.array_equal(usr.getMaterial(0).getStress(), ref.getMaterial(0).getStress())
     assert law.calls == calls + 1  # computeAllStresses ran the law once
     bad, _ = _user_model(mesh, blocked, v0, "failing", dt, **mat)
     with pytest.raises(ValueError, match="on purpose"):
-        bad.solveStep(1)
+        bad.solveSteps(1)
