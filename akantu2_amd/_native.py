@@ -82,6 +82,7 @@ def _declare(lib):
         "akx_model_get_internal": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
         "akx_model_synchronize": (C.c_int, [_vp]),
         "akx_model_get_energy": (C.c_int, [_vp, C.c_char_p, C.c_int, _dp]),
+        "akx_model_get_energy_noalloc": (C.c_int, [_vp, C.c_char_p, C.c_int, _dp]),
         "akx_model_apply_traction": (C.c_int, [_vp, C.c_int, _i64, _vp, _dp]),
         "akx_model_set_pipeline": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
         "akx_model_enable_kernel_timing": (C.c_int, [_vp, C.c_int]),

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -364,13 +365,11 @@ struct EnergyArgs {
   double *out;               // [n] integral over each element
 };
 
-// FEEngine::integrate of the density: sum_q f_q wJ_q per element
-// (integrator_gauss_inline_impl.hh:401-441), wJ as at init
-template <int ET, int LAW> __global__ void k_energy_elements(EnergyArgs A) {
+// FEEngine::integrate of the density: sum_q f_q wJ_q of the i-th element of
+// the material (integrator_gauss_inline_impl.hh:401-441), wJ as at init
+template <int ET, int LAW>
+__device__ __forceinline__ double element_energy(const EnergyArgs &A, int64_t i) {
   constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n)
-    return;
   const int64_t e = A.filter ? (int64_t)A.filter[i] : A.first + i;
   double X[NN][3];
 #pragma unroll
@@ -405,43 +404,63 @@ template <int ET, int LAW> __global__ void k_energy_elements(EnergyArgs A) {
     }
     sum = (q == 0) ? f * wJ : sum + f * wJ;
   }
-  A.out[i] = sum;
+  return sum;
+}
+
+template <int ET, int LAW> __global__ void k_energy_elements(EnergyArgs A) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n)
+    return;
+  A.out[i] = element_energy<ET, LAW>(A, i);
 }
 
 // kinetic energy per node: sum_i v_i v_i m (m > epsilon) (solid_mechanics_model
 // .cc:476-522); external work per node: blocked ? -f_int v : f_ext v
 // (solid_mechanics_model.cc:545-583)
-__global__ void k_node_energy(int64_t nn, int kind, const double *v, const double *mass,
-                              const double *fint, const double *fext,
-                              const uint8_t *blocked, double *out) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn)
-    return;
+struct NodeEnergyArgs {
+  int64_t nn;
+  int kind; // 0 kinetic, 1 external work
+  const double *v, *mass, *fint, *fext;
+  const uint8_t *blocked;
+};
+
+__device__ __forceinline__ double node_energy(const NodeEnergyArgs &A, int64_t n) {
   double w = 0.;
   for (int c = 0; c < 3; ++c) {
-    const int64_t o = (int64_t)c * nn + n;
-    if (kind == 0) {
-      const double m = mass[n];
+    const int64_t o = (int64_t)c * A.nn + n;
+    if (A.kind == 0) {
+      const double m = A.mass[n];
       if (m > 2.220446049250313e-16)
-        w += v[o] * v[o] * m;
-    } else if ((blocked[n] >> c) & 1) {
-      w -= fint[o] * v[o];
+        w += A.v[o] * A.v[o] * m;
+    } else if ((A.blocked[n] >> c) & 1) {
+      w -= A.fint[o] * A.v[o];
     } else {
-      w += (fext ? fext[o] : 0.) * v[o];
+      w += (A.fext ? A.fext[o] : 0.) * A.v[o];
     }
   }
-  out[n] = w;
+  return w;
 }
 
-// one level of the tree sum: block b sums in[b*2048 .. b*2048+2047]
-__global__ void __launch_bounds__(256) k_tree_sum(const double *in, int64_t n, double *out) {
+__global__ void k_node_energy(NodeEnergyArgs A, double *out) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= A.nn)
+    return;
+  out[n] = node_energy(A, n);
+}
+
+// One level of the tree sum: block b sums the values of items b*2048 ..
+// b*2048+2047, thread t first adding items t, t+256, .., t+1792 in that order
+// from 0, then the 256 thread sums pairwise in shared memory.  value(i) reads
+// a stored value or computes it in place; both give the same bits.
+template <class F>
+__device__ __forceinline__ void block_tree_sum(int64_t n, double *out, F &&value) {
   __shared__ double red[256];
   const int64_t base = (int64_t)blockIdx.x * 2048;
   double s = 0.;
   for (int k = 0; k < 8; ++k) {
     const int64_t i = base + threadIdx.x + 256 * k;
     if (i < n)
-      s += in[i];
+      s += value(i);
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -452,6 +471,22 @@ __global__ void __launch_bounds__(256) k_tree_sum(const double *in, int64_t n, d
   }
   if (threadIdx.x == 0)
     out[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_tree_sum(const double *in, int64_t n, double *out) {
+  block_tree_sum(n, out, [&](int64_t i) { return in[i]; });
+}
+
+// First tree level fused with the per-node / per-element values (the
+// no-allocation energy): the same sums as k_node_energy / k_energy_elements
+// followed by k_tree_sum, without the [n] array of values.
+__global__ void __launch_bounds__(256) k_node_energy_sum(NodeEnergyArgs A, double *out) {
+  block_tree_sum(A.nn, out, [&](int64_t n) { return node_energy(A, n); });
+}
+
+template <int ET, int LAW>
+__global__ void __launch_bounds__(256) k_energy_elements_sum(EnergyArgs A, double *out) {
+  block_tree_sum(A.n, out, [&](int64_t i) { return element_energy<ET, LAW>(A, i); });
 }
 
 } // namespace akx
@@ -604,6 +639,10 @@ struct akx_model {
 namespace {
 
 constexpr int TPB = 256;
+
+// device address of the no-allocation energy's scratch array (set up once
+// per device)
+double *device_scratch(int device);
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 inline int64_t padded_blocks(int64_t n) {
@@ -1353,6 +1392,7 @@ int akx_model_create(int device, akx_model **out) {
       throw Error(AKX_ERR_UNSUPPORTED,
                   std::string("device is not gfx950: ") + p.gcnArchName);
     HIPCHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    (void)device_scratch(device); // the no-allocation energy's array on this device
     m->scratch_u64.alloc(4);
     m->scratch_i64.alloc(4);
     *out = m.release();
@@ -2011,11 +2051,10 @@ int akx_model_set_stress_callback(akx_model *m, akx_compute_stress_fn fn, void *
 
 namespace {
 
-// deterministic sum of n device values (buffer a is overwritten)
-double device_sum(akx_model *m, DevBuf<double> &a, DevBuf<double> &b, int64_t n) {
-  if (n == 0)
-    return 0.;
-  double *in = a.p, *out = b.p;
+// Tree sum of the n values at in[] down to one (the deterministic shape of
+// k_tree_sum); in and out are overwritten, out holds >= ceil(n / 2048).
+// Returns the pointer to the final value.
+const double *tree_reduce(akx_model *m, double *in, double *out, int64_t n) {
   while (n > 1) {
     const int64_t nb = (n + 2047) / 2048;
     hipLaunchKernelGGL(k_tree_sum, dim3((unsigned)nb), dim3(256), 0, m->stream,
@@ -2024,10 +2063,34 @@ double device_sum(akx_model *m, DevBuf<double> &a, DevBuf<double> &b, int64_t n)
     std::swap(in, out);
     n = nb;
   }
+  return in;
+}
+
+double read_back(akx_model *m, const double *p) {
   double r = 0.;
-  HIPCHECK(hipMemcpyAsync(&r, in, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIPCHECK(hipMemcpyAsync(&r, p, sizeof(double), hipMemcpyDeviceToHost, m->stream));
   HIPCHECK(hipStreamSynchronize(m->stream));
   return r;
+}
+
+// deterministic sum of n device values (buffer a is overwritten)
+double device_sum(akx_model *m, DevBuf<double> &a, DevBuf<double> &b, int64_t n) {
+  if (n == 0)
+    return 0.;
+  return read_back(m, tree_reduce(m, a.p, b.p, n));
+}
+
+NodeEnergyArgs node_energy_args(akx_model *m, int kind) {
+  NodeEnergyArgs A{};
+  A.nn = m->nn;
+  A.kind = kind;
+  A.v = m->v.p;
+  A.mass = m->mass.p;
+  A.fint = m->fint.p;
+  A.fext = m->has_fext ? m->fext.p : nullptr;
+  A.blocked = m->blocked.p;
+  require(A.v && A.mass && (kind == 0 || (A.fint && A.blocked)), "energy inputs missing");
+  return A;
 }
 
 template <int ET>
@@ -2051,14 +2114,33 @@ void launch_energy_elements(akx_model *m, int law, const EnergyArgs &A) {
   HIPCHECK(hipGetLastError());
 }
 
-// Material::getEnergy("potential" / "plastic") of one material
-double material_energy(akx_model *m, Material &mat, bool plastic_energy,
-                       DevBuf<double> &a, DevBuf<double> &b) {
+// the fused first tree level: ceil(n / 2048) partial sums into out
+template <int ET>
+void launch_energy_elements_sum(akx_model *m, int law, const EnergyArgs &A, double *out) {
+  dim3 grid((unsigned)((A.n + 2047) / 2048)), block(256);
+  switch (law) {
+  case AKX_LAW_ELASTIC:
+    hipLaunchKernelGGL((k_energy_elements_sum<ET, LAW_ELASTIC>), grid, block, 0, m->stream,
+                       A, out);
+    break;
+  case AKX_LAW_ELASTIC_FD:
+    hipLaunchKernelGGL((k_energy_elements_sum<ET, LAW_ELASTIC_FD>), grid, block, 0,
+                       m->stream, A, out);
+    break;
+  case AKX_LAW_NEOHOOKEAN:
+    hipLaunchKernelGGL((k_energy_elements_sum<ET, LAW_NEOHOOKEAN>), grid, block, 0,
+                       m->stream, A, out);
+    break;
+  default:
+    hipLaunchKernelGGL((k_energy_elements_sum<ET, LAW_PLASTIC>), grid, block, 0, m->stream,
+                       A, out);
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+// inputs of Material::getEnergy("potential" / "plastic") of one material
+EnergyArgs energy_args(akx_model *m, Material &mat, bool plastic_energy) {
   const bool plastic = mat.law == AKX_LAW_PLASTIC_LIH;
-  if (plastic_energy && !plastic)
-    return 0.; // Material::getEnergy gives 0 for ids it does not know
-  if (mat.n == 0)
-    return 0.;
   EnergyArgs A{};
   A.n = mat.n;
   A.first = mat.first;
@@ -2082,6 +2164,21 @@ double material_energy(akx_model *m, Material &mat, bool plastic_energy,
     A.s = mat.stress.p;
   }
   require(A.conn && A.X && (A.pe || (A.g && A.s)), "energy inputs missing");
+  return A;
+}
+
+// Material::getEnergy gives 0 for ids it does not know: "plastic" of a
+// material that is not plastic
+bool material_has_energy(const Material &mat, bool plastic_energy) {
+  return mat.n > 0 && (!plastic_energy || mat.law == AKX_LAW_PLASTIC_LIH);
+}
+
+// Material::getEnergy("potential" / "plastic") of one material
+double material_energy(akx_model *m, Material &mat, bool plastic_energy,
+                       DevBuf<double> &a, DevBuf<double> &b) {
+  if (!material_has_energy(mat, plastic_energy))
+    return 0.;
+  EnergyArgs A = energy_args(m, mat, plastic_energy);
   a.reserve(mat.n);
   b.reserve((mat.n + 2047) / 2048 + 1);
   A.out = a.p;
@@ -2090,6 +2187,94 @@ double material_energy(akx_model *m, Material &mat, bool plastic_energy,
   else
     launch_energy_elements<ET_HEX8>(m, mat.law, A);
   return device_sum(m, a, b, mat.n);
+}
+
+void refuse_user_potential(akx_model *m, int material_id) {
+  for (int k = 0; k < (int)m->mats.size(); ++k)
+    if ((material_id == -1 || material_id == k) && m->mats[k]->user())
+      throw Error(AKX_ERR_UNSUPPORTED,
+                  "the potential energy of a user material is not computed here");
+}
+
+// The potential energy reads the stress of the current displacement: one
+// element pass that materialises grad u and sigma of the non-plastic
+// materials and recomputes the plastic state from the previous one.
+// allocate = false: the internals must exist already (no device allocation).
+void evaluate_stresses_for_energy(akx_model *m, bool allocate) {
+  bool need = false;
+  for (auto &mp : m->mats)
+    need = need || mp->law != AKX_LAW_PLASTIC_LIH;
+  if (!need)
+    return;
+  for (auto &mp : m->mats) {
+    if (mp->user() || mp->law == AKX_LAW_PLASTIC_LIH)
+      continue;
+    const size_t nq9 = (size_t)(9 * mp->n * m->nqp);
+    if (allocate) {
+      mp->grad.alloc(nq9);
+      mp->stress.alloc(nq9);
+    } else if (nq9 > 0) {
+      require(mp->grad.p && mp->stress.p && mp->grad.n == nq9 && mp->stress.n == nq9,
+              "the stress arrays of the materials do not exist yet: call "
+              "akx_model_get_energy(\"potential\") or akx_model_compute_all_stresses once "
+              "before akx_model_get_energy_noalloc(\"potential\")");
+    }
+  }
+  user_stress_phase(m);
+  for (auto &mp : m->mats)
+    if (mp->law == AKX_LAW_PLASTIC_LIH)
+      mp->cur = 1 - mp->cur;
+  launch_element_pass(m, true);
+  flip_plastic_state(m);
+}
+
+// ---------------------------------------------------------------------------
+// No-allocation energy (akx_model_get_energy_noalloc): the partial sums live
+// in one file-scope device array.  Each device holds its own copy of it; the
+// host reaches it only through the address hipGetSymbolAddress gives on that
+// device (the array's name is a host-side stand-in, not a device address).
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_PARTIALS = AKX_NOALLOC_MAX_ITEMS / 2048; // first level
+constexpr int64_t NOALLOC_SCRATCH = NOALLOC_PARTIALS + 256;        // + second level
+
+} // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Set up once per device, with that device current (akx_model_create).
+double *device_scratch(int device) {
+  require(device >= 0 && device < MAX_DEVICES, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    HIPCHECK(hipGetDevice(&cur));
+    require(cur == device, "scratch set-up needs its device current");
+    void *p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "no device address for the energy scratch array");
+    g_scratch_ptr[device] = static_cast<double *>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
+// partials of the first level at sc[0 ..), the second level at
+// sc[NOALLOC_PARTIALS ..); returns the final value
+double noalloc_reduce(akx_model *m, double *sc, int64_t first_level_sums) {
+  return read_back(m, tree_reduce(m, sc, sc + NOALLOC_PARTIALS, first_level_sums));
+}
+
+void require_noalloc_size(int64_t n, const char *what) {
+  if (n > AKX_NOALLOC_MAX_ITEMS)
+    throw Error(AKX_ERR_INVALID,
+                std::string("akx_model_get_energy_noalloc: ") + what + " (" +
+                    std::to_string(n) + ") exceed the scratch limit of " +
+                    std::to_string((long long)AKX_NOALLOC_MAX_ITEMS));
 }
 
 } // namespace
@@ -2111,10 +2296,7 @@ int akx_model_get_energy(akx_model *m, const char *energy_id, int material_id,
       a.reserve(m->nn);
       b.reserve((m->nn + 2047) / 2048 + 1);
       hipLaunchKernelGGL(k_node_energy, dim3(grid_for(m->nn)), dim3(TPB), 0, m->stream,
-                         m->nn, kind, (const double *)m->v.p, (const double *)m->mass.p,
-                         (const double *)m->fint.p,
-                         (const double *)(m->has_fext ? m->fext.p : nullptr),
-                         (const uint8_t *)m->blocked.p, a.p);
+                         node_energy_args(m, kind), a.p);
       HIPCHECK(hipGetLastError());
       const double s = device_sum(m, a, b, m->nn);
       *energy = kind == 0 ? s / 2. : s * m->dt;
@@ -2126,33 +2308,71 @@ int akx_model_get_energy(akx_model *m, const char *energy_id, int material_id,
     }
     const bool plastic_energy = id == "plastic";
     if (!plastic_energy) {
-      for (int k = 0; k < (int)m->mats.size(); ++k)
-        if ((material_id == -1 || material_id == k) && m->mats[k]->user())
-          throw Error(AKX_ERR_UNSUPPORTED,
-                      "the potential energy of a user material is not computed here");
-      // the potential energy reads the stress of the current displacement
-      bool need = false;
-      for (auto &mp : m->mats)
-        need = need || mp->law != AKX_LAW_PLASTIC_LIH;
-      if (need) {
-        user_stress_phase(m);
-        for (auto &mp : m->mats)
-          if (mp->user())
-            continue;
-          else if (mp->law != AKX_LAW_PLASTIC_LIH) {
-            mp->grad.alloc(9 * mp->n * m->nqp);
-            mp->stress.alloc(9 * mp->n * m->nqp);
-          } else {
-            mp->cur = 1 - mp->cur;
-          }
-        launch_element_pass(m, true);
-        flip_plastic_state(m);
-      }
+      refuse_user_potential(m, material_id);
+      evaluate_stresses_for_energy(m, true);
     }
     double e = 0.;
     for (int k = 0; k < (int)m->mats.size(); ++k)
       if (material_id == -1 || material_id == k)
         e += material_energy(m, *m->mats[k], plastic_energy, a, b);
+    *energy = e;
+  });
+}
+
+int akx_model_get_energy_noalloc(akx_model *m, const char *energy_id, int material_id,
+                                 double *energy) {
+  return guard([&] {
+    check_ready(m);
+    require(energy_id != nullptr && energy != nullptr, "null argument");
+    require(material_id >= -1 && material_id < (int)m->mats.size(), "bad material id");
+    m->set_device();
+    double *sc = device_scratch(m->device);
+    const std::string id(energy_id);
+    if (id == "kinetic" || id == "external work") {
+      require(material_id == -1, "kinetic energy and external work are model energies");
+      require_noalloc_size(m->nn, "nodes");
+      const int kind = id == "kinetic" ? 0 : 1;
+      double s = 0.;
+      if (m->nn > 0) {
+        const int64_t nb = (m->nn + 2047) / 2048;
+        hipLaunchKernelGGL(k_node_energy_sum, dim3((unsigned)nb), dim3(256), 0, m->stream,
+                           node_energy_args(m, kind), sc);
+        HIPCHECK(hipGetLastError());
+        s = noalloc_reduce(m, sc, nb);
+      }
+      *energy = kind == 0 ? s / 2. : s * m->dt;
+      return;
+    }
+    if (id != "potential" && id != "plastic") {
+      *energy = 0.;
+      return;
+    }
+    const bool plastic_energy = id == "plastic";
+    for (int k = 0; k < (int)m->mats.size(); ++k)
+      if (material_id == -1 || material_id == k)
+        require_noalloc_size(m->mats[k]->n, "elements of a material");
+    if (!plastic_energy) {
+      refuse_user_potential(m, material_id);
+      if (m->has_user())
+        throw Error(AKX_ERR_UNSUPPORTED,
+                    "akx_model_get_energy_noalloc(\"potential\") runs no computeStress "
+                    "plugin: the model has a user material");
+      evaluate_stresses_for_energy(m, false);
+    }
+    double e = 0.;
+    for (int k = 0; k < (int)m->mats.size(); ++k) {
+      if (material_id != -1 && material_id != k)
+        continue;
+      Material &mat = *m->mats[k];
+      if (!material_has_energy(mat, plastic_energy))
+        continue;
+      EnergyArgs A = energy_args(m, mat, plastic_energy);
+      if (m->etype == AKX_TET4)
+        launch_energy_elements_sum<ET_TET4>(m, mat.law, A, sc);
+      else
+        launch_energy_elements_sum<ET_HEX8>(m, mat.law, A, sc);
+      e += noalloc_reduce(m, sc, (mat.n + 2047) / 2048);
+    }
     *energy = e;
   });
 }

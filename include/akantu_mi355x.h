@@ -235,6 +235,26 @@ int akx_model_get_internal(akx_model *m, int material_id, int internal,
 int akx_model_get_energy(akx_model *m, const char *energy_id, int material_id,
                          double *energy);
 
+/* No-allocation variant of akx_model_get_energy, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * ids, same values (the same tree sum, its first level fused with the
+ * per-node / per-element values); it calls no hipMalloc, hipFree or
+ * allocating library routine.  Its partial sums live in one library-owned
+ * device array per GPU, AKX_NOALLOC_MAX_ITEMS / 2048 + 256 doubles (about
+ * 0.5 MB), held until the process ends: the model's node count ("kinetic",
+ * "external work") or each selected material's element count ("potential",
+ * "plastic") may be at most AKX_NOALLOC_MAX_ITEMS, else AKX_ERR_INVALID.
+ * "potential" needs the stress arrays an earlier akx_model_get_energy(
+ * "potential") or akx_model_compute_all_stresses allocated (AKX_ERR_INVALID
+ * otherwise) and a model without user materials (AKX_ERR_UNSUPPORTED).
+ * Calls on one GPU share its array: calls that overlap in time on one GPU
+ * (two threads, or models on different streams) race and must be
+ * serialised by the caller. */
+#define AKX_NOALLOC_MAX_ITEMS ((int64_t)1 << 27) /* 134,217,728 */
+int akx_model_get_energy_noalloc(akx_model *m, const char *energy_id, int material_id,
+                                 double *energy);
+
 /* Launch schedule of solve_step, no counterpart in the reference: with
  * parts > 1 the elements are cut in `parts` consecutive parts and the nodal
  * gather of the nodes a part completes runs on a second stream while the
