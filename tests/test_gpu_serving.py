@@ -117,3 +117,52 @@ def test_bad_requests_do_not_poison_the_mix():
         alone = S.run_alone(r, "fp32")
         assert np.array_equal(alone["displacement"], r.result["displacement"])
     srv.close()
+
+
+def test_large_requests_among_small_ones():
+    """The upper end of the serving workload's sizes: a 2e6-element Hex8
+    neo-Hookean request of 1,000 steps and a 1.2e5-element Tet4 elastic
+    request of 300 steps, served in a stream with small requests that arrive
+    while they run.
+    - both large results are bit-identical to the request run alone;
+    - the 1.2e5-element result is within 1e-5 of the FP64 oracle (on the
+      host's cores), the 2e6-element one within 1e-5 of the plain FP64 path
+      (bit-pinned to the oracle at this size by the full-size tests)."""
+    import oracle
+    from akantu2_amd import mesh as M
+    from tests.helpers import make_gpu
+    from tests.test_gpu_full_size import _oracle, _threads
+
+    big = S.Request(0, 0.0, "_hexahedron_8", M.cells_for("_hexahedron_8", 2e6, 2.0),
+                    "neohookean", dict(rho=1.0, E=1.3, nu=0.27), 1000, 20_000)
+    mid = S.Request(1, 0.0, "_tetrahedron_4", M.cells_for("_tetrahedron_4", 1.2e5, 3.0),
+                    "elastic", dict(rho=1.0, E=0.7, nu=0.31), 300, 20_001)
+    small = _requests(6, seed=12, rate=1000.0, steps=(200, 400))
+    for k, r in enumerate(small):
+        r.rid = 2 + k
+    rs = [big.materialize(), mid.materialize()] + small
+    assert big.mesh.nb_elements >= 1.9e6 and mid.mesh.nb_elements >= 1e5
+    srv = S.Server("fp32", hbm_bytes=8 << 30, chunk_ms=5.0)
+    rep = srv.run(rs, clock=Tick())
+    srv.close()
+    print(f"served {len(rep.done)} in {rep.chunks} chunks, max live {rep.max_live}",
+          flush=True)
+    assert len(rep.done) == len(rs) and not rep.failed and rep.max_live >= 3
+    for r in (big, mid):
+        alone = S.run_alone(r, "fp32")
+        for f in ("displacement", "velocity", "internal_force"):
+            assert np.array_equal(alone[f], r.result[f]), (r.rid, f)
+    o = _oracle(mid.mesh, mid.blocked, mid.velocity, mid.law, **mid.material)
+    o.solve_steps(mid.steps)
+    oracle.set_threads(1)
+    eu = relerr(mid.result["displacement"], o.u)
+    print(f"request 1: {mid.mesh.nb_elements} Tet4, {mid.steps} steps, u vs oracle {eu:.2e}"
+          f" ({_threads()} oracle threads)", flush=True)
+    assert eu <= 1e-5
+    g, _ = make_gpu(big.mesh, big.blocked, big.velocity, big.law, **big.material)
+    g.solveSteps(big.steps)
+    eu = relerr(big.result["displacement"], g.getDisplacement())
+    ev = relerr(big.result["velocity"], g.getVelocity())
+    print(f"request 0: {big.mesh.nb_elements} Hex8, {big.steps} steps, u vs plain FP64 "
+          f"{eu:.2e}, v {ev:.2e}", flush=True)
+    assert eu <= 1e-5
