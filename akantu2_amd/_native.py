@@ -85,6 +85,8 @@ def _declare(lib):
         "akx_model_get_energy_noalloc": (C.c_int, [_vp, C.c_char_p, C.c_int, _dp]),
         "akx_model_apply_traction": (C.c_int, [_vp, C.c_int, _i64, _vp, _dp]),
         "akx_model_set_pipeline": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
+        "akx_model_set_block_assembly": (C.c_int, [_vp, C.c_int]),
+        "akx_model_get_block_assembly": (C.c_int, [_vp, C.POINTER(C.c_int)]),
         "akx_model_enable_kernel_timing": (C.c_int, [_vp, C.c_int]),
         "akx_model_get_kernel_timing": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(_i64)]),
         "akx_model_get_kernel_bytes": (C.c_int, [_vp, C.c_int, _dp]),

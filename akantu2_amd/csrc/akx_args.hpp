@@ -28,6 +28,15 @@ struct ElemArgs {
   const int32_t *bl_nodes;
   const uint16_t *lconn;
   int32_t bl_maxu; // most distinct nodes of one block
+  // FP32 Tet4 with block-partial assembly (null otherwise): each block sums
+  // the forces its elements put on each of its distinct nodes and writes one
+  // partial per (block, distinct node).  lslot [n][4]: the LDS slot of each
+  // element corner (the corners of distinct node k of the block take slots
+  // bl_soff[b * (bl_maxu + 1) + k] .. [k + 1] in ascending element order);
+  // bl_pos [b * bl_maxu + k]: the table entry of the partial (-1: padding).
+  const uint16_t *lslot;
+  const uint16_t *bl_soff;
+  const int32_t *bl_pos;
   const int32_t *pos;     // [(pos_offset + i) * nnpe + a] -> entry of the
                           // node-major elemental-force table
   // Hex8, plain FP64: J^-1 and w det J of every Gauss point of the mesh,

@@ -327,6 +327,45 @@ template <int ET> __global__ void k_batch_mass(BatchInitArgs A) {
     A.me[A.pos[e * NN + n] - A.ell0] = m[n];
 }
 
+// Lumped-mass contributions of one instance under block-partial assembly
+// (Tet4, FP32 batches): block b of 256 elements puts each corner's m_e(a)
+// into an LDS slot (the corners of one distinct node take consecutive slots
+// in ascending element order), then one thread per distinct node adds its
+// slots in that order and writes the block's partial to the node's table
+// entry; k_batch_gather_mass adds the partials in ascending block order.
+__global__ void __launch_bounds__(256) k_batch_mass_bpa(BatchInitArgs A, const BBlk *blks,
+                                                        const uint16_t *ls,
+                                                        const uint16_t *bsend,
+                                                        const int32_t *bpos) {
+  __shared__ double s_m[1024];
+  const BBlk B = blks[blockIdx.x];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < A.n) {
+    const int64_t e = A.first + i;
+    double X[4][3];
+    batch_load_X<ET_TET4>(A, e, X);
+    double detJ;
+    const double wJ = batch_wj<ET_TET4>(A, X, 0, detJ);
+    if (detJ < 0.)
+      atomicMin(A.first_bad, (long long)i);
+    // the single Gauss point: m[n] = 0 + (rho N_n) wJ with N_n = 1/4
+    const double mn = 0. + (A.rho * .25) * wJ;
+    const ushort4 sl = *reinterpret_cast<const ushort4 *>(ls + e * 4);
+    s_m[sl.x] = mn;
+    s_m[sl.y] = mn;
+    s_m[sl.z] = mn;
+    s_m[sl.w] = mn;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < B.count; k += blockDim.x) {
+    const int s0 = k > 0 ? (int)bsend[B.off + k - 1] : 0, s1 = bsend[B.off + k];
+    double m = 0.;
+    for (int q = s0; q < s1; ++q)
+      m += s_m[q];
+    A.me[bpos[B.off + k] - A.ell0] = m;
+  }
+}
+
 template <int ET> __global__ void __launch_bounds__(256) k_batch_dt(BatchInitArgs A) {
   constexpr int NN = Elem<ET>::nnpe;
   __shared__ double red[256];
@@ -459,6 +498,10 @@ struct HostInst {
   int64_t elem0 = 0, nel = 0;            // in the arena of its element type
   int64_t ell0 = 0, nell = 0;            // force-table entries
   int64_t steps = 0;                     // steps done
+  // block-partial assembly (Tet4, FP32): the instance's block-entry range
+  // and the distinct-node count of each of its 256-element blocks
+  int64_t bent0 = 0, nbent = 0;
+  std::vector<int32_t> blk_cnt;
 };
 
 constexpr int NGROUP = 8; // (element type, law)
@@ -486,6 +529,17 @@ struct akx_batch {
   DevBuf<long long> bad;
   DevBuf<double> staging;
   FreeList fl_nodes, fl_tet, fl_hex, fl_ell;
+  // FP32 Tet4 block-partial assembly: per element its local corners and LDS
+  // slots, per (block, distinct node) the arena node id, the end of its slot
+  // range and its table entry
+  bool bpa = false;
+  int64_t bcap = 0;
+  DevBuf<uint16_t> lc_t, ls_t;
+  DevBuf<int32_t> bnode, bpos;
+  DevBuf<uint16_t> bsend;
+  FreeList fl_bent;
+  DevBuf<BBlk> d_blks; // parallel to d_tiles
+  int32_t bmaxu_live = 0;
   std::vector<HostInst> inst;
   // device tables
   DevBuf<BInst> d_inst; // MAX_INSTANCES entries, address fixed
@@ -558,30 +612,52 @@ void upload_inst(akx_batch *b) {
 // work lists of the live instances: element tiles per group, node slices
 void rebuild_lists(akx_batch *b) {
   std::vector<std::vector<BTile>> groups(NGROUP);
+  std::vector<std::vector<BBlk>> gblks(NGROUP);
   std::vector<BSlice> slices;
+  b->bmaxu_live = 0;
   for (size_t id = 0; id < b->inst.size(); ++id) {
     const HostInst &I = b->inst[id];
     if (!I.live)
       continue;
     auto &G = groups[I.etype * 4 + I.law];
-    for (int64_t f = 0; f < I.nel; f += b->tile)
-      G.push_back({I.elem0 + f, (int32_t)std::min<int64_t>(b->tile, I.nel - f), (int32_t)id});
+    auto &GB = gblks[I.etype * 4 + I.law];
+    if (!I.blk_cnt.empty()) { // block-partial assembly: one tile per block
+      int64_t off = I.bent0;
+      for (size_t k = 0; k < I.blk_cnt.size(); ++k) {
+        const int64_t f = (int64_t)k * 256;
+        G.push_back({I.elem0 + f, (int32_t)std::min<int64_t>(256, I.nel - f), (int32_t)id});
+        GB.push_back({off, I.blk_cnt[k], 0});
+        off += I.blk_cnt[k];
+        b->bmaxu_live = std::max(b->bmaxu_live, I.blk_cnt[k]);
+      }
+    } else {
+      for (int64_t f = 0; f < I.nel; f += b->tile) {
+        G.push_back({I.elem0 + f, (int32_t)std::min<int64_t>(b->tile, I.nel - f), (int32_t)id});
+        GB.push_back({0, 0, 0});
+      }
+    }
     for (int64_t s = 0; s < I.nn; s += 256)
       slices.push_back({I.node0 + s, (int32_t)std::min<int64_t>(256, I.nn - s), (int32_t)id});
   }
   std::vector<BTile> tiles;
+  std::vector<BBlk> blks;
   b->group_first[0] = 0;
   for (int g = 0; g < NGROUP; ++g) {
     tiles.insert(tiles.end(), groups[g].begin(), groups[g].end());
+    blks.insert(blks.end(), gblks[g].begin(), gblks[g].end());
     b->group_first[g + 1] = (int64_t)tiles.size();
   }
   sync(b);
   drop_graph(b);
   b->d_tiles.reserve(tiles.size());
+  b->d_blks.reserve(blks.size());
   b->d_slices.reserve(slices.size());
-  if (!tiles.empty())
+  if (!tiles.empty()) {
     HIPCHECK(hipMemcpy(b->d_tiles.p, tiles.data(), tiles.size() * sizeof(BTile),
                        hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b->d_blks.p, blks.data(), blks.size() * sizeof(BBlk),
+                       hipMemcpyHostToDevice));
+  }
   if (!slices.empty())
     HIPCHECK(hipMemcpy(b->d_slices.p, slices.data(), slices.size() * sizeof(BSlice),
                        hipMemcpyHostToDevice));
@@ -617,6 +693,18 @@ BatchElemArgs elem_args(akx_batch *b, int etype, int64_t t0, int64_t t1, int par
   A.hex_tab = b->hex_tab.p;
   A.parity = parity;
   A.nblocks = persistent_grid(A.ntiles, (int64_t)b->cus * b->blocks_per_cu);
+  if (b->bpa && etype == AKX_TET4) {
+    A.blks = b->d_blks.p + t0;
+    A.bnode = b->bnode.p;
+    A.bsend = b->bsend.p;
+    A.bpos = b->bpos.p;
+    A.lc = b->lc_t.p;
+    A.ls = b->ls_t.p;
+    A.bmaxu = b->bmaxu_live;
+    require(A.blks && A.bnode && A.bsend && A.bpos && A.lc && A.ls && A.bmaxu > 0 &&
+                A.bmaxu <= 1024,
+            "block-partial pass launched without its block tables");
+  }
   return A;
 }
 
@@ -809,6 +897,9 @@ int64_t footprint_bytes(int precision, int etype, int law, int64_t nn, int64_t n
   double bytes = (double)nn_pad * (5 * 3 * 8 + 8 + 1 + 1 + (f32 ? 2 * 3 * 4 : 0)) +
                  (double)nn_pad / 256 * 8;
   bytes += (double)nel * nnpe * (4 + 4 + 1.25 * FE_W * sf);
+  if (f32 && etype == AKX_TET4) // block partials: corners and slots, and at most
+                                // two distinct-node entries per element
+    bytes += (double)nel * (4 * 2 * 2 + 2 * (4 + 4 + 2));
   if (law == AKX_LAW_PLASTIC_LIH) // two buffers of the history
     bytes += (double)nel * nqp * 2 * (f32 ? 7 * 4 : 28 * 8);
   return (int64_t)bytes;
@@ -818,6 +909,8 @@ void release_ranges(akx_batch *b, const HostInst &I) {
   b->fl_nodes.release(I.node0, I.nn_pad);
   (I.etype == AKX_TET4 ? b->fl_tet : b->fl_hex).release(I.elem0, I.nel);
   b->fl_ell.release(I.ell0, I.nell);
+  if (I.nbent > 0)
+    b->fl_bent.release(I.bent0, I.nbent);
 }
 
 // Hex8 device table: dN/dxi [8][3][8], N [8][8], w [8] (the numbers the
@@ -882,6 +975,19 @@ int akx_batch_create(int device, int precision, int64_t node_capacity,
     if (b->tcap) {
       b->conn_t.alloc(4 * b->tcap);
       b->pos_t.alloc(4 * b->tcap);
+    }
+    // FP32 Tet4 instances assemble by block partials; a block of 256 box
+    // elements has about 0.7 distinct nodes per element, so twice the
+    // element capacity holds the distinct-node entries of any box mesh
+    b->bpa = b->fp32() && b->tcap > 0;
+    if (b->bpa) {
+      b->bcap = 2 * b->tcap + 1024;
+      b->lc_t.alloc(4 * b->tcap);
+      b->ls_t.alloc(4 * b->tcap);
+      b->bnode.alloc(b->bcap);
+      b->bpos.alloc(b->bcap);
+      b->bsend.alloc(b->bcap);
+      b->fl_bent.reset(b->bcap);
     }
     if (b->hexcap) {
       b->conn_h.alloc(8 * b->hexcap);
@@ -960,11 +1066,59 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
     I.nn = nb_nodes;
     I.nn_pad = (nb_nodes + 255) / 256 * 256;
     I.nel = nb_elements;
+    for (int64_t i = 0; i < nb_elements * nnpe; ++i)
+      require(conn[i] >= 0 && conn[i] < nb_nodes, "connectivity refers to a missing node");
+    // block-partial assembly (FP32 Tet4): per 256-element block its distinct
+    // nodes in order of first use, each corner's index among them and its
+    // LDS slot (the corners of one distinct node take consecutive slots in
+    // ascending element order); the table then holds one entry per (block,
+    // distinct node) and a node's degree is the number of blocks touching it
+    const bool bpa = b->bpa && element_type == AKX_TET4;
+    const int64_t nblk = bpa ? (nb_elements + 255) / 256 : 0;
+    std::vector<int32_t> bl_nodes; // instance-local node ids, blocks in order
+    std::vector<uint16_t> lc, ls, send;
+    if (bpa) {
+      lc.resize(nb_elements * 4);
+      ls.resize(nb_elements * 4);
+      bl_nodes.reserve(nb_elements);
+      I.blk_cnt.resize(nblk);
+      std::vector<int64_t> stamp(nb_nodes, -1);
+      std::vector<int32_t> local(nb_nodes, 0);
+      std::vector<uint16_t> cnt(1025);
+      for (int64_t bk = 0; bk < nblk; ++bk) {
+        const int64_t e0 = bk * 256, e1 = std::min<int64_t>(nb_elements, e0 + 256);
+        const int64_t start = (int64_t)bl_nodes.size();
+        for (int64_t i = e0 * 4; i < e1 * 4; ++i) {
+          const int32_t nd = conn[i];
+          if (stamp[nd] != bk) {
+            stamp[nd] = bk;
+            local[nd] = (int32_t)((int64_t)bl_nodes.size() - start);
+            bl_nodes.push_back(nd);
+          }
+          lc[i] = (uint16_t)local[nd];
+        }
+        const int32_t nloc = (int32_t)((int64_t)bl_nodes.size() - start);
+        I.blk_cnt[bk] = nloc;
+        std::fill(cnt.begin(), cnt.begin() + nloc + 1, 0);
+        for (int64_t i = e0 * 4; i < e1 * 4; ++i)
+          cnt[lc[i] + 1]++;
+        for (int32_t k = 0; k < nloc; ++k) { // slot ends (inclusive prefix)
+          cnt[k + 1] = (uint16_t)(cnt[k + 1] + cnt[k]);
+          send.push_back(cnt[k + 1]);
+        }
+        for (int64_t i = e0 * 4; i < e1 * 4; ++i) // cnt[k] now = start of k's slots
+          ls[i] = cnt[lc[i]]++;
+      }
+      I.nbent = (int64_t)bl_nodes.size();
+    }
     // node degrees and the sliced-ELL layout of this instance
     std::vector<int32_t> deg(I.nn_pad, 0);
-    for (int64_t i = 0; i < nb_elements * nnpe; ++i) {
-      require(conn[i] >= 0 && conn[i] < nb_nodes, "connectivity refers to a missing node");
-      deg[conn[i]]++;
+    if (bpa) {
+      for (int32_t nd : bl_nodes)
+        deg[nd]++;
+    } else {
+      for (int64_t i = 0; i < nb_elements * nnpe; ++i)
+        deg[conn[i]]++;
     }
     const int64_t nsl = I.nn_pad / 256;
     std::vector<int64_t> off(nsl + 1, 0);
@@ -999,18 +1153,37 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
       fl_el.release(I.elem0, I.nel);
       throw Error(AKX_ERR_MEMORY, "force table full");
     }
+    if (bpa) {
+      I.bent0 = b->fl_bent.alloc(I.nbent);
+      if (I.bent0 < 0) {
+        b->fl_nodes.release(I.node0, I.nn_pad);
+        fl_el.release(I.elem0, I.nel);
+        b->fl_ell.release(I.ell0, I.nell);
+        throw Error(AKX_ERR_MEMORY, "block arena full");
+      }
+    }
     try {
       // arena connectivity, table entries, slice offsets, degrees, blocked bits
-      std::vector<int32_t> gconn(nb_elements * nnpe), gpos(nb_elements * nnpe);
+      std::vector<int32_t> gconn(nb_elements * nnpe), gpos(bpa ? 0 : nb_elements * nnpe);
       std::vector<int32_t> fill(I.nn_pad, 0);
       for (int64_t e = 0; e < nb_elements; ++e)
         for (int a = 0; a < nnpe; ++a) {
           const int32_t nd = conn[e * nnpe + a];
           gconn[e * nnpe + a] = (int32_t)(I.node0 + nd);
+          if (bpa)
+            continue;
           const int32_t d = fill[nd]++; // elements in ascending order
           gpos[e * nnpe + a] =
               (int32_t)(I.ell0 + off[nd >> 8] + (int64_t)d * 256 + (nd & 255));
         }
+      // block partials: entry of (block, distinct node) in ascending block order
+      std::vector<int32_t> gbnode(I.nbent), gbpos(I.nbent);
+      for (int64_t k = 0; k < I.nbent; ++k) {
+        const int32_t nd = bl_nodes[k];
+        gbnode[k] = (int32_t)(I.node0 + nd);
+        const int32_t d = fill[nd]++;
+        gbpos[k] = (int32_t)(I.ell0 + off[nd >> 8] + (int64_t)d * 256 + (nd & 255));
+      }
       std::vector<int64_t> goff(nsl);
       for (int64_t s = 0; s < nsl; ++s)
         goff[s] = I.ell0 + off[s];
@@ -1027,8 +1200,21 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
       int32_t *dpos = element_type == AKX_TET4 ? b->pos_t.p : b->pos_h.p;
       HIPCHECK(hipMemcpy(dconn + I.elem0 * nnpe, gconn.data(), gconn.size() * 4,
                          hipMemcpyHostToDevice));
-      HIPCHECK(hipMemcpy(dpos + I.elem0 * nnpe, gpos.data(), gpos.size() * 4,
-                         hipMemcpyHostToDevice));
+      if (!bpa)
+        HIPCHECK(hipMemcpy(dpos + I.elem0 * nnpe, gpos.data(), gpos.size() * 4,
+                           hipMemcpyHostToDevice));
+      if (bpa) {
+        HIPCHECK(hipMemcpy(b->lc_t.p + I.elem0 * 4, lc.data(), lc.size() * 2,
+                           hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(b->ls_t.p + I.elem0 * 4, ls.data(), ls.size() * 2,
+                           hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(b->bnode.p + I.bent0, gbnode.data(), gbnode.size() * 4,
+                           hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(b->bpos.p + I.bent0, gbpos.data(), gbpos.size() * 4,
+                           hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(b->bsend.p + I.bent0, send.data(), send.size() * 2,
+                           hipMemcpyHostToDevice));
+      }
       HIPCHECK(hipMemcpy(b->ell_off.p + I.node0 / 256, goff.data(), nsl * 8,
                          hipMemcpyHostToDevice));
       HIPCHECK(hipMemcpy(b->deg.p + I.node0, deg8.data(), I.nn_pad, hipMemcpyHostToDevice));
@@ -1106,7 +1292,22 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
       HIPCHECK(hipMemcpy(b->dt_bits.p, &dmax, 8, hipMemcpyHostToDevice));
       A.first_bad = b->bad.p;
       A.min_bits = b->dt_bits.p;
-      if (element_type == AKX_TET4) {
+      DevBuf<BBlk> iblk; // this instance's blocks, for the mass kernel
+      if (bpa) {
+        std::vector<BBlk> hb(nblk);
+        int64_t o = I.bent0;
+        for (int64_t k = 0; k < nblk; ++k) {
+          hb[k] = BBlk{o, I.blk_cnt[k], 0};
+          o += I.blk_cnt[k];
+        }
+        iblk.alloc(nblk);
+        HIPCHECK(hipMemcpy(iblk.p, hb.data(), hb.size() * sizeof(BBlk), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_batch_mass_bpa, dim3((unsigned)nblk), dim3(256), 0, b->stream, A,
+                           (const BBlk *)iblk.p, (const uint16_t *)b->ls_t.p,
+                           (const uint16_t *)b->bsend.p, (const int32_t *)b->bpos.p);
+        hipLaunchKernelGGL(k_batch_dt<ET_TET4>, dim3(gridn(I.nel)), dim3(TPB), 0,
+                           b->stream, A);
+      } else if (element_type == AKX_TET4) {
         hipLaunchKernelGGL(k_batch_mass<ET_TET4>, dim3(gridn(I.nel)), dim3(TPB), 0,
                            b->stream, A);
         hipLaunchKernelGGL(k_batch_dt<ET_TET4>, dim3(gridn(I.nel)), dim3(TPB), 0,
@@ -1385,7 +1586,8 @@ int akx_batch_capacity_left(akx_batch *b, int64_t *nodes, int64_t *tets, int64_t
 // Unique bytes one launch of each kernel class moves over the live instances:
 // element pass = connectivity + table positions + the elemental forces
 // written + X and u of every node once (+ plastic history read and written);
-// gather = the forces read (one entry per element corner) + per node degree,
+// gather = the forces read (one entry per element corner, or per block
+// partial) + per node degree,
 // blocked bits, mass, f_int written and u, v, a (+ compensation) read and
 // written.
 int akx_batch_get_kernel_bytes(akx_batch *b, int kernel_class, double *bytes) {
@@ -1399,13 +1601,21 @@ int akx_batch_get_kernel_bytes(akx_batch *b, int kernel_class, double *bytes) {
         continue;
       const double nnpe = I.etype == AKX_TET4 ? 4 : 8, nqp = I.etype == AKX_TET4 ? 1 : 8;
       const double corners = (double)I.nel * nnpe, nn = (double)I.nn;
+      // table entries written and read: one per element corner, or one per
+      // (block, distinct node) under block-partial assembly
+      const bool bpa = !I.blk_cnt.empty();
+      const double entries = bpa ? (double)I.nbent : corners;
       if (kernel_class == 0) {
-        total += corners * (4 + 4 + 3 * fs) + nn * 2 * 3 * 8;
+        if (bpa) // local corners + slots (2 B each), per entry node id, slot
+                 // end and table position, the partial written
+          total += corners * (2 + 2) + entries * (4 + 2 + 4 + 3 * fs) + nn * 2 * 3 * 8;
+        else
+          total += corners * (4 + 4 + 3 * fs) + nn * 2 * 3 * 8;
         if (I.law == AKX_LAW_PLASTIC_LIH) // history read and written
           total += (double)I.nel * nqp * 2 * (b->fp32() ? 7 : 28) * hs;
       } else {
         // a step followed by another (f_int is written by the last one only)
-        total += corners * 3 * fs + nn * (1 + 1 + 8 + 3 * 3 * 2 * 8 +
+        total += entries * 3 * fs + nn * (1 + 1 + 8 + 3 * 3 * 2 * 8 +
                                           (b->fp32() ? 2 * 3 * 2 * 4 : 0));
       }
     }

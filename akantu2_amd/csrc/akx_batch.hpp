@@ -37,9 +37,27 @@ struct BSlice {
   int32_t inst;
 };
 
+// FP32 Tet4 block-partial assembly: the distinct nodes of one 256-element
+// block of an instance (its tile) are entries off .. off + count - 1 of the
+// block arenas (node ids, slot ends, table positions)
+struct BBlk {
+  int64_t off;
+  int32_t count;
+  int32_t pad;
+};
+
 struct BatchElemArgs {
   const BTile *tiles;
   int64_t ntiles;
+  // block-partial assembly (FP32 Tet4 groups; null otherwise): per tile (one
+  // 256-element block) its distinct-node range, the block arenas, and each
+  // element's local corners and LDS slots ([elem][4] uint16)
+  const BBlk *blks;
+  const int32_t *bnode;   // arena node id of each distinct node
+  const uint16_t *bsend;  // end of each distinct node's slot range
+  const int32_t *bpos;    // table entry of each (block, distinct node) partial
+  const uint16_t *lc, *ls;
+  int32_t bmaxu;          // most distinct nodes of one live block
   const BInst *inst;
   const int32_t *conn; // [elem][nnpe] arena node ids
   const int32_t *pos;  // [elem][nnpe] table entries

@@ -33,6 +33,7 @@ using akx::BatchElemArgs;
 using akx::BInst;
 using akx::NodesSoA;
 using akx::NodesXU;
+using akx::BBlk;
 using akx::BTile;
 using akx::ElemArgs;
 using akx::QpCompact;
@@ -496,19 +497,19 @@ __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
 // math, a third memory round trip per element).  Faster for the one-element-
 // per-thread FP32 pass; slower inside the persistent batch loop, and in FP64
 // it costs the pass a wave per SIMD.
-template <int LAW, class CT, class FT, class VT, class NA, int W = 3, bool EARLY_POS = false>
-__device__ __forceinline__ void tet4_one(const int4 nd, const NA &nodes_of,
-                                         CT lambda, CT mu, CT sy, CT hh, const VT &V,
-                                         int64_t iq, const int32_t *pos_e, FT *fe) {
-  int4 pv;
-  if constexpr (EARLY_POS)
-    pv = *reinterpret_cast<const int4 *>(pos_e);
+// Q = w detJ P J^-1 of one Tet4 element (its corner forces are the columns
+// of Q and minus their sum).  BARRIER: a scheduling barrier after the node
+// loads, so that a load issued before the call stays ahead of the math.
+template <int LAW, class CT, class VT, class NA, bool BARRIER>
+__device__ __forceinline__ void tet4_Q(const int4 nd, const NA &nodes_of, CT lambda, CT mu,
+                                       CT sy, CT hh, const VT &V, int64_t iq,
+                                       CT (&Q)[3][3]) {
   const int32_t nodes[4] = {nd.x, nd.y, nd.z, nd.w};
   double X[4][3], U[4][3];
 #pragma unroll
   for (int n = 0; n < 4; ++n)
     nodes_of.load(nodes[n], X[n], U[n]);
-  if constexpr (EARLY_POS)
+  if constexpr (BARRIER)
     __builtin_amdgcn_sched_barrier(0);
   // natural directions: xi -> node 3, eta -> node 1, zeta -> node 2
   const int nm[3] = {3, 1, 2};
@@ -536,7 +537,6 @@ __device__ __forceinline__ void tet4_one(const int4 nd, const NA &nodes_of,
     }
   CT S[3][3], P[3][3];
   law_at<LAW, CT>(V, iq, lambda, mu, sy, hh, g, S, P);
-  CT Q[3][3];
 #pragma unroll
   for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -547,6 +547,17 @@ __device__ __forceinline__ void tet4_one(const int4 nd, const NA &nodes_of,
         s += P[k][J2] * Ji[J2][m];
       Q[k][m] = wJ * s;
     }
+}
+
+template <int LAW, class CT, class FT, class VT, class NA, int W = 3, bool EARLY_POS = false>
+__device__ __forceinline__ void tet4_one(const int4 nd, const NA &nodes_of,
+                                         CT lambda, CT mu, CT sy, CT hh, const VT &V,
+                                         int64_t iq, const int32_t *pos_e, FT *fe) {
+  int4 pv;
+  if constexpr (EARLY_POS)
+    pv = *reinterpret_cast<const int4 *>(pos_e);
+  CT Q[3][3];
+  tet4_Q<LAW, CT, VT, NA, EARLY_POS>(nd, nodes_of, lambda, mu, sy, hh, V, iq, Q);
   if constexpr (!EARLY_POS)
     pv = *reinterpret_cast<const int4 *>(pos_e);
   akx::put_entry<W>(fe, (int64_t)pv.x, FT(-(Q[0][0] + Q[0][1] + Q[0][2])),
@@ -603,6 +614,66 @@ __global__ void __launch_bounds__(256) k_tet4_lds(ElemArgs A) {
       reinterpret_cast<float *>(A.fe));
 }
 
+// FP32 Tet4 with block-partial assembly.  As k_tet4_lds, then the block sums
+// the forces its elements put on each of its distinct nodes and writes one
+// partial per (block, distinct node) instead of one table entry per element
+// corner: the corner forces go to LDS slots (the corners of one distinct node
+// take consecutive slots in ascending element order), and one thread per
+// distinct node adds its slots in that order (in FP64, rounded once) and
+// writes the partial.  The gather then adds a node's partials in ascending
+// block order.  The table traffic falls from four entries per element to
+// one per (block, node) pair: about 0.7 per element on the box meshes.
+template <int LAW>
+__global__ void __launch_bounds__(256) k_tet4_bpa(ElemArgs A) {
+  extern __shared__ double2 s_xu[]; // [bl_maxu][3] records, then [1024][3] floats
+  const int64_t blk = xcd_block(A.nblocks_padded);
+  if (blk * 256 >= A.n) // padding block: no elements, no node list
+    return;
+  const int64_t i = blk * 256 + threadIdx.x;
+  const bool active = i < A.n;
+  ushort4 lc = make_ushort4(0, 0, 0, 0), ls = make_ushort4(0, 0, 0, 0);
+  float lambda = 0.f, mu = 0.f, sy = 0.f, hh = 0.f;
+  if (active) {
+    lc = *reinterpret_cast<const ushort4 *>(A.lconn + i * 4);
+    ls = *reinterpret_cast<const ushort4 *>(A.lslot + i * 4);
+    element_params(A, i, lambda, mu, sy, hh);
+  }
+  akx::stage_block_records(A, blk, s_xu);
+  __syncthreads();
+  float *s_f = reinterpret_cast<float *>(s_xu + 3 * A.bl_maxu);
+  if (active) {
+    float Q[3][3];
+    tet4_Q<LAW, float, QpViewT<double>, akx::NodesLDS, false>(
+        make_int4(lc.x, lc.y, lc.z, lc.w),
+        akx::NodesLDS{reinterpret_cast<const double *>(s_xu)}, lambda, mu, sy, hh,
+        qp_view(A, 1), i, Q);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      s_f[3 * ls.x + c] = -(Q[c][0] + Q[c][1] + Q[c][2]);
+      s_f[3 * ls.y + c] = Q[c][1];
+      s_f[3 * ls.z + c] = Q[c][2];
+      s_f[3 * ls.w + c] = Q[c][0];
+    }
+  }
+  __syncthreads();
+  const uint16_t *soff = A.bl_soff + blk * (A.bl_maxu + 1);
+  const int32_t *dst = A.bl_pos + blk * A.bl_maxu;
+  float *fe = reinterpret_cast<float *>(A.fe);
+  for (int k = threadIdx.x; k < A.bl_maxu; k += blockDim.x) {
+    const int32_t d = dst[k];
+    if (d < 0)
+      continue;
+    const int s0 = soff[k], s1 = soff[k + 1];
+    double fx = 0., fy = 0., fz = 0.;
+    for (int sl = s0; sl < s1; ++sl) {
+      fx += (double)s_f[3 * sl];
+      fy += (double)s_f[3 * sl + 1];
+      fz += (double)s_f[3 * sl + 2];
+    }
+    akx::put_entry<3>(fe, (int64_t)d, float(fx), float(fy), float(fz));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Batched FP32 form: persistent over the tiles of one (element type, law)
 // group; per-instance parameters, compact FP32 plastic state (ping-pong by
@@ -624,6 +695,66 @@ __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
                                          float(P.h), V, A.hoff + e, A.pos + e * 4,
                                          reinterpret_cast<float *>(A.fe));
     }
+  }
+}
+
+// Batched FP32 Tet4 with block-partial assembly: each tile is one block of
+// at most 256 consecutive elements of one instance.  The block's distinct
+// node records are staged in LDS, the elements' corner forces go to LDS
+// slots (the corners of one distinct node take consecutive slots in
+// ascending element order), and one thread per distinct node adds its slots
+// in that order (FP64, rounded once) and writes one table entry per (block,
+// node).  The gather adds a node's partials in ascending block order.
+template <int LAW>
+__global__ void __launch_bounds__(256) k_batch_tet4_bpa(BatchElemArgs A) {
+  extern __shared__ double2 s_xu[]; // [bmaxu][3] records, then [1024][3] floats
+  float *s_f = reinterpret_cast<float *>(s_xu + 3 * A.bmaxu);
+  float *fe = reinterpret_cast<float *>(A.fe);
+  int64_t t0, t1;
+  batch_chunk(A.ntiles, A.nblocks, t0, t1);
+  for (int64_t tt = t0; tt < t1; ++tt) {
+    const BTile T = A.tiles[tt];
+    const BBlk B = A.blks[tt];
+    const BInst P = A.inst[T.inst];
+    const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
+    for (int k = threadIdx.x; k < B.count; k += blockDim.x) {
+      const double2 *p =
+          reinterpret_cast<const double2 *>(A.xu + 6 * (int64_t)A.bnode[B.off + k]);
+      s_xu[3 * k] = p[0];
+      s_xu[3 * k + 1] = p[1];
+      s_xu[3 * k + 2] = p[2];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < T.count; j += blockDim.x) {
+      const int64_t e = T.first + j;
+      const ushort4 lc = *reinterpret_cast<const ushort4 *>(A.lc + e * 4);
+      const ushort4 ls = *reinterpret_cast<const ushort4 *>(A.ls + e * 4);
+      float Q[3][3];
+      tet4_Q<LAW, float, QpCompact, akx::NodesLDS, false>(
+          make_int4(lc.x, lc.y, lc.z, lc.w),
+          akx::NodesLDS{reinterpret_cast<const double *>(s_xu)}, float(P.lambda),
+          float(P.mu), float(P.sigma_y), float(P.h), V, A.hoff + e, Q);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        s_f[3 * ls.x + c] = -(Q[c][0] + Q[c][1] + Q[c][2]);
+        s_f[3 * ls.y + c] = Q[c][1];
+        s_f[3 * ls.z + c] = Q[c][2];
+        s_f[3 * ls.w + c] = Q[c][0];
+      }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < B.count; k += blockDim.x) {
+      const int s0 = k > 0 ? (int)A.bsend[B.off + k - 1] : 0;
+      const int s1 = A.bsend[B.off + k];
+      double fx = 0., fy = 0., fz = 0.;
+      for (int sl = s0; sl < s1; ++sl) {
+        fx += (double)s_f[3 * sl];
+        fy += (double)s_f[3 * sl + 1];
+        fz += (double)s_f[3 * sl + 2];
+      }
+      akx::put_entry<FE_W>(fe, (int64_t)A.bpos[B.off + k], float(fx), float(fy), float(fz));
+    }
+    __syncthreads(); // the next tile overwrites the records and slots
   }
 }
 
@@ -662,7 +793,11 @@ static void launch_one(int etype, const ElemArgs &A, hipStream_t s) {
   if (etype != 1) {
     if constexpr (sizeof(CT) == 4)
       if (A.lconn) {
-        hipLaunchKernelGGL((k_tet4_lds<L>), grid, block, (size_t)A.bl_maxu * 48, s, A);
+        if (A.lslot) // block-partial assembly: records, then 1024 corner slots
+          hipLaunchKernelGGL((k_tet4_bpa<L>), grid, block,
+                             (size_t)A.bl_maxu * 48 + 1024 * 12, s, A);
+        else
+          hipLaunchKernelGGL((k_tet4_lds<L>), grid, block, (size_t)A.bl_maxu * 48, s, A);
         return;
       }
     hipLaunchKernelGGL((k_tet4_fast<L, CT, FT>), grid, block, 0, s, A);
@@ -697,7 +832,11 @@ static void launch_batch_law(int etype, int block, const BatchElemArgs &A,
                              hipStream_t s) {
   dim3 grid((unsigned)A.nblocks);
   if (etype == 0) {
-    hipLaunchKernelGGL(k_batch_tet4_f32<LAW>, grid, dim3(block), 0, s, A);
+    if (A.blks) // block-partial assembly: records, then 1024 corner slots
+      hipLaunchKernelGGL(k_batch_tet4_bpa<LAW>, grid, dim3(block),
+                         (size_t)A.bmaxu * 48 + 1024 * 12, s, A);
+    else
+      hipLaunchKernelGGL(k_batch_tet4_f32<LAW>, grid, dim3(block), 0, s, A);
   } else if (block == 64) {
     hipLaunchKernelGGL((k_batch_hex8_f32<LAW, 8>), grid, dim3(64), 0, s, A);
   } else if (block == 128) {

@@ -512,6 +512,11 @@ struct Material {
   DevBuf<int32_t> bl_nodes;
   DevBuf<uint16_t> lconn;
   int32_t bl_maxu = 0;
+  std::vector<int32_t> bl_nodes_host; // [blocks][bl_maxu], -1 padded
+  // block-partial assembly (FP32 Tet4): LDS slot of every corner, slot
+  // offsets of every distinct node, table entry of every (block, node) partial
+  DevBuf<uint16_t> lslot, bl_soff;
+  DevBuf<int32_t> bl_pos;
   bool per_element = false;
   // materialised internals and plastic history [2][9][n*nqp] / [2][n*nqp]
   DevBuf<double> grad, stress; // materialised by compute_all_stresses
@@ -542,6 +547,13 @@ struct akx_model {
   DevBuf<uint8_t> deg;     // [nn]
   int64_t ell_total = 0;   // entries of the node-major table
   DevBuf<unsigned char> fe; // [entries][3] of double or float
+  // FP32 Tet4 block-partial assembly: the table holds one entry per (block,
+  // distinct node of the block) instead of one per element corner
+  bool bpa = false;
+  bool bpa_allowed = true; // akx_model_set_block_assembly
+  DevBuf<int64_t> bp_off;
+  DevBuf<uint8_t> bp_deg;
+  int64_t bp_total = 0;
   DevBuf<unsigned char> staging;
   DevBuf<unsigned long long> scratch_u64;
   DevBuf<long long> scratch_i64;
@@ -736,6 +748,78 @@ void build_block_nodes(akx_model *m, Material &mat) {
   HIPCHECK(hipMemcpy(mat.lconn.p, lc.data(), lc.size() * sizeof(uint16_t),
                      hipMemcpyHostToDevice));
   mat.bl_maxu = maxu;
+  mat.bl_nodes_host = std::move(padded);
+  if (m->precision != AKX_PRECISION_FP32)
+    return;
+  // block-partial assembly: the corners of distinct node k of a block take
+  // consecutive LDS slots, in ascending element (then corner) order
+  std::vector<uint16_t> soff((size_t)nb * (maxu + 1), 0), slot(n * 4);
+  std::vector<uint16_t> cnt(maxu + 1);
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t e0 = b * 256, e1 = std::min<int64_t>(n, b * 256 + 256);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int64_t i = e0 * 4; i < e1 * 4; ++i)
+      cnt[lc[i]]++;
+    uint16_t *o = &soff[b * (maxu + 1)];
+    o[0] = 0;
+    for (int k = 0; k < maxu; ++k)
+      o[k + 1] = (uint16_t)(o[k] + cnt[k]);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int64_t i = e0 * 4; i < e1 * 4; ++i)
+      slot[i] = (uint16_t)(o[lc[i]] + cnt[lc[i]]++);
+  }
+  mat.lslot.alloc(slot.size());
+  mat.bl_soff.alloc(soff.size());
+  HIPCHECK(hipMemcpy(mat.lslot.p, slot.data(), slot.size() * sizeof(uint16_t),
+                     hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(mat.bl_soff.p, soff.data(), soff.size() * sizeof(uint16_t),
+                     hipMemcpyHostToDevice));
+}
+
+// Gather table of the block-partial assembly: node n's entries are the
+// partials of the (material, block) pairs whose elements touch it, ranked
+// in material order, then ascending block -- a fixed order, so repeated
+// runs give the same bits.  Same node-major sliced-ELL layout as the
+// per-element table.
+void build_bpa_table(akx_model *m) {
+  std::vector<int32_t> deg(m->nn, 0);
+  for (auto &mp : m->mats)
+    for (int32_t node : mp->bl_nodes_host)
+      if (node >= 0)
+        deg[node]++;
+  const int64_t nslices = (m->nn + 255) / 256;
+  std::vector<int64_t> off(nslices + 1, 0);
+  for (int64_t sl = 0; sl < nslices; ++sl) {
+    int w = 0;
+    for (int64_t nd = sl * 256; nd < std::min<int64_t>(m->nn, sl * 256 + 256); ++nd)
+      w = std::max(w, deg[nd]);
+    require(w <= 255, "a node belongs to more than 255 element blocks");
+    off[sl + 1] = off[sl] + (int64_t)w * 256;
+  }
+  m->bp_total = off[nslices];
+  require(m->bp_total < (int64_t)INT32_MAX, "block-partial table too large for int32");
+  std::vector<int32_t> fill(m->nn, 0);
+  for (auto &mp : m->mats) {
+    std::vector<int32_t> pos(mp->bl_nodes_host.size(), -1);
+    for (size_t k = 0; k < pos.size(); ++k) {
+      const int32_t node = mp->bl_nodes_host[k];
+      if (node < 0)
+        continue;
+      const int32_t d = fill[node]++;
+      pos[k] = (int32_t)(off[node >> 8] + (int64_t)d * 256 + (node & 255));
+    }
+    mp->bl_pos.alloc(pos.size());
+    HIPCHECK(hipMemcpy(mp->bl_pos.p, pos.data(), pos.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+  }
+  std::vector<uint8_t> deg8(m->nn);
+  for (int64_t nd = 0; nd < m->nn; ++nd)
+    deg8[nd] = (uint8_t)deg[nd];
+  m->bp_off.alloc(off.size());
+  m->bp_deg.alloc(std::max<int64_t>(m->nn, 1));
+  HIPCHECK(hipMemcpy(m->bp_off.p, off.data(), off.size() * sizeof(int64_t),
+                     hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(m->bp_deg.p, deg8.data(), deg8.size(), hipMemcpyHostToDevice));
 }
 
 // FP32 form: (float(lambda), float(mu)) pairs of a material with per-element
@@ -777,6 +861,11 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
   A.bl_nodes = mat.bl_nodes.p;
   A.lconn = mat.lconn.p;
   A.bl_maxu = mat.bl_maxu;
+  if (m->bpa) {
+    A.lslot = mat.lslot.p;
+    A.bl_soff = mat.bl_soff.p;
+    A.bl_pos = mat.bl_pos.p;
+  }
   A.sy_el = mat.sy_el.p;
   A.h_el = mat.h_el.p;
   A.pos = m->pos.p;
@@ -935,8 +1024,12 @@ void launch_element_pass(akx_model *m, bool materialize) {
     check_elem_args(A, mat.law);
     require(m->etype != AKX_TET4 || A.xu != nullptr, "Tet4 element pass without its node records");
     require(m->fe.n >= (size_t)(m->ell_total * 3 * m->fe_elem_size()) &&
+                m->fe.n >= (size_t)(m->bp_total * 3 * m->fe_elem_size()) &&
                 m->pos.n >= (size_t)((mat.pos_offset + mat.n) * m->nnpe),
             "element force table too small for this material");
+    if (m->bpa)
+      require(A.lslot && A.bl_soff && A.bl_pos && A.bl_nodes && A.lconn,
+              "block-partial pass launched without its block tables");
     if (strict && m->etype == AKX_HEX8) {
       int64_t nb = (mat.n + HEX_STRICT_EPB - 1) / HEX_STRICT_EPB;
       A.nblocks_padded = (nb + 7) / 8 * 8;
@@ -1013,8 +1106,8 @@ GatherArgs gather_args(akx_model *m) {
   G.nn = m->nn;
   G.node0 = 0;
   G.node_end = m->nn;
-  G.ell_off = m->ell_off.p;
-  G.deg = m->deg.p;
+  G.ell_off = m->bpa ? m->bp_off.p : m->ell_off.p;
+  G.deg = m->bpa ? m->bp_deg.p : m->deg.p;
   G.fe = m->fe.p;
   G.fint = m->fint.p;
   G.fext = m->has_fext ? m->fext.p : nullptr;
@@ -1098,7 +1191,7 @@ void plan_pipeline(akx_model *m) {
   m->pipe_planned = true;
   m->pipe_ok = false;
   const int P = m->pipe_parts;
-  if (P < 2 || m->mats.size() != 1)
+  if (P < 2 || m->mats.size() != 1 || m->bpa)
     return;
   Material &mat = *m->mats[0];
   if (!mat.contiguous || mat.law == AKX_LAW_PLASTIC_LIH || mat.first != 0 ||
@@ -1576,6 +1669,22 @@ int akx_model_set_precision(akx_model *m, int precision) {
   });
 }
 
+int akx_model_set_block_assembly(akx_model *m, int enable) {
+  return guard([&] {
+    require(m != nullptr, "null model");
+    require(!m->initialized, "the assembly must be chosen before initFull");
+    m->bpa_allowed = enable != 0;
+  });
+}
+
+int akx_model_get_block_assembly(akx_model *m, int *active) {
+  return guard([&] {
+    check_ready(m);
+    require(active != nullptr, "null output");
+    *active = m->bpa ? 1 : 0;
+  });
+}
+
 int akx_model_init_full(akx_model *m) {
   return guard([&] {
     require(m != nullptr && m->etype >= 0, "set the mesh first");
@@ -1636,7 +1745,14 @@ int akx_model_init_full(akx_model *m) {
           build_block_nodes(m, *mp);
     }
     build_gather_table(m);
-    m->fe.alloc(std::max<int64_t>(m->ell_total, 1) * 3 * m->fe_elem_size());
+    m->bpa = m->bpa_allowed && m->etype == AKX_TET4 &&
+             m->precision == AKX_PRECISION_FP32 && m->nn > 0;
+    for (auto &mp : m->mats)
+      m->bpa = m->bpa && mp->contiguous && mp->n > 0 && mp->lslot.p != nullptr;
+    if (m->bpa)
+      build_bpa_table(m);
+    m->fe.alloc(std::max<int64_t>(std::max(m->ell_total, m->bp_total), 1) * 3 *
+                m->fe_elem_size());
     // lumped mass
     {
       DevBuf<double> me;
@@ -2762,17 +2878,28 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
         extra += mp->n * 4.0;
     }
     (void)s_c;
-    const double deg = (double)nel_all * m->nnpe / (double)m->nn;
-    (void)deg;
+    // table entries the element pass writes and the gather reads: one per
+    // element corner, or one per (block, distinct node) partial
+    double entries = (double)nel_all * m->nnpe;
+    if (m->bpa) {
+      double partials = 0.;
+      for (auto &mp : m->mats)
+        for (int32_t node : mp->bl_nodes_host)
+          partials += node >= 0 ? 1. : 0.;
+      // corner slots (2 B) instead of table positions (4 B); per partial its
+      // slot offset (2 B) and table position (4 B)
+      extra += -2.0 * m->nnpe * nel_all + 6.0 * partials;
+      entries = partials;
+    }
     switch (kernel_class) {
     case 0: // element pass: connectivity + table entries, X and u once per
             // node, r_e write; with the Hex8 jacobian table, the table (80 B
             // per Gauss point) instead of X
       if (m->jw.p)
-        *bytes = nel_all * (8.0 * m->nnpe + 3.0 * s_f * m->nnpe + 80.0 * m->nqp) +
+        *bytes = nel_all * (8.0 * m->nnpe + 80.0 * m->nqp) + entries * 3.0 * s_f +
                  m->nn * 3.0 * s_node + extra;
       else
-        *bytes = nel_all * (8.0 * m->nnpe + 3.0 * s_f * m->nnpe) +
+        *bytes = nel_all * 8.0 * m->nnpe + entries * 3.0 * s_f +
                  m->nn * (3.0 * s_node + 3.0 * s_node) + extra;
       break;
     case 1: { // gather + update of a step followed by another: degree, r_e
@@ -2783,7 +2910,7 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
                         2 * 3 * s_node /*u*/ + (m->has_fext ? 3 * s_node : 0.);
       if (m->precision == AKX_PRECISION_FP32)
         per_node += 2 * 2 * 3 * 4.0; // compensation terms read + write
-      *bytes = nel_all * m->nnpe * (3.0 * s_f) + m->nn * (per_node - 3.0);
+      *bytes = entries * (3.0 * s_f) + m->nn * (per_node - 3.0);
       break;
     }
     case 2:

@@ -604,6 +604,17 @@ class SolidMechanicsModel:
         N.check(N.lib().akx_model_set_pipeline(self._h, int(parts), C.byref(active)))
         return bool(active.value)
 
+    def setBlockAssembly(self, enable: bool):
+        """FP32 Tet4 block-partial assembly on or off (see
+        akx_model_set_block_assembly); before initFull."""
+        N.check(N.lib().akx_model_set_block_assembly(self._h, int(bool(enable))))
+
+    def blockAssembly(self) -> bool:
+        """Whether the initialised model assembles by block partials."""
+        active = C.c_int(0)
+        N.check(N.lib().akx_model_get_block_assembly(self._h, C.byref(active)))
+        return bool(active.value)
+
     # ----------------------------------------------------------- measurement
     def enableKernelTiming(self, enable=True):
         N.check(N.lib().akx_model_enable_kernel_timing(self._h, int(enable)))

@@ -256,7 +256,127 @@ def run_plain(args):
             out["cpu_baseline"] = cpu_baseline_plain()
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"error": repr(e)}
+    if args.precision == "fp64" and not args.no_forms:
+        print("# forms: FP32 form, 512-instance batch, serving stream", file=sys.stderr,
+              flush=True)
+        out["forms"] = run_forms(args, model, mesh, blocked, v0, dt, value,
+                                 args.warmup + steps + min(steps, 100))
+        out["north_star"] = north_star_summary(out)
     return out
+
+
+# --------------------------------------------------------------------------
+# the three forms beside the headline (default run)
+# --------------------------------------------------------------------------
+def run_forms(args, plain, mesh, blocked, v0, dt, plain_value, plain_steps):
+    """The precision, batching and serving forms measured in the same run as
+    the plain FP64 line, each on its own workload:
+    - precision: the headline workload (200 x 100 x 100 Hex8 neo-Hookean) in
+      the FP32 form, same warmup / timed / kernel-timing steps as the plain
+      line, and its u, v, f_int errors against the plain model after the same
+      number of steps (the plain path is bit-pinned to the oracle);
+    - batch: the 512 Tet4 plasticity instances (5e3-4e5 elements, seed 4) in
+      the FP32 form, autotuned, hipGraph, the same timed step count;
+    - serve: a Poisson stream of 24 requests at 8/s from the serving
+      workload's distribution (Tet4 / Hex8, 1e3-2e6 elements, 200-5,000
+      steps), FP32 form, continuous batching: throughput and p50 / p99
+      time-to-result."""
+    from akantu2_amd import SolidMechanicsModel
+    from akantu2_amd import serving as S
+    steps = args.steps or 1000
+    forms = {}
+    t = time.perf_counter()
+    m = SolidMechanicsModel(mesh)
+    m.addMaterial("neohookean", rho=1.0, E=1.0, nu=0.3)
+    m.setPrecision("fp32")
+    m.initFull()
+    m.getBlockedDOFs()[:] = blocked
+    m.getVelocity()[:] = v0
+    m.setTimeStep(dt)
+    m.solveSteps(args.warmup)
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.solveSteps(steps)
+    m.synchronize()
+    el = time.perf_counter() - t0
+    m.enableKernelTiming(True)
+    m.solveSteps(min(steps, 100))
+    m.synchronize()
+    kern = {}
+    for k, name in {0: "element_pass", 1: "gather_update"}.items():
+        ms, n = m.kernelTiming(k)
+        by = m.kernelBytes(k)
+        kern[name] = {"avg_ms": ms / n, "bytes": by, "achieved_gbs": by / (ms / n * 1e-3) / 1e9}
+    assert args.warmup + steps + min(steps, 100) == plain_steps
+    err = {"u": rel_errors(m.getDisplacement(), plain.getDisplacement()),
+           "v": rel_errors(m.getVelocity(), plain.getVelocity()),
+           "f_int": rel_errors(m.getInternalForce(), plain.getInternalForce())}
+    value = mesh.nb_elements * 8 * steps / el
+    forms["precision"] = {
+        "workload": "the headline workload in the FP32 form (FP64 compensated nodal update)",
+        "value": value, "unit": "elem*qp/s", "ms_per_step": 1e3 * el / steps,
+        "vs_plain_fp64": value / plain_value, "kernels": kern,
+        "error_vs_plain_fp64_after_steps": plain_steps, "error": err,
+        "setup_s": t0 - t}
+    del m
+    print(f"# forms: precision {1e3 * el / steps:.3f} ms/step", file=sys.stderr, flush=True)
+
+    t = time.perf_counter()
+    specs = c4_instances(512)
+    b, ids = _fill_batch("fp32", specs)
+    st = b.stats()
+    setup = time.perf_counter() - t
+    el, bk, best = _time_batch(b, 3, steps, tune=True)
+    b.close()
+    value = st["quadrature_points"] * steps / el
+    forms["batch"] = {
+        "workload": "512 Tet4 boxes of 5e3-4e5 elements (log-uniform, seed 4), aspect "
+                    "U(1, 8), linear isotropic hardening with per-instance E, nu, sigma_y, "
+                    "h, FP32 form, persistent launches in a hipGraph, autotuned",
+        "instances": st["live"], "elements": st["elements"], "value": value,
+        "unit": "elem*qp/s", "ms_per_step": 1e3 * el / steps, "steps": steps,
+        "vs_plain_fp64": value / plain_value, "kernels": bk, "launch": best,
+        "hbm_frac": {k: v["achieved_gbs"] / HBM_PEAK_GBS for k, v in bk.items()},
+        "setup_s": setup}
+    print(f"# forms: batch {1e3 * el / steps:.3f} ms/step, setup {setup:.1f} s",
+          file=sys.stderr, flush=True)
+
+    reqs = S.synthetic_requests(24, rate=8.0, seed=5)
+    for r in reqs:
+        r.materialize()
+    server = S.Server(precision="fp32", hbm_bytes=16 << 30)
+    rep = server.run(reqs)
+    server.close()
+    tt = np.array([r.time_to_result for r in rep.done])
+    forms["serve"] = {
+        "workload": "Poisson stream of 24 requests at 8/s, Tet4 or Hex8 boxes of 1e3-2e6 "
+                    "elements (log-uniform), elastic or neo-Hookean, 200-5,000 steps, FP32 "
+                    "form, continuous batching with HBM-footprint admission",
+        "value": rep.throughput, "unit": "elem*qp/s",
+        "vs_plain_fp64": rep.throughput / plain_value,
+        "busy_throughput": sum(r.work for r in rep.done) / rep.busy if rep.busy else None,
+        "latency_s": {"p50": float(np.percentile(tt, 50)), "p99": float(np.percentile(tt, 99)),
+                      "max": float(tt.max())},
+        "makespan_s": rep.makespan, "done": len(rep.done), "failed": len(rep.failed),
+        "max_live": rep.max_live, "chunks": rep.chunks}
+    print(f"# forms: serve p50 {forms['serve']['latency_s']['p50']:.2f} s", file=sys.stderr,
+          flush=True)
+    return forms
+
+
+def north_star_summary(out):
+    """The target stated for the forms: the FP32 batched / served form at >= 3x
+    the plain FP64 path's element*qp throughput, <= 1e-5 relative
+    displacement error, >= 70 % of peak HBM bandwidth."""
+    f = out["forms"]
+    return {"target_ratio": 3.0, "target_hbm_frac": 0.70, "target_u_error": 1e-5,
+            "plain_fp64_value": out["value"],
+            "ratios": {k: f[k]["vs_plain_fp64"] for k in ("precision", "batch", "serve")},
+            "precision_u_error": f["precision"]["error"]["u"]["max"],
+            "batch_hbm_frac": f["batch"]["hbm_frac"],
+            "precision_hbm_frac": {k: v["achieved_gbs"] / HBM_PEAK_GBS
+                                   for k, v in f["precision"]["kernels"].items()},
+            "serve_latency_s": f["serve"]["latency_s"]}
 
 
 # --------------------------------------------------------------------------
@@ -582,6 +702,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="plain form: skip the FP32 / batch / serving legs")
     ap.add_argument("--shape", type=str, default=None,
                     help="override cells per direction, e.g. 20,10,10")
     ap.add_argument("--instances", type=int, default=512, help="batch form")

@@ -264,6 +264,21 @@ int akx_model_get_energy_noalloc(akx_model *m, const char *energy_id, int materi
  * whether it applies.  parts = 0 or 1: the plain schedule (default). */
 int akx_model_set_pipeline(akx_model *m, int parts, int *active);
 
+/* Assembly of the FP32 Tet4 element pass, no counterpart in the reference
+ * (call before akx_model_init_full).  enable != 0 (default): block-partial
+ * assembly where it applies (AKX_PRECISION_FP32, Tet4, every material a
+ * contiguous element range): each block of 256 elements sums the forces of
+ * its elements on each of its distinct nodes in a fixed order, and the nodal
+ * gather adds a node's block partials in ascending block order, so the
+ * element-force table holds one entry per (block, node) instead of one per
+ * element corner; results are bit-reproducible and within the FP32 form's
+ * bound, but not bit-identical to the per-element order.  enable = 0: one
+ * table entry per element corner, added in ascending element order (the
+ * order of the FP64 path).  get_block_assembly reports whether the
+ * initialised model uses it. */
+int akx_model_set_block_assembly(akx_model *m, int enable);
+int akx_model_get_block_assembly(akx_model *m, int *active);
+
 /* Synchronise the model's stream (all queued steps finished). */
 int akx_model_synchronize(akx_model *m);
 

@@ -92,21 +92,72 @@ def test_fp32_form_every_law(etype, law, amp, mat):
 
 
 
+def _fp32_tet4(mesh, blocked, v0, block_assembly, dt=None, law="elastic", **mat):
+    from akantu2_amd import SolidMechanicsModel
+    g = SolidMechanicsModel(mesh)
+    g.addMaterial(law, rho=1.0, E=1.0, nu=0.3, **mat)
+    g.setPrecision("fp32")
+    g.setBlockAssembly(block_assembly)
+    g.initFull()
+    assert g.blockAssembly() == block_assembly
+    g.getBlockedDOFs()[:] = blocked
+    g.getVelocity()[:] = v0
+    g.setTimeStep(dt if dt is not None else 0.5 * g.getStableTimeStep())
+    return g
+
+
 def test_fp32_tet4_staged_pass_equals_per_element_pass():
-    """The FP32 Tet4 element pass of a whole contiguous material stages each
-    256-element block's node records in LDS; the pipelined schedule's partial
-    launches read the records per element instead.  Same arithmetic, so the
-    two give the same bits (per-element E and nu, several step counts)."""
+    """With one table entry per element corner, the FP32 Tet4 element pass of
+    a whole contiguous material stages each 256-element block's node records
+    in LDS; the pipelined schedule's partial launches read the records per
+    element instead.  Same arithmetic, so the two give the same bits
+    (per-element E and nu, several step counts)."""
     mesh, blocked, v0 = workload("_tetrahedron_4", (30, 7, 5), jitter=0.1, seed=4)
     rng = np.random.default_rng(4)
     E = np.exp(rng.uniform(np.log(0.5), np.log(2.0), mesh.nb_elements))
     nu = rng.uniform(0.2, 0.35, mesh.nb_elements)
-    staged = make_gpu(mesh, blocked, v0, "elastic", precision="fp32", E_el=E, nu_el=nu)[0]
-    parts = make_gpu(mesh, blocked, v0, "elastic", precision="fp32", E_el=E, nu_el=nu,
-                     dt=staged.getTimeStep())[0]
+    staged = _fp32_tet4(mesh, blocked, v0, False, E_el=E, nu_el=nu)
+    parts = _fp32_tet4(mesh, blocked, v0, False, dt=staged.getTimeStep(), E_el=E, nu_el=nu)
     assert parts.setPipeline(3)
     for n in (1, 2, 40):
         staged.solveSteps(n)
         parts.solveSteps(n)
         for get in ("getDisplacement", "getVelocity", "getAcceleration", "getInternalForce"):
             assert np.array_equal(getattr(staged, get)(), getattr(parts, get)()), (get, n)
+
+
+@pytest.mark.parametrize("law,amp,mat", [
+    ("elastic", 1e-3, {}),
+    ("neohookean", 1e-3, {}),
+    ("plastic_linear_isotropic_hardening", 3e-2, dict(sigma_y=2e-3, h=0.05)),
+])
+def test_fp32_tet4_block_assembly(law, amp, mat):
+    """Block-partial assembly (one table entry per (256-element block, node)
+    instead of one per element corner) against the FP64 oracle after 1,000
+    steps (bar 1e-5 on u, as the per-element assembly), against the
+    per-element assembly, and bit-reproducible: two runs give the same bits.
+    The internal forces still sum to zero over the nodes."""
+    mesh, blocked, v0 = workload("_tetrahedron_4", (24, 9, 7), jitter=0.1, seed=9, amp=amp)
+    if law == "elastic":
+        rng = np.random.default_rng(9)
+        mat = dict(E_el=np.exp(rng.uniform(np.log(0.5), np.log(2.0), mesh.nb_elements)),
+                   nu_el=rng.uniform(0.2, 0.35, mesh.nb_elements))
+    o, dt = make_oracle(mesh, blocked, v0, law, **mat)
+    bpa = _fp32_tet4(mesh, blocked, v0, True, dt=dt, law=law, **mat)
+    again = _fp32_tet4(mesh, blocked, v0, True, dt=dt, law=law, **mat)
+    per_el = _fp32_tet4(mesh, blocked, v0, False, dt=dt, law=law, **mat)
+    for n in (1, 99, 900):
+        o.solve_steps(n)
+        for g in (bpa, again, per_el):
+            g.solveSteps(n)
+        eu = relerr(bpa.getDisplacement(), o.u)
+        eu_el = relerr(per_el.getDisplacement(), o.u)
+        ef = relerr(bpa.getInternalForce(), o.f_int)
+        f = bpa.getInternalForce()
+        tr = float((np.abs(f.sum(axis=0)) / np.abs(f).sum(axis=0)).max())
+        print(f"{law} step {n}: block u {eu:.2e} f_int {ef:.2e}, per-element u {eu_el:.2e}, "
+              f"sum f_int {tr:.1e}", flush=True)
+        assert eu <= 1e-5 and eu_el <= 1e-5
+        assert tr <= 1e-5
+        for get in ("getDisplacement", "getVelocity", "getInternalForce"):
+            assert np.array_equal(getattr(bpa, get)(), getattr(again, get)()), (get, n)
