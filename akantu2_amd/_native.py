@@ -62,6 +62,8 @@ def _declare(lib):
         "akx_model_set_element_parameters": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
         "akx_material_law_by_name": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
         "akx_model_set_precision": (C.c_int, [_vp, C.c_int]),
+        "akx_model_set_eigen_grad_u": (C.c_int, [_vp, C.c_int, _vp]),
+        "akx_model_set_thermal": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
         "akx_model_set_stress_callback": (C.c_int, [_vp, COMPUTE_STRESS_FN, _vp]),
         "akx_fe_gradient_on_integration_points": (C.c_int, [_vp, _vp, C.c_int, _vp, _i64,
                                                             _vp]),

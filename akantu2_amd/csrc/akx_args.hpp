@@ -57,6 +57,11 @@ struct ElemArgs {
   // plastic energy [n*nqp]: previous (read) / current (write), or null
   const double *p_pe;
   double *c_pe;
+  // Material::eigengradu, [9][n*nqp], subtracted from grad u (null: zero);
+  // thermal stress sigma_th of every Gauss point [n*nqp] (null: -0), and for
+  // the plastic law its previous (read) / current (write) values
+  const double *eig, *sth_now, *sth_p;
+  double *sth_c;
   int64_t nblocks_padded;
 };
 
@@ -73,13 +78,17 @@ template <class HT> struct QpViewT {
   int64_t nq_all;
   const HT *p_pe; // accumulated plastic energy, or null (not tracked)
   HT *c_pe;
+  // eigen grad u [9][nq_all] and thermal stress [nq_all] (plastic: previous /
+  // current), read by the element passes instantiated with them
+  const double *eig = nullptr, *sth_now = nullptr, *sth_p = nullptr;
+  double *sth_c = nullptr;
 };
 using QpView = QpViewT<double>;
 
 __host__ __device__ inline QpView qp_view(const ElemArgs &A, int nqp) {
-  return QpView{A.p_grad, A.p_sig,  A.p_eps,  A.p_iso,       A.c_grad,
-                A.c_sig,  A.c_eps,  A.c_iso,  A.grad_u,      A.stress,
-                A.n * nqp, A.p_pe,  A.c_pe};
+  return QpView{A.p_grad, A.p_sig,  A.p_eps,  A.p_iso,  A.c_grad, A.c_sig,
+                A.c_eps,  A.c_iso,  A.grad_u, A.stress, A.n * nqp, A.p_pe,
+                A.c_pe,   A.eig,    A.sth_now, A.sth_p, A.sth_c};
 }
 
 // ---------------------------------------------------------------------------

@@ -326,16 +326,45 @@ __device__ __forceinline__ CT plastic_energy_update(CT pe_prev, const CT (&S)[3]
 // (material_elastic_inline_impl.hh:31-73, material_neohookean_inline_impl.hh
 // :39-50, material_linear_isotropic_hardening_inline_impl.hh:31-86), with the
 // history and output traffic of Material::computeAllStresses.
+// Material::computeAllStresses's gradu -= eigengradu (material.cc:147) and
+// MaterialThermal::computeStress's sigma_th (material_thermal.hh:91-95,
+// computed per Gauss point at set-up from alpha and delta_T), for the
+// element passes instantiated with them (EXT).  The plastic law also reads
+// the previous sigma_th and records the current one (its history,
+// material_thermal.cc:34).
+template <int LAW, class CT, class HT>
+__device__ __forceinline__ void ext_terms(const QpViewT<HT> &V, int64_t iq, CT (&g)[3][3],
+                                          CT &sth, CT &dsth) {
+  if (V.eig) {
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii)
+        g[ii][jj] = g[ii][jj] - CT(V.eig[(int64_t)(ii + 3 * jj) * V.nq_all + iq]);
+  }
+  if (V.sth_now) {
+    sth = CT(V.sth_now[iq]);
+    if constexpr (LAW == LAW_PLASTIC) {
+      dsth = CT(V.sth_p[iq]) - sth;
+      V.sth_c[iq] = double(sth);
+    }
+  }
+}
+
+// sth: MaterialThermal's sigma_th (-0 with the default alpha = 0,
+// material_thermal.hh:91-95), added by the elastic laws; dsth = previous -
+// current sigma_th, added to the plastic trial stress.
 template <int LAW, class CT, class HT>
 __device__ __forceinline__ void strict_law(const QpViewT<HT> &V, int64_t iq, CT lambda,
                                            CT mu, CT sy, CT hh, const CT (&g)[3][3],
-                                           CT (&S)[3][3]) {
+                                           CT (&S)[3][3], CT sth = CT(-0.0),
+                                           CT dsth = CT(0)) {
   if constexpr (LAW == LAW_ELASTIC) {
-    elastic_stress(lambda, mu, g, CT(-0.0), S);
+    elastic_stress(lambda, mu, g, sth, S);
   } else if constexpr (LAW == LAW_ELASTIC_FD) {
     CT E[3][3];
     green_strain(g, E);
-    elastic_stress(lambda, mu, E, CT(-0.0), S);
+    elastic_stress(lambda, mu, E, sth, S);
   } else if constexpr (LAW == LAW_NEOHOOKEAN) {
     neohookean_pk2(lambda, mu, g, S);
   } else if constexpr (LAW == LAW_USER_GRAD) {
@@ -366,7 +395,7 @@ __device__ __forceinline__ void strict_law(const QpViewT<HT> &V, int64_t iq, CT 
         ps[ii][jj] = CT(V.p_sig[o]);
         pe[ii][jj] = CT(V.p_eps[o]);
       }
-    plastic_lih(lambda, mu, sy, hh, g, pg, ps, CT(V.p_iso[iq]), pe, S, iso, eps);
+    plastic_lih(lambda, mu, sy, hh, g, pg, ps, CT(V.p_iso[iq]), pe, S, iso, eps, dsth);
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
@@ -398,7 +427,7 @@ __device__ __forceinline__ void strict_law(const QpViewT<HT> &V, int64_t iq, CT 
 // over the Gauss points in ascending order.  Shared by the single-model
 // launch (one material) and the batched persistent launch.
 // ---------------------------------------------------------------------------
-template <int ET, int LAW, class CT, class FT, class HT, class NA, int W = 3>
+template <int ET, int LAW, class CT, class FT, class HT, class NA, int W = 3, bool EXT = false>
 __device__ __forceinline__ void element_forces_one(
     const int32_t *conn_e, const NA &nodes_of,
     CT lambda, CT mu, CT sy, CT hh, const QpViewT<HT> &V, int64_t iq0,
@@ -448,7 +477,13 @@ __device__ __forceinline__ void element_forces_one(
           g[ii][jj] = (n == 0) ? U[n][ii] * b[jj] : g[ii][jj] + U[n][ii] * b[jj];
     }
     CT S[3][3];
-    strict_law<LAW, CT, HT>(V, iq0 + q, lambda, mu, sy, hh, g, S);
+    if constexpr (EXT) {
+      CT sth = CT(-0.0), dsth = CT(0);
+      ext_terms<LAW, CT, HT>(V, iq0 + q, g, sth, dsth);
+      strict_law<LAW, CT, HT>(V, iq0 + q, lambda, mu, sy, hh, g, S, sth, dsth);
+    } else {
+      strict_law<LAW, CT, HT>(V, iq0 + q, lambda, mu, sy, hh, g, S);
+    }
     if constexpr (LAW == LAW_USER_GRAD)
       continue;
 #pragma unroll
@@ -480,7 +515,7 @@ __device__ __forceinline__ void element_forces_one(
   }
 }
 
-template <int ET, int LAW, class CT, class FT>
+template <int ET, int LAW, class CT, class FT, bool EXT = false>
 __global__ void __launch_bounds__(256)
 k_element_forces(ElemArgs A) {
   constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
@@ -501,13 +536,13 @@ k_element_forces(ElemArgs A) {
   if (A.h_el)
     hh = CT(A.h_el[i]);
   if constexpr (ET == ET_TET4)
-    element_forces_one<ET, LAW, CT, FT, double>(A.conn + e * NN, NodesXU{A.xu}, lambda, mu,
-                                                sy, hh, qp_view(A, NQ), i * NQ,
-                                                A.pos + (A.pos_offset + i) * NN, A.fe);
+    element_forces_one<ET, LAW, CT, FT, double, NodesXU, 3, EXT>(
+        A.conn + e * NN, NodesXU{A.xu}, lambda, mu, sy, hh, qp_view(A, NQ), i * NQ,
+        A.pos + (A.pos_offset + i) * NN, A.fe);
   else
-    element_forces_one<ET, LAW, CT, FT, double>(A.conn + e * NN, NodesSoA{A.X, A.u, A.nn},
-                                                lambda, mu, sy, hh, qp_view(A, NQ), i * NQ,
-                                                A.pos + (A.pos_offset + i) * NN, A.fe);
+    element_forces_one<ET, LAW, CT, FT, double, NodesSoA, 3, EXT>(
+        A.conn + e * NN, NodesSoA{A.X, A.u, A.nn}, lambda, mu, sy, hh, qp_view(A, NQ),
+        i * NQ, A.pos + (A.pos_offset + i) * NN, A.fe);
 }
 
 
@@ -515,7 +550,7 @@ k_element_forces(ElemArgs A) {
 // records staged in LDS (the block lists of initFull): the same arithmetic
 // as k_element_forces on the same values, read from LDS instead of once per
 // element corner from global memory.
-template <int LAW, class CT, class FT>
+template <int LAW, class CT, class FT, bool EXT = false>
 __global__ void __launch_bounds__(256) k_tet4_strict_lds(ElemArgs A) {
   extern __shared__ double2 s_rec[]; // [bl_maxu][3]
   const int64_t blk = xcd_block(A.nblocks_padded);
@@ -543,7 +578,7 @@ __global__ void __launch_bounds__(256) k_tet4_strict_lds(ElemArgs A) {
   if (!active)
     return;
   const int4 lc = make_int4(l.x, l.y, l.z, l.w);
-  element_forces_one<ET_TET4, LAW, CT, FT, double>(
+  element_forces_one<ET_TET4, LAW, CT, FT, double, NodesLDS, 3, EXT>(
       reinterpret_cast<const int32_t *>(&lc), NodesLDS{reinterpret_cast<const double *>(s_rec)},
       lambda, mu, sy, hh, qp_view(A, 1), i, reinterpret_cast<const int32_t *>(&pv), A.fe);
 }
@@ -568,7 +603,7 @@ __global__ void __launch_bounds__(256) k_tet4_strict_lds(ElemArgs A) {
 // ---------------------------------------------------------------------------
 constexpr int HEX_STRICT_EPB = 8;
 
-template <int LAW, int EPB, class HT, class NA, int W = 3, bool PRE = false>
+template <int LAW, int EPB, class HT, class NA, int W = 3, bool PRE = false, bool EXT = false>
 __device__ __forceinline__ void hex8_strict_group(
     double (&sN)[EPB][8][6], const double (&sD)[8][3][8],
     double wq, int el, int q, bool active, const int32_t *conn_e, const NA &nodes_of,
@@ -656,7 +691,13 @@ __device__ __forceinline__ void hex8_strict_group(
           g[a][j] = (n == 0) ? un * b[j] : g[a][j] + un * b[j];
       }
     }
-    strict_law<LAW, double, HT>(V, iq, lambda, mu, sy, hh, g, S);
+    if constexpr (EXT) {
+      double sth = -0.0, dsth = 0.;
+      ext_terms<LAW, double, HT>(V, iq, g, sth, dsth);
+      strict_law<LAW, double, HT>(V, iq, lambda, mu, sy, hh, g, S, sth, dsth);
+    } else {
+      strict_law<LAW, double, HT>(V, iq, lambda, mu, sy, hh, g, S);
+    }
   }
   if constexpr (LAW == LAW_USER_GRAD)
     return; // the gradient pass forms no forces (uniform: no lane waits)
@@ -723,7 +764,7 @@ __device__ __forceinline__ void hex8_strict_group(
 // fills (k_hex8_jacobian_table): they depend on the reference coordinates
 // only, as the reference's FEEngine precompute does (fe_engine_template_tmpl
 // .hh:98-120), so the pass loads u but not X.
-template <int LAW>
+template <int LAW, bool EXT = false>
 __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
   __shared__ double sN[HEX_STRICT_EPB][8][6];
   __shared__ double sD[8][3][8];
@@ -745,7 +786,7 @@ __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
     if (A.h_el)
       hh = A.h_el[i];
   }
-  hex8_strict_group<LAW, HEX_STRICT_EPB, double, NodesSoA, 3, true>(
+  hex8_strict_group<LAW, HEX_STRICT_EPB, double, NodesSoA, 3, true, EXT>(
       sN, sD, c_hex8.w[q], el, q, active, A.conn + e * 8, NodesSoA{A.X, A.u, A.nn},
       lambda, mu, sy, hh, qp_view(A, 8), i * 8 + q, A.pos + (A.pos_offset + (active ? i : 0)) * 8,
       reinterpret_cast<double *>(A.fe), A.jw + e * 8 + q, A.nel * 8);

@@ -328,18 +328,20 @@ AKX_HD void neohookean_pk2_stable(R lambda, R mu, const R (&g)[3][3],
 // Linear isotropic hardening, small strain, radial return:
 // material_linear_isotropic_hardening_inline_impl.hh:31-86 and
 // material_plastic.hh:65-87.  Reads the previous state, writes the new one.
+// dsth: previous sigma_th - sigma_th, added to the trial stress's diagonal
+// (:33-47; 0 without a temperature field).
 template <class R>
 AKX_HD void plastic_lih(R lambda, R mu, R sigma_y, R h, const R (&g)[3][3],
                         const R (&pg)[3][3], const R (&ps)[3][3], R piso,
                         const R (&pe)[3][3], R (&sigma)[3][3], R &iso,
-                        R (&eps)[3][3]) {
+                        R (&eps)[3][3], R dsth = R(0)) {
   R dg[3][3], str[3][3], dev[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j)
       dg[i][j] = g[i][j] - pg[i][j];
-  elastic_stress(lambda, mu, dg, R(0), str);
+  elastic_stress(lambda, mu, dg, dsth, str);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll

@@ -523,6 +523,11 @@ struct Material {
   DevBuf<double> hgrad[2], hsig[2], heps[2], hiso[2];
   DevBuf<double> hpe[2]; // accumulated plastic energy [n*nqp]
   int cur = 0; // index of the current plastic state
+  // Material::eigengradu [9][n*nqp] (null: zero); MaterialThermal's sigma_th
+  // [n*nqp] (null: -0) and, for the plastic law, its history [2][n*nqp]
+  DevBuf<double> eig, sth_now, hsth[2];
+  std::vector<double> E_el_host, nu_el_host; // per-element E, nu as given
+  bool ext() const { return eig.p != nullptr || sth_now.p != nullptr; }
   // a law computed by the caller (the computeStress plugin)
   bool user() const { return law == AKX_LAW_USER || law == AKX_LAW_USER_FD; }
 };
@@ -890,7 +895,13 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
     A.c_iso = mat.hiso[c].p;
     A.p_pe = mat.hpe[p].p;
     A.c_pe = mat.hpe[c].p;
+    if (mat.sth_now.p) {
+      A.sth_p = mat.hsth[p].p;
+      A.sth_c = mat.hsth[c].p;
+    }
   }
+  A.eig = mat.eig.p;
+  A.sth_now = mat.sth_now.p;
   A.nblocks_padded = padded_blocks(mat.n);
   return A;
 }
@@ -923,30 +934,30 @@ void check_elem_args(const ElemArgs &A, int law) {
     require(A.grad_u && A.stress, "materialised internals incomplete");
 }
 
-template <int ET, class CT, class FT>
-void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
+template <int ET, class CT, class FT, bool EXT>
+void launch_elem_law_x(akx_model *m, Material &mat, const ElemArgs &A) {
   dim3 grid((unsigned)A.nblocks_padded), block(TPB);
   if constexpr (ET == ET_TET4) {
     if (A.lconn && TPB == 256) { // block node lists of a whole contiguous material
       const size_t lds = (size_t)A.bl_maxu * 48;
       switch (mat.law) {
       case AKX_LAW_ELASTIC:
-        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_ELASTIC, CT, FT>), grid, block, lds,
+        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_ELASTIC, CT, FT, EXT>), grid, block, lds,
                            m->stream, A);
         HIPCHECK(hipGetLastError());
         return;
       case AKX_LAW_ELASTIC_FD:
-        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_ELASTIC_FD, CT, FT>), grid, block, lds,
+        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_ELASTIC_FD, CT, FT, EXT>), grid, block, lds,
                            m->stream, A);
         HIPCHECK(hipGetLastError());
         return;
       case AKX_LAW_NEOHOOKEAN:
-        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_NEOHOOKEAN, CT, FT>), grid, block, lds,
+        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_NEOHOOKEAN, CT, FT, EXT>), grid, block, lds,
                            m->stream, A);
         HIPCHECK(hipGetLastError());
         return;
       case AKX_LAW_PLASTIC_LIH:
-        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_PLASTIC, CT, FT>), grid, block, lds,
+        hipLaunchKernelGGL((k_tet4_strict_lds<LAW_PLASTIC, CT, FT, EXT>), grid, block, lds,
                            m->stream, A);
         HIPCHECK(hipGetLastError());
         return;
@@ -957,27 +968,27 @@ void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
   }
   switch (mat.law) {
   case AKX_LAW_ELASTIC:
-    hipLaunchKernelGGL((k_element_forces<ET, LAW_ELASTIC, CT, FT>), grid, block,
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_ELASTIC, CT, FT, EXT>), grid, block,
                        0, m->stream, A);
     break;
   case AKX_LAW_ELASTIC_FD:
-    hipLaunchKernelGGL((k_element_forces<ET, LAW_ELASTIC_FD, CT, FT>), grid,
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_ELASTIC_FD, CT, FT, EXT>), grid,
                        block, 0, m->stream, A);
     break;
   case AKX_LAW_NEOHOOKEAN:
-    hipLaunchKernelGGL((k_element_forces<ET, LAW_NEOHOOKEAN, CT, FT>), grid,
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_NEOHOOKEAN, CT, FT, EXT>), grid,
                        block, 0, m->stream, A);
     break;
   case AKX_LAW_PLASTIC_LIH:
-    hipLaunchKernelGGL((k_element_forces<ET, LAW_PLASTIC, CT, FT>), grid, block,
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_PLASTIC, CT, FT, EXT>), grid, block,
                        0, m->stream, A);
     break;
   case AKX_LAW_USER:
-    hipLaunchKernelGGL((k_element_forces<ET, LAW_USER, CT, FT>), grid, block, 0,
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_USER, CT, FT, EXT>), grid, block, 0,
                        m->stream, A);
     break;
   case AKX_LAW_USER_FD:
-    hipLaunchKernelGGL((k_element_forces<ET, LAW_USER_FD, CT, FT>), grid, block, 0,
+    hipLaunchKernelGGL((k_element_forces<ET, LAW_USER_FD, CT, FT, EXT>), grid, block, 0,
                        m->stream, A);
     break;
   default:
@@ -986,32 +997,48 @@ void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
   HIPCHECK(hipGetLastError());
 }
 
-void launch_hex8_strict(akx_model *m, Material &mat, const ElemArgs &A) {
+template <int ET, class CT, class FT>
+void launch_elem_law(akx_model *m, Material &mat, const ElemArgs &A) {
+  if (mat.ext())
+    launch_elem_law_x<ET, CT, FT, true>(m, mat, A);
+  else
+    launch_elem_law_x<ET, CT, FT, false>(m, mat, A);
+}
+
+template <bool EXT>
+void launch_hex8_strict_x(akx_model *m, Material &mat, const ElemArgs &A) {
   require(A.jw != nullptr, "Hex8 element pass without its jacobian table");
   dim3 grid((unsigned)A.nblocks_padded), block(64);
   switch (mat.law) {
   case AKX_LAW_ELASTIC:
-    hipLaunchKernelGGL(k_hex8_strict<LAW_ELASTIC>, grid, block, 0, m->stream, A);
+    hipLaunchKernelGGL((k_hex8_strict<LAW_ELASTIC, EXT>), grid, block, 0, m->stream, A);
     break;
   case AKX_LAW_ELASTIC_FD:
-    hipLaunchKernelGGL(k_hex8_strict<LAW_ELASTIC_FD>, grid, block, 0, m->stream, A);
+    hipLaunchKernelGGL((k_hex8_strict<LAW_ELASTIC_FD, EXT>), grid, block, 0, m->stream, A);
     break;
   case AKX_LAW_NEOHOOKEAN:
-    hipLaunchKernelGGL(k_hex8_strict<LAW_NEOHOOKEAN>, grid, block, 0, m->stream, A);
+    hipLaunchKernelGGL((k_hex8_strict<LAW_NEOHOOKEAN, EXT>), grid, block, 0, m->stream, A);
     break;
   case AKX_LAW_PLASTIC_LIH:
-    hipLaunchKernelGGL(k_hex8_strict<LAW_PLASTIC>, grid, block, 0, m->stream, A);
+    hipLaunchKernelGGL((k_hex8_strict<LAW_PLASTIC, EXT>), grid, block, 0, m->stream, A);
     break;
   case AKX_LAW_USER:
-    hipLaunchKernelGGL(k_hex8_strict<LAW_USER>, grid, block, 0, m->stream, A);
+    hipLaunchKernelGGL((k_hex8_strict<LAW_USER, EXT>), grid, block, 0, m->stream, A);
     break;
   case AKX_LAW_USER_FD:
-    hipLaunchKernelGGL(k_hex8_strict<LAW_USER_FD>, grid, block, 0, m->stream, A);
+    hipLaunchKernelGGL((k_hex8_strict<LAW_USER_FD, EXT>), grid, block, 0, m->stream, A);
     break;
   default:
     throw Error(AKX_ERR_UNSUPPORTED, "unknown law");
   }
   HIPCHECK(hipGetLastError());
+}
+
+void launch_hex8_strict(akx_model *m, Material &mat, const ElemArgs &A) {
+  if (mat.ext())
+    launch_hex8_strict_x<true>(m, mat, A);
+  else
+    launch_hex8_strict_x<false>(m, mat, A);
 }
 
 void launch_element_pass(akx_model *m, bool materialize) {
@@ -1022,6 +1049,10 @@ void launch_element_pass(akx_model *m, bool materialize) {
       continue;
     ElemArgs A = elem_args(m, mat, materialize);
     check_elem_args(A, mat.law);
+    require(!mat.ext() || strict,
+            "eigen_grad_u and thermal stresses run on the plain FP64 path");
+    require(!(mat.sth_now.p && mat.law == AKX_LAW_PLASTIC_LIH) || (A.sth_p && A.sth_c),
+            "plastic thermal stress without its history");
     require(m->etype != AKX_TET4 || A.xu != nullptr, "Tet4 element pass without its node records");
     require(m->fe.n >= (size_t)(m->ell_total * 3 * m->fe_elem_size()) &&
                 m->fe.n >= (size_t)(m->bp_total * 3 * m->fe_elem_size()) &&
@@ -1076,11 +1107,19 @@ void user_stress_phase(akx_model *m) {
       require(A.jw != nullptr, "Hex8 gradient pass without its jacobian table");
       const int64_t nb = (mat.n + HEX_STRICT_EPB - 1) / HEX_STRICT_EPB;
       A.nblocks_padded = (nb + 7) / 8 * 8;
-      hipLaunchKernelGGL(k_hex8_strict<LAW_USER_GRAD>, dim3((unsigned)A.nblocks_padded),
-                         dim3(64), 0, m->stream, A);
+      if (mat.ext())
+        hipLaunchKernelGGL((k_hex8_strict<LAW_USER_GRAD, true>),
+                           dim3((unsigned)A.nblocks_padded), dim3(64), 0, m->stream, A);
+      else
+        hipLaunchKernelGGL((k_hex8_strict<LAW_USER_GRAD, false>),
+                           dim3((unsigned)A.nblocks_padded), dim3(64), 0, m->stream, A);
     } else {
-      hipLaunchKernelGGL((k_element_forces<ET_TET4, LAW_USER_GRAD, double, double>),
-                         dim3((unsigned)A.nblocks_padded), dim3(TPB), 0, m->stream, A);
+      if (mat.ext())
+        hipLaunchKernelGGL((k_element_forces<ET_TET4, LAW_USER_GRAD, double, double, true>),
+                           dim3((unsigned)A.nblocks_padded), dim3(TPB), 0, m->stream, A);
+      else
+        hipLaunchKernelGGL((k_element_forces<ET_TET4, LAW_USER_GRAD, double, double, false>),
+                           dim3((unsigned)A.nblocks_padded), dim3(TPB), 0, m->stream, A);
     }
     HIPCHECK(hipGetLastError());
     m->user_grad.resize(9 * nq);
@@ -1187,11 +1226,15 @@ void launch_predictor(akx_model *m) {
 // arithmetic.  Planned only for one contiguous non-plastic material whose
 // slice groups are monotone (otherwise the plain schedule is used).
 // ---------------------------------------------------------------------------
+// a change of the materials' inputs re-decides whether the pipelined
+// schedule applies
+void refresh_pipeline_plan(akx_model *m) { m->pipe_planned = false; }
+
 void plan_pipeline(akx_model *m) {
   m->pipe_planned = true;
   m->pipe_ok = false;
   const int P = m->pipe_parts;
-  if (P < 2 || m->mats.size() != 1 || m->bpa)
+  if (P < 2 || m->mats.size() != 1 || m->bpa || m->mats[0]->ext())
     return;
   Material &mat = *m->mats[0];
   if (!mat.contiguous || mat.law == AKX_LAW_PLASTIC_LIH || mat.first != 0 ||
@@ -1635,6 +1678,12 @@ int akx_model_set_element_parameters(akx_model *m, int material_id,
       HIPCHECK(hipMemcpy(b.p, src, n * sizeof(double), hipMemcpyHostToDevice));
     };
     if (E || nu) {
+      mat.E_el_host.assign(n, mat.E);
+      mat.nu_el_host.assign(n, mat.nu);
+      if (E)
+        mat.E_el_host.assign(E, E + n);
+      if (nu)
+        mat.nu_el_host.assign(nu, nu + n);
       DevBuf<double> dE, dnu;
       if (E)
         up(dE, E);
@@ -1658,10 +1707,68 @@ int akx_model_set_element_parameters(akx_model *m, int material_id,
   });
 }
 
+int akx_model_set_eigen_grad_u(akx_model *m, int material_id, const double *eigen_grad_u) {
+  return guard([&] {
+    require(m != nullptr && m->etype >= 0, "set the mesh first");
+    require(material_id >= 0 && material_id < (int)m->mats.size(), "bad material id");
+    if (m->precision != AKX_PRECISION_FP64)
+      throw Error(AKX_ERR_UNSUPPORTED, "eigen_grad_u runs on the plain FP64 path");
+    m->set_device();
+    Material &mat = *m->mats[material_id];
+    if (!eigen_grad_u || mat.n == 0) {
+      mat.eig.release();
+      return;
+    }
+    const int64_t nq = mat.n * m->nqp;
+    mat.eig.alloc(9 * nq);
+    upload_soa(m, eigen_grad_u, mat.eig.p, nq, 9);
+    refresh_pipeline_plan(m);
+  });
+}
+
+int akx_model_set_thermal(akx_model *m, int material_id, double alpha, const double *delta_T) {
+  return guard([&] {
+    require(m != nullptr && m->etype >= 0, "set the mesh first");
+    require(material_id >= 0 && material_id < (int)m->mats.size(), "bad material id");
+    if (m->precision != AKX_PRECISION_FP64)
+      throw Error(AKX_ERR_UNSUPPORTED, "thermal stresses run on the plain FP64 path");
+    m->set_device();
+    Material &mat = *m->mats[material_id];
+    if (!delta_T || mat.n == 0) {
+      mat.sth_now.release();
+      return;
+    }
+    require(std::isfinite(alpha), "non-finite alpha");
+    const int64_t nq = mat.n * m->nqp;
+    // sigma_th = -E / (1 - 2 nu) * alpha * delta_T (material_thermal.hh:91-95)
+    std::vector<double> sth(nq);
+    for (int64_t i = 0; i < mat.n; ++i) {
+      const double E = mat.E_el_host.empty() ? mat.E : mat.E_el_host[i];
+      const double nu = mat.nu_el_host.empty() ? mat.nu : mat.nu_el_host[i];
+      for (int q = 0; q < m->nqp; ++q)
+        sth[i * m->nqp + q] = -E / (1. - 2. * nu) * alpha * delta_T[i * m->nqp + q];
+    }
+    mat.sth_now.alloc(nq);
+    HIPCHECK(hipMemcpy(mat.sth_now.p, sth.data(), nq * sizeof(double), hipMemcpyHostToDevice));
+    if (mat.law == AKX_LAW_PLASTIC_LIH && !mat.hsth[0].p) {
+      // the history starts at zero (material_thermal.cc:34)
+      for (int k = 0; k < 2; ++k) {
+        mat.hsth[k].alloc(nq);
+        zero_async(m, mat.hsth[k].p, nq);
+      }
+      HIPCHECK(hipStreamSynchronize(m->stream));
+    }
+    refresh_pipeline_plan(m);
+  });
+}
+
 int akx_model_set_precision(akx_model *m, int precision) {
   return guard([&] {
     require(m != nullptr, "null model");
     require(!m->initialized, "precision must be set before initFull");
+    for (auto &mp : m->mats)
+      require(precision == AKX_PRECISION_FP64 || !mp->ext(),
+              "eigen_grad_u and thermal stresses run on the plain FP64 path");
     require(precision == AKX_PRECISION_FP64 || precision == AKX_PRECISION_FP32 ||
                 precision == AKX_PRECISION_FP64_FAST,
             "unknown precision");

@@ -152,7 +152,66 @@ class Material:
         self._param_doc[name] = description
 
     def getReal(self, name):
+        if name == "alpha":
+            return self._alpha
         return float(self.params[name])
+
+    # MaterialThermal's parameters (material_thermal.cc:36-43): "alpha" and
+    # "delta_T", the uniform temperature field; the plain FP64 path applies
+    # sigma_th = -E / (1 - 2 nu) alpha delta_T (akx_model_set_thermal)
+    _alpha = 0.0
+    _delta_T = None
+
+    def setReal(self, name, value):
+        """ParameterRegistry::set (py_parser.cc:68-69) for the parameters a
+        running model can change here: "alpha" and "delta_T"."""
+        if name == "alpha":
+            self._alpha = float(value)
+        elif name == "delta_T":
+            self._delta_T = np.full(self._nq(), float(value))
+        else:
+            raise NotImplementedError(f"parameter {name!r} cannot be changed on this path")
+        self._push_thermal()
+
+    def setDeltaT(self, delta_T):
+        """The "delta_T" internal field, one value per Gauss point
+        ([nb_elements * nqp], the material's element order)."""
+        d = np.ascontiguousarray(delta_T, np.float64).reshape(-1)
+        if d.size != self._nq():
+            raise ValueError("delta_T needs one value per Gauss point")
+        self._delta_T = d
+        self._push_thermal()
+
+    def _nq(self):
+        return self.nb_elements * self._model.nqp
+
+    def _push_thermal(self):
+        if self._delta_T is None:
+            return
+        N.check(N.lib().akx_model_set_thermal(self._model._h, self.id, self._alpha,
+                                              _p(self._delta_T)))
+        self._model._stresses_fresh = False
+
+    def applyEigenGradU(self, prescribed_eigen_grad_u, ghost_type=_not_ghost):
+        """Material::applyEigenGradU (material.cc:811-824): the same 3x3
+        eigen grad u at every Gauss point of the material."""
+        g = np.asarray(prescribed_eigen_grad_u, np.float64).reshape(3, 3)
+        self.setEigenGradU(np.broadcast_to(g, (self._nq(), 3, 3)))
+
+    def setEigenGradU(self, eigen_grad_u):
+        """The "eigen_grad_u" internal field: [nb_elements * nqp][3][3] (or
+        [..][9] column-major, du_i/dX_j at i + 3 j); None clears it."""
+        if eigen_grad_u is None:
+            N.check(N.lib().akx_model_set_eigen_grad_u(self._model._h, self.id, None))
+        else:
+            e = np.asarray(eigen_grad_u, np.float64)
+            if e.ndim == 3:  # matrices -> column-major rows
+                e = np.transpose(e, (0, 2, 1)).reshape(-1, 9)
+            e = np.ascontiguousarray(e.reshape(-1, 9))
+            if e.shape[0] != self._nq():
+                raise ValueError("eigen_grad_u needs one matrix per Gauss point")
+            N.check(N.lib().akx_model_set_eigen_grad_u(self._model._h, self.id, _p(e)))
+        self._model._stresses_fresh = False
 
     def initMaterial(self):
         """Called once at initFull (Material::initMaterial)."""

@@ -58,6 +58,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_hex8_neohookean_f64.json"
 # rocprofv3 (SQ_INSTS_VALU_FLOPS_FP32 / _FP64 x 64 lanes)
 FLOPS_FILE = os.path.join(ROOT, "profiles", "flops_hex8_neohookean_200x100x100.json")
 PL = "plastic_linear_isotropic_hardening"
+TIMING_STEPS = 100  # steps of the event-timed per-kernel pass
 
 
 def build_c2(shape=SHAPE):
@@ -150,6 +151,22 @@ def run_plain(args):
     schedule = "plain"
     if args.pipeline > 1 and model.setPipeline(args.pipeline):
         schedule = f"pipelined, {args.pipeline} parts"
+    # per-kernel times first: a separate pass of TIMING_STEPS steps with HIP
+    # events around every launch (plain schedule), so the timed loop below
+    # carries no event overhead.  Running it first also brings the GPU clocks
+    # to their steady state: after the idle set-up, the element passes of the
+    # first ~20 steps run up to 25 % slower while the clocks ramp.
+    model.enableKernelTiming(True)
+    model.solveSteps(TIMING_STEPS)
+    model.synchronize()
+    kern = {}
+    names = {0: "element_pass", 1: "gather_update", 2: "predictor"}
+    for k, name in names.items():
+        ms, n = model.kernelTiming(k)
+        if n:
+            kern[name] = {"avg_ms": ms / n, "launches": n, "bytes": model.kernelBytes(k)}
+    model.enableKernelTiming(False)
+    copy_gbs = copy_bandwidth()
     model.solveSteps(args.warmup)
     model.synchronize()
     t0 = time.perf_counter()
@@ -159,21 +176,8 @@ def run_plain(args):
     nqp = mesh.nb_elements * 8
     value = nqp * steps / elapsed
     ms_step = 1e3 * elapsed / steps
-    # per-kernel times: a separate pass with HIP events around every launch
-    # (plain schedule), so the timed loop above carries no event overhead
-    model.enableKernelTiming(True)
-    model.solveSteps(min(steps, 100))
-    model.synchronize()
-
-    kern = {}
-    names = {0: "element_pass", 1: "gather_update", 2: "predictor"}
-    for k, name in names.items():
-        ms, n = model.kernelTiming(k)
-        if n:
-            kern[name] = {"avg_ms": ms / n, "launches": n, "bytes": model.kernelBytes(k)}
     el = kern["element_pass"]
     hbm_gbs = el["bytes"] / (el["avg_ms"] * 1e-3) / 1e9
-    copy_gbs = copy_bandwidth()
     if args.precision == "fp64":
         flops = HEX8_NEOHOOKEAN_FLOP_PER_QP * nqp
         tflops = flops / (el["avg_ms"] * 1e-3) / 1e12
@@ -189,7 +193,7 @@ def run_plain(args):
         # reads run at a third of the HBM bandwidth.  MI355X's FP64 peak is
         # the same 78.6 TFLOP/s on the matrix cores and the vector ALUs; the
         # reference operation order is FMA-free, so its own ceiling is half.
-        roofline = {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
+        roofline = {"bound": "fp64_valu", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
                     "traffic": traffic,
                     "kernel": "k_hex8_strict<neo-Hookean> (FP64, reference operation order)",
@@ -207,7 +211,7 @@ def run_plain(args):
             fq = json.load(fh)["f32_sf"]["flops_per_qp"]
         flops = fq * nqp
         tflops = flops / (el["avg_ms"] * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
+        roofline = {"bound": "fp32_valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS, "traffic": None,
                     "kernel": "k_hex8_sf<neo-Hookean> (FP32, sum-factorised)",
                     "compute": "FP32 VALU; packed FP32 instructions are not used, so the "
@@ -260,7 +264,7 @@ def run_plain(args):
         print("# forms: FP32 form, 512-instance batch, serving stream", file=sys.stderr,
               flush=True)
         out["forms"] = run_forms(args, model, mesh, blocked, v0, dt, value,
-                                 args.warmup + steps + min(steps, 100))
+                                 TIMING_STEPS + args.warmup + steps)
         out["north_star"] = north_star_summary(out)
     return out
 
@@ -270,16 +274,19 @@ def run_plain(args):
 # --------------------------------------------------------------------------
 def run_forms(args, plain, mesh, blocked, v0, dt, plain_value, plain_steps):
     """The precision, batching and serving forms measured in the same run as
-    the plain FP64 line, each on its own workload:
+    the plain FP64 line, each on its own workload, each also against the
+    plain FP64 arithmetic on that same workload:
     - precision: the headline workload (200 x 100 x 100 Hex8 neo-Hookean) in
-      the FP32 form, same warmup / timed / kernel-timing steps as the plain
-      line, and its u, v, f_int errors against the plain model after the same
-      number of steps (the plain path is bit-pinned to the oracle);
+      the FP32 form, the same kernel-timing / warmup / timed steps as the
+      plain line, and its u, v, f_int errors against the plain model after
+      the same number of steps (the plain path is bit-pinned to the oracle);
     - batch: the 512 Tet4 plasticity instances (5e3-4e5 elements, seed 4) in
-      the FP32 form, autotuned, hipGraph, the same timed step count;
+      the FP32 form and in FP64 (the plain path's arithmetic), autotuned,
+      hipGraph, the same timed step count;
     - serve: a Poisson stream of 24 requests at 8/s from the serving
       workload's distribution (Tet4 / Hex8, 1e3-2e6 elements, 200-5,000
-      steps), FP32 form, continuous batching: throughput and p50 / p99
+      steps), continuous batching, FP32 form and the same stream in FP64:
+      throughput over the makespan and over the device-busy time, p50 / p99
       time-to-result."""
     from akantu2_amd import SolidMechanicsModel
     from akantu2_amd import serving as S
@@ -293,21 +300,22 @@ def run_forms(args, plain, mesh, blocked, v0, dt, plain_value, plain_steps):
     m.getBlockedDOFs()[:] = blocked
     m.getVelocity()[:] = v0
     m.setTimeStep(dt)
-    m.solveSteps(args.warmup)
-    m.synchronize()
-    t0 = time.perf_counter()
-    m.solveSteps(steps)
-    m.synchronize()
-    el = time.perf_counter() - t0
     m.enableKernelTiming(True)
-    m.solveSteps(min(steps, 100))
+    m.solveSteps(TIMING_STEPS)
     m.synchronize()
     kern = {}
     for k, name in {0: "element_pass", 1: "gather_update"}.items():
         ms, n = m.kernelTiming(k)
         by = m.kernelBytes(k)
         kern[name] = {"avg_ms": ms / n, "bytes": by, "achieved_gbs": by / (ms / n * 1e-3) / 1e9}
-    assert args.warmup + steps + min(steps, 100) == plain_steps
+    m.enableKernelTiming(False)
+    m.solveSteps(args.warmup)
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.solveSteps(steps)
+    m.synchronize()
+    el = time.perf_counter() - t0
+    assert TIMING_STEPS + args.warmup + steps == plain_steps
     err = {"u": rel_errors(m.getDisplacement(), plain.getDisplacement()),
            "v": rel_errors(m.getVelocity(), plain.getVelocity()),
            "f_int": rel_errors(m.getInternalForce(), plain.getInternalForce())}
@@ -321,57 +329,75 @@ def run_forms(args, plain, mesh, blocked, v0, dt, plain_value, plain_steps):
     del m
     print(f"# forms: precision {1e3 * el / steps:.3f} ms/step", file=sys.stderr, flush=True)
 
-    t = time.perf_counter()
     specs = c4_instances(512)
-    b, ids = _fill_batch("fp32", specs)
-    st = b.stats()
-    setup = time.perf_counter() - t
-    el, bk, best = _time_batch(b, 3, steps, tune=True)
-    b.close()
-    value = st["quadrature_points"] * steps / el
-    forms["batch"] = {
+    batch = {}
+    for prec in ("fp32", "fp64"):
+        t = time.perf_counter()
+        b, ids = _fill_batch(prec, specs)
+        st = b.stats()
+        setup = time.perf_counter() - t
+        el, bk, best = _time_batch(b, 3, steps, tune=True)
+        b.close()
+        batch[prec] = {"value": st["quadrature_points"] * steps / el,
+                       "ms_per_step": 1e3 * el / steps, "kernels": bk, "launch": best,
+                       "hbm_frac": {k: v["achieved_gbs"] / HBM_PEAK_GBS for k, v in bk.items()},
+                       "setup_s": setup}
+        print(f"# forms: batch {prec} {1e3 * el / steps:.3f} ms/step, setup {setup:.1f} s",
+              file=sys.stderr, flush=True)
+    forms["batch"] = dict(batch["fp32"])
+    forms["batch"].update({
         "workload": "512 Tet4 boxes of 5e3-4e5 elements (log-uniform, seed 4), aspect "
                     "U(1, 8), linear isotropic hardening with per-instance E, nu, sigma_y, "
                     "h, FP32 form, persistent launches in a hipGraph, autotuned",
-        "instances": st["live"], "elements": st["elements"], "value": value,
-        "unit": "elem*qp/s", "ms_per_step": 1e3 * el / steps, "steps": steps,
-        "vs_plain_fp64": value / plain_value, "kernels": bk, "launch": best,
-        "hbm_frac": {k: v["achieved_gbs"] / HBM_PEAK_GBS for k, v in bk.items()},
-        "setup_s": setup}
-    print(f"# forms: batch {1e3 * el / steps:.3f} ms/step, setup {setup:.1f} s",
-          file=sys.stderr, flush=True)
+        "instances": st["live"], "elements": st["elements"], "unit": "elem*qp/s",
+        "steps": steps, "vs_plain_fp64": batch["fp32"]["value"] / plain_value,
+        "fp64_same_workload": batch["fp64"],
+        "vs_fp64_same_workload": batch["fp32"]["value"] / batch["fp64"]["value"]})
 
-    reqs = S.synthetic_requests(24, rate=8.0, seed=5)
-    for r in reqs:
-        r.materialize()
-    server = S.Server(precision="fp32", hbm_bytes=16 << 30)
-    rep = server.run(reqs)
-    server.close()
-    tt = np.array([r.time_to_result for r in rep.done])
-    forms["serve"] = {
+    served = {}
+    for prec in ("fp32", "fp64"):
+        reqs = S.synthetic_requests(24, rate=8.0, seed=5)
+        for r in reqs:
+            r.materialize()
+        server = S.Server(precision=prec, hbm_bytes=16 << 30)
+        rep = server.run(reqs)
+        server.close()
+        tt = np.array([r.time_to_result for r in rep.done])
+        served[prec] = {
+            "value": rep.throughput, "unit": "elem*qp/s",
+            "busy_throughput": sum(r.work for r in rep.done) / rep.busy if rep.busy else None,
+            "latency_s": {"p50": float(np.percentile(tt, 50)),
+                          "p99": float(np.percentile(tt, 99)), "max": float(tt.max())},
+            "makespan_s": rep.makespan, "done": len(rep.done), "failed": len(rep.failed),
+            "max_live": rep.max_live, "chunks": rep.chunks}
+        print(f"# forms: serve {prec} p50 {served[prec]['latency_s']['p50']:.2f} s",
+              file=sys.stderr, flush=True)
+    forms["serve"] = dict(served["fp32"])
+    forms["serve"].update({
         "workload": "Poisson stream of 24 requests at 8/s, Tet4 or Hex8 boxes of 1e3-2e6 "
                     "elements (log-uniform), elastic or neo-Hookean, 200-5,000 steps, FP32 "
                     "form, continuous batching with HBM-footprint admission",
-        "value": rep.throughput, "unit": "elem*qp/s",
-        "vs_plain_fp64": rep.throughput / plain_value,
-        "busy_throughput": sum(r.work for r in rep.done) / rep.busy if rep.busy else None,
-        "latency_s": {"p50": float(np.percentile(tt, 50)), "p99": float(np.percentile(tt, 99)),
-                      "max": float(tt.max())},
-        "makespan_s": rep.makespan, "done": len(rep.done), "failed": len(rep.failed),
-        "max_live": rep.max_live, "chunks": rep.chunks}
-    print(f"# forms: serve p50 {forms['serve']['latency_s']['p50']:.2f} s", file=sys.stderr,
-          flush=True)
+        "vs_plain_fp64": served["fp32"]["busy_throughput"] / plain_value,
+        "fp64_same_stream": served["fp64"],
+        "vs_fp64_same_stream": served["fp32"]["busy_throughput"] /
+                               served["fp64"]["busy_throughput"]})
     return forms
 
 
 def north_star_summary(out):
     """The target stated for the forms: the FP32 batched / served form at >= 3x
     the plain FP64 path's element*qp throughput, <= 1e-5 relative
-    displacement error, >= 70 % of peak HBM bandwidth."""
+    displacement error, >= 70 % of peak HBM bandwidth.  Ratios are given
+    against the headline plain line and against the plain FP64 arithmetic on
+    each form's own workload (served forms: device-busy throughput)."""
     f = out["forms"]
     return {"target_ratio": 3.0, "target_hbm_frac": 0.70, "target_u_error": 1e-5,
             "plain_fp64_value": out["value"],
-            "ratios": {k: f[k]["vs_plain_fp64"] for k in ("precision", "batch", "serve")},
+            "ratios_vs_headline": {k: f[k]["vs_plain_fp64"]
+                                   for k in ("precision", "batch", "serve")},
+            "ratios_same_workload": {"precision": f["precision"]["vs_plain_fp64"],
+                                     "batch": f["batch"]["vs_fp64_same_workload"],
+                                     "serve": f["serve"]["vs_fp64_same_stream"]},
             "precision_u_error": f["precision"]["error"]["u"]["max"],
             "batch_hbm_frac": f["batch"]["hbm_frac"],
             "precision_hbm_frac": {k: v["achieved_gbs"] / HBM_PEAK_GBS
@@ -425,13 +451,21 @@ def run_precision(args):
     mesh, blocked, v0, E, nu = build_c3(shape)
     print(f"# mesh {time.perf_counter() - t:.1f} s, {mesh.nb_elements} Tet4",
           file=sys.stderr, flush=True)
-    res = {}
+    res, models = {}, {}
     for prec in ("fp32", "fp64"):
         m, el, kern = _timed_model(mesh, blocked, v0, E, nu, prec, args.warmup, steps)
         res[prec] = {"value": mesh.nb_elements * steps / el, "ms_per_step": 1e3 * el / steps,
                      "kernels": kern}
         print(f"# {prec}: {1e3 * el / steps:.3f} ms/step", file=sys.stderr, flush=True)
-        del m
+        models[prec] = m
+    # both ran warmup + steps + 10 steps: the FP32 form against the plain
+    # FP64 path (bit-identical to the oracle at this size) on the full mesh
+    g32, g64 = models["fp32"], models["fp64"]
+    full_error = {"after_steps": args.warmup + steps + 10,
+                  "u": rel_errors(g32.getDisplacement(), g64.getDisplacement()),
+                  "v": rel_errors(g32.getVelocity(), g64.getVelocity()),
+                  "f_int": rel_errors(g32.getInternalForce(), g64.getInternalForce())}
+    del g32, g64, models
     el = res["fp32"]["kernels"]["element_pass"]
     out = {
         "metric": "elem*qp stress+force updates/s", "value": res["fp32"]["value"],
@@ -449,6 +483,7 @@ def run_precision(args):
                      "avg_kernel_ms": el["avg_ms"]},
         "forms": res,
         "fp32_over_fp64": res["fp32"]["value"] / res["fp64"]["value"],
+        "fp32_error_vs_plain_fp64_full_mesh": full_error,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_precision()

@@ -136,6 +136,25 @@ int akx_model_set_element_parameters(akx_model *m, int material_id,
 int akx_material_law_by_name(const char *name, int finite_deformation,
                              int *law);
 
+/* Material::eigengradu (material.cc:35-36,80-85; applyEigenGradU
+ * material.cc:811-824): subtracted from grad u at every Gauss point of the
+ * material before computeStress (gradu_vect -= eigengradu, material.cc:147),
+ * so the law, the finite-deformation force (B_NL) and the plastic history all
+ * see grad u - eigen_grad_u.  eigen_grad_u [nb_filter * nqp][9]
+ * (column-major, points in the material's element order); NULL: zero (the
+ * default).  Plain FP64 path only (AKX_ERR_UNSUPPORTED otherwise). */
+int akx_model_set_eigen_grad_u(akx_model *m, int material_id, const double *eigen_grad_u);
+/* MaterialThermal (material_thermal.cc:30-44, .hh:91-95): alpha and the
+ * temperature change delta_T [nb_filter * nqp] give sigma_th = -E / (1 -
+ * 2 nu) * alpha * delta_T at every Gauss point (the material's E, nu, per
+ * element when set); the elastic laws add it to the diagonal of the stress
+ * (material_elastic_inline_impl.hh:33-49; of S for finite deformation), the
+ * plastic law adds previous sigma_th - sigma_th to its trial stress
+ * (material_linear_isotropic_hardening_inline_impl.hh:33-47), the
+ * neo-Hookean law does not read it.  delta_T NULL: no thermal stress (the
+ * default, alpha = 0).  Plain FP64 path only. */
+int akx_model_set_thermal(akx_model *m, int material_id, double alpha, const double *delta_T);
+
 int akx_model_set_precision(akx_model *m, int precision);
 
 /* computeStress of the AKX_LAW_USER / AKX_LAW_USER_FD materials.  Called on

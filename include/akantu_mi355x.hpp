@@ -162,6 +162,15 @@ public:
   // Material::getGradU / getStress (or PK2 for finite deformation)
   Array<double> getGradU() const;
   Array<double> getStress() const;
+  // Material::applyEigenGradU (material.cc:811-824): the same eigen grad u
+  // (column-major 3x3, du_i/dX_j at i + 3 j) at every Gauss point, or one
+  // per Gauss point ([n * nqp][9]); subtracted from grad u before the law
+  void applyEigenGradU(const double (&eigen_grad_u)[9]);
+  void setEigenGradU(const Array<double> &eigen_grad_u);
+  // MaterialThermal's "alpha" and "delta_T" (uniform, or one value per
+  // Gauss point): sigma_th = -E / (1 - 2 nu) alpha delta_T
+  void setThermal(double alpha, double delta_T);
+  void setThermal(double alpha, const Array<double> &delta_T);
 
 private:
   SolidMechanicsModel *model;
@@ -281,7 +290,7 @@ public:
 private:
   friend class Material;
   static constexpr int NF = 6;
-  static constexpr std::uint32_t ALL = (1u << NF) - 1 | 1u << AKX_FIELD_BLOCKED_DOFS;
+  static constexpr std::uint32_t ALL = ((1u << NF) - 1) | (1u << AKX_FIELD_BLOCKED_DOFS);
   static void check_solver(const std::string &id) {
     if (!id.empty() && id != "explicit_lumped")
       throw Exception(AKX_ERR_INVALID, "The non linear solver dof_manager:tss:" + id +
@@ -416,6 +425,30 @@ inline Array<double> Material::getStress() const {
   Array<double> out(n * model->nqp, 9);
   check(akx_model_get_internal(model->h, id, AKX_INTERNAL_STRESS, out.data()));
   return out;
+}
+
+inline void Material::applyEigenGradU(const double (&eigen_grad_u)[9]) {
+  Array<double> e(n * model->nqp, 9);
+  for (std::int64_t q = 0; q < n * model->nqp; ++q)
+    for (int k = 0; k < 9; ++k)
+      e(q, k) = eigen_grad_u[k];
+  setEigenGradU(e);
+}
+
+inline void Material::setEigenGradU(const Array<double> &eigen_grad_u) {
+  if (eigen_grad_u.size() != n * model->nqp || eigen_grad_u.getNbComponent() != 9)
+    throw Exception(AKX_ERR_INVALID, "eigen_grad_u needs [nb_qp][9]");
+  check(akx_model_set_eigen_grad_u(model->h, id, eigen_grad_u.data()));
+}
+
+inline void Material::setThermal(double alpha, double delta_T) {
+  setThermal(alpha, Array<double>(n * model->nqp, 1, delta_T));
+}
+
+inline void Material::setThermal(double alpha, const Array<double> &delta_T) {
+  if (delta_T.size() != n * model->nqp)
+    throw Exception(AKX_ERR_INVALID, "delta_T needs one value per Gauss point");
+  check(akx_model_set_thermal(model->h, id, alpha, delta_T.data()));
 }
 
 } // namespace akantu_mi355x

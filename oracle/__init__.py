@@ -43,6 +43,8 @@ class _Model(C.Structure):
         ("prev_eps_p", _dp), ("iso", _dp), ("prev_iso", _dp),
         ("plastic_energy", _dp),
         ("dt", C.c_double), ("step", C.c_long),
+        ("eigen_grad_u", _dp), ("alpha", C.c_double), ("delta_T", _dp),
+        ("sigma_th", _dp), ("prev_sigma_th", _dp),
     ]
 
 
@@ -97,6 +99,10 @@ def lib():
         L.orc_law_stress.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
                                      C.c_double, _dp, _dp, _dp, C.c_double, _dp,
                                      _dp, _dp, _dp]
+        L.orc_law_stress_th.restype = None
+        L.orc_law_stress_th.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, _dp, _dp, _dp, C.c_double, _dp,
+                                        C.c_double, C.c_double, _dp, _dp, _dp]
         L.orc_lumped_step_cb.restype = None
         L.orc_lumped_step_cb.argtypes = [C.c_long, C.c_double, _dp, _dp, _dp, _bp,
                                          _dp, RESIDUAL_CB, C.c_void_p]
@@ -166,7 +172,36 @@ class OracleModel:
         s.iso, s.prev_iso = _ptr(self.iso), _ptr(self.prev_iso)
         s.plastic_energy = _ptr(self.plastic_energy)
         s.dt, s.step = 0.0, 0
+        self.eigen_grad_u = self.delta_T = self.sigma_th = self.prev_sigma_th = None
+        s.alpha = 0.0
         self._s = s
+
+    def set_eigen_grad_u(self, eig):
+        """Material::eigengradu: a 3x3 matrix for every quadrature point
+        (the "eigen_grad_u" parameter, material.cc:80-85) or an array
+        [nel*nqp][3][3]; None clears it."""
+        if eig is None:
+            self.eigen_grad_u = None
+        else:
+            e = np.asarray(eig, np.float64)
+            if e.shape == (3, 3):
+                e = np.broadcast_to(e, (self.nel * self.nqp, 3, 3))
+            # column-major 3x3 per point, as grad_u
+            self.eigen_grad_u = np.ascontiguousarray(
+                np.transpose(e.reshape(-1, 3, 3), (0, 2, 1)).reshape(-1, 9))
+        self._s.eigen_grad_u = _ptr(self.eigen_grad_u)
+
+    def set_thermal(self, alpha, delta_T):
+        """MaterialThermal's alpha and delta_T (a value for every quadrature
+        point, or an array [nel*nqp])."""
+        nq = self.nel * self.nqp
+        self.delta_T = np.ascontiguousarray(np.broadcast_to(
+            np.asarray(delta_T, np.float64), (nq,)))
+        if self.sigma_th is None:
+            self.sigma_th, self.prev_sigma_th = np.zeros(nq), np.zeros(nq)
+        self._s.alpha = float(alpha)
+        self._s.delta_T = _ptr(self.delta_T)
+        self._s.sigma_th, self._s.prev_sigma_th = _ptr(self.sigma_th), _ptr(self.prev_sigma_th)
 
     # initFull(_explicit_lumped_mass): precompute + lumped mass
     def init_full(self):
@@ -275,8 +310,10 @@ def lame(E, nu):
 
 
 def law_stress(law, lam, mu, grad_u, sigma_y=0.0, h=0.0, prev_grad_u=None,
-               prev_sigma=None, prev_iso=0.0, prev_eps_p=None):
-    """Per-quadrature-point constitutive law; tensors are 3x3 numpy arrays."""
+               prev_sigma=None, prev_iso=0.0, prev_eps_p=None, sigma_th=None,
+               delta_sigma_th=0.0):
+    """Per-quadrature-point constitutive law; tensors are 3x3 numpy arrays.
+    sigma_th / delta_sigma_th: the thermal terms (None: the default -0)."""
     law = LAWS[law] if isinstance(law, str) else int(law)
     col = lambda m: np.ascontiguousarray(np.asarray(m, np.float64).T).reshape(9)
     g = col(grad_u)
@@ -285,8 +322,13 @@ def law_stress(law, lam, mu, grad_u, sigma_y=0.0, h=0.0, prev_grad_u=None,
     pe = col(prev_eps_p if prev_eps_p is not None else np.zeros((3, 3)))
     sig, eps = np.zeros(9), np.zeros(9)
     iso = C.c_double(0.0)
-    lib().orc_law_stress(law, lam, mu, sigma_y, h, _ptr(g), _ptr(pg), _ptr(ps),
-                         prev_iso, _ptr(pe), _ptr(sig), C.cast(C.byref(iso), _dp),
-                         _ptr(eps))
+    if sigma_th is None:
+        lib().orc_law_stress(law, lam, mu, sigma_y, h, _ptr(g), _ptr(pg), _ptr(ps),
+                             prev_iso, _ptr(pe), _ptr(sig), C.cast(C.byref(iso), _dp),
+                             _ptr(eps))
+    else:
+        lib().orc_law_stress_th(law, lam, mu, sigma_y, h, _ptr(g), _ptr(pg), _ptr(ps),
+                                prev_iso, _ptr(pe), float(sigma_th), float(delta_sigma_th),
+                                _ptr(sig), C.cast(C.byref(iso), _dp), _ptr(eps))
     unc = lambda v: v.reshape(3, 3).T.copy()
     return unc(sig), iso.value, unc(eps)

@@ -484,13 +484,14 @@ static void neohookean_on_quad(double lambda, double mu, const double *g,
 static void plastic_lih_on_quad(double lambda, double mu, double sigma_y,
                                 double h, const double *g, const double *pg,
                                 const double *psigma, double piso,
-                                const double *peps, double *sigma,
+                                const double *peps, double dsth, double *sigma,
                                 double *iso, double *eps) {
   double dg[9], str[9], dev[9];
   for (int k = 0; k < 9; ++k)
     dg[k] = g[k] - pg[k];
-  /* trial stress: elastic on grad(delta u) with delta sigma_th = 0 */
-  elastic_on_quad(lambda, mu, dg, 0., str);
+  /* trial stress: elastic on grad(delta u) with delta sigma_th = previous
+   * sigma_th - sigma_th (:33-36; 0 without a temperature field) */
+  elastic_on_quad(lambda, mu, dg, dsth, str);
   for (int k = 0; k < 9; ++k)
     str[k] += psigma[k];
   double tr = T(str, 0, 0) + T(str, 1, 1) + T(str, 2, 2);
@@ -522,31 +523,49 @@ static void plastic_lih_on_quad(double lambda, double mu, double sigma_y,
   }
 }
 
-void orc_law_stress(int law, double lambda, double mu, double sigma_y,
-                    double h, const double *g, const double *pg,
-                    const double *psigma, double piso, const double *peps,
-                    double *sigma, double *iso, double *eps) {
+void orc_law_stress_th(int law, double lambda, double mu, double sigma_y,
+                       double h, const double *g, const double *pg,
+                       const double *psigma, double piso, const double *peps,
+                       double sth, double dsth, double *sigma, double *iso,
+                       double *eps) {
   switch (law) {
   case ORC_ELASTIC:
-    /* sigma_th = -E/(1-2nu) alpha dT = -0 with the default alpha = 0
-     * (material_thermal.hh:91-95); adding -0 changes nothing */
-    elastic_on_quad(lambda, mu, g, -0., sigma);
+    elastic_on_quad(lambda, mu, g, sth, sigma);
     break;
   case ORC_ELASTIC_FD: {
     /* material_elastic.cc:92-96: the law applied to the Green strain */
     double E[9];
     grad_u_to_green(g, E);
-    elastic_on_quad(lambda, mu, E, -0., sigma);
+    elastic_on_quad(lambda, mu, E, sth, sigma);
     break;
   }
   case ORC_NEOHOOKEAN:
+    /* material_neohookean_inline_impl.hh:39-50 does not read sigma_th */
     neohookean_on_quad(lambda, mu, g, sigma);
     break;
   case ORC_PLASTIC_LIH:
     plastic_lih_on_quad(lambda, mu, sigma_y, h, g, pg, psigma, piso, peps,
-                        sigma, iso, eps);
+                        dsth, sigma, iso, eps);
     break;
   }
+}
+
+/* sigma_th = -E/(1-2nu) alpha dT = -0 with the default alpha = 0
+ * (material_thermal.hh:91-95), and the plastic trial stress's
+ * previous - current sigma_th is then 0 */
+void orc_law_stress(int law, double lambda, double mu, double sigma_y,
+                    double h, const double *g, const double *pg,
+                    const double *psigma, double piso, const double *peps,
+                    double *sigma, double *iso, double *eps) {
+  orc_law_stress_th(law, lambda, mu, sigma_y, h, g, pg, psigma, piso, peps, -0.,
+                    0., sigma, iso, eps);
+}
+
+/* MaterialThermal::computeStressOnQuad (material_thermal.hh:91-95) with the
+ * element's E and nu */
+static double thermal_stress(const orc_model *m, long e, long iq) {
+  double E = m->E_el ? m->E_el[e] : m->E, nu = m->nu_el ? m->nu_el[e] : m->nu;
+  return -E / (1. - 2. * nu) * m->alpha * m->delta_T[iq];
 }
 
 static int law_is_fd(int law) {
@@ -585,16 +604,28 @@ void orc_compute_all_stresses(orc_model *m) {
           double s = 0.;
           for (int n = 0; n < nnpe; ++n)
             s += ue[i + 3 * n] * Bq[j + 3 * n];
-          T(g, i, j) = s; /* minus eigen_grad_u = 0 */
+          T(g, i, j) = s;
         }
+      /* gradu_vect -= eigengradu (material.cc:147) */
+      if (m->eigen_grad_u)
+        for (int k = 0; k < 9; ++k)
+          g[k] -= m->eigen_grad_u[9 * iq + k];
+      /* MaterialThermal::computeStress, called first by the laws'
+       * computeStress (material_elastic.cc:85-86) */
+      double sth = -0., dsth = 0.;
+      if (m->delta_T) {
+        sth = thermal_stress(m, e, iq);
+        m->sigma_th[iq] = sth;
+        dsth = m->prev_sigma_th[iq] - sth;
+      }
       if (m->law == ORC_PLASTIC_LIH) {
-        orc_law_stress(m->law, lambda, mu, sy, h, g, m->prev_grad_u + 9 * iq,
-                       m->prev_stress + 9 * iq, m->prev_iso[iq],
-                       m->prev_eps_p + 9 * iq, m->stress + 9 * iq,
-                       m->iso + iq, m->eps_p + 9 * iq);
+        orc_law_stress_th(m->law, lambda, mu, sy, h, g, m->prev_grad_u + 9 * iq,
+                          m->prev_stress + 9 * iq, m->prev_iso[iq],
+                          m->prev_eps_p + 9 * iq, sth, dsth, m->stress + 9 * iq,
+                          m->iso + iq, m->eps_p + 9 * iq);
       } else {
-        orc_law_stress(m->law, lambda, mu, sy, h, g, NULL, NULL, 0., NULL,
-                       m->stress + 9 * iq, NULL, NULL);
+        orc_law_stress_th(m->law, lambda, mu, sy, h, g, NULL, NULL, 0., NULL, sth,
+                          dsth, m->stress + 9 * iq, NULL, NULL);
       }
     }
   }
@@ -807,6 +838,9 @@ void orc_solve_step(orc_model *m) {
     memcpy(m->prev_eps_p, m->eps_p, sizeof(double) * 9 * nq);
     memcpy(m->prev_iso, m->iso, sizeof(double) * nq);
   }
+  if (m->delta_T) /* sigma_th has a history (material_thermal.cc:34) */
+    memcpy(m->prev_sigma_th, m->sigma_th,
+           sizeof(double) * m->nel * orc_nqp(m->type));
   /* savePreviousDOFs: u_prev = u (time_step_solver_default.cc:175-177) */
   if (m->u_prev)
     memcpy(m->u_prev, m->u, sizeof(double) * ndof);

@@ -64,6 +64,18 @@ typedef struct orc_model {
 
   double dt;
   long step;
+
+  /* Material::eigengradu (material.cc:35-36,80-85): subtracted from grad u at
+   * every quadrature point before computeStress (material.cc:147),
+   * [nel*nqp][9]; NULL: zero */
+  const double *eigen_grad_u;
+  /* MaterialThermal (material_thermal.cc:30-44, .hh:91-95): alpha and the
+   * per-quadrature-point delta_T [nel*nqp]; NULL: sigma_th = -0 (alpha = 0
+   * by default).  sigma_th and its previous value (history,
+   * material_thermal.cc:34) [nel*nqp], written when delta_T is set. */
+  double alpha;
+  const double *delta_T;
+  double *sigma_th, *prev_sigma_th;
 } orc_model;
 
 /* Threads of the element and DOF loops (default 1); results are
@@ -136,6 +148,15 @@ void orc_law_stress(int law, double lambda, double mu, double sigma_y,
                     const double *prev_eps_p, double *sigma, double *iso,
                     double *eps_p);
 void orc_lame(double E, double nu, double *lambda, double *mu);
+/* The same with the thermal terms: sigma_th of the elastic laws, and
+ * delta_sigma_th = previous sigma_th - sigma_th of the plastic trial stress
+ * (material_linear_isotropic_hardening_inline_impl.hh:33-47). */
+void orc_law_stress_th(int law, double lambda, double mu, double sigma_y,
+                       double h, const double *grad_u, const double *prev_grad_u,
+                       const double *prev_sigma, double prev_iso,
+                       const double *prev_eps_p, double sigma_th,
+                       double delta_sigma_th, double *sigma, double *iso,
+                       double *eps_p);
 
 #ifdef __cplusplus
 }
