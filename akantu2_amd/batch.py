@@ -133,7 +133,9 @@ class Batch:
         return self.get(iid, "internal_force")
 
     # ---------------------------------------------------------- launch shape
-    def setLaunch(self, block=256, tile=2048, blocks_per_cu=4, graph=True):
+    def setLaunch(self, block=256, tile=2048, blocks_per_cu=0, graph=True):
+        """Threads per block, elements per tile and hipGraph on/off.  Every
+        workgroup does one tile; blocks_per_cu (0..32) has no effect."""
         N.check(N.lib().akx_batch_set_launch(self._h, int(block), int(tile),
                                              int(blocks_per_cu), int(bool(graph))))
 

@@ -8,10 +8,9 @@
 //            tile; one list per (element type, law) group.
 //   slices : 256-node slices of the live instances (an instance owns whole
 //            slices: its node range starts on a multiple of 256).
-// A persistent launch gives each block a contiguous chunk of the list;
-// blocks are dealt round-robin to the 8 XCDs, so the chunk of hardware block
-// b is logical chunk (b % 8) * (nblocks / 8) + b / 8 and neighbouring chunks
-// (neighbouring elements, shared nodes) stay in one XCD's L2.
+// Every launch gives each workgroup one tile (element passes) or one slice
+// (gathers): persistent workgroups walking chunks of the lists measured
+// slower, their dependent load latencies running back to back.
 #pragma once
 #include "akx_args.hpp"
 #include <hip/hip_runtime.h>
@@ -73,7 +72,6 @@ struct BatchElemArgs {
   int64_t hcap, hoff;
   const double *hex_tab; // Hex8: dN/dxi [8][3][8], N [8][8], w [8]
   int32_t parity;        // step index within the solve call, mod 2
-  int64_t nblocks;       // grid size, a multiple of 8
 };
 
 constexpr int HEX_TAB_N = 192, HEX_TAB_W = 256, HEX_TAB_SIZE = 264;
@@ -82,16 +80,6 @@ constexpr int HEX_TAB_N = 192, HEX_TAB_W = 256, HEX_TAB_SIZE = 264;
 // 4-wide entry (one vector store per corner) measured slower, its extra
 // bytes cost more than the saved cache accesses.
 constexpr int FE_W = 3;
-
-// [w0, w1): the part of a work list of nwork items owned by this block
-__device__ __forceinline__ void batch_chunk(int64_t nwork, int64_t nblocks, int64_t &w0,
-                                            int64_t &w1) {
-  const int64_t b = blockIdx.x, per = nblocks >> 3;
-  const int64_t lb = (b & 7) * per + (b >> 3);
-  const int64_t chunk = (nwork + nblocks - 1) / nblocks;
-  w0 = lb * chunk < nwork ? lb * chunk : nwork;
-  w1 = w0 + chunk < nwork ? w0 + chunk : nwork;
-}
 
 // the Gauss-point view of one step: read buffer `par`, write buffer par ^ 1
 template <class HT>

@@ -38,7 +38,6 @@ using akx::BTile;
 using akx::ElemArgs;
 using akx::QpCompact;
 using akx::QpViewT;
-using akx::batch_chunk;
 using akx::batch_compact_view;
 using akx::FE_W;
 using akx::qp_view;
@@ -681,9 +680,10 @@ __global__ void __launch_bounds__(256) k_tet4_bpa(ElemArgs A) {
 // ---------------------------------------------------------------------------
 template <int LAW>
 __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
-  int64_t t0, t1;
-  batch_chunk(A.ntiles, A.nblocks, t0, t1);
-  for (int64_t tt = t0; tt < t1; ++tt) {
+  {
+    const int64_t tt = blockIdx.x; // one tile per workgroup
+    if (tt >= A.ntiles)
+      return;
     const BTile T = A.tiles[tt];
     const BInst P = A.inst[T.inst];
     const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
@@ -699,7 +699,10 @@ __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
 }
 
 // Batched FP32 Tet4 with block-partial assembly: each tile is one block of
-// at most 256 consecutive elements of one instance.  The block's distinct
+// at most 256 consecutive elements of one instance, and each workgroup does
+// one tile (a persistent loop over tiles measured 25% slower: a workgroup's
+// staging, element and summing phases then ran one after another, their
+// load latencies exposed).  The block's distinct
 // node records are staged in LDS, the elements' corner forces go to LDS
 // slots (the corners of one distinct node take consecutive slots in
 // ascending element order), and one thread per distinct node adds its slots
@@ -710,9 +713,10 @@ __global__ void __launch_bounds__(256) k_batch_tet4_bpa(BatchElemArgs A) {
   extern __shared__ double2 s_xu[]; // [bmaxu][3] records, then [1024][3] floats
   float *s_f = reinterpret_cast<float *>(s_xu + 3 * A.bmaxu);
   float *fe = reinterpret_cast<float *>(A.fe);
-  int64_t t0, t1;
-  batch_chunk(A.ntiles, A.nblocks, t0, t1);
-  for (int64_t tt = t0; tt < t1; ++tt) {
+  {
+    const int64_t tt = blockIdx.x;
+    if (tt >= A.ntiles)
+      return;
     const BTile T = A.tiles[tt];
     const BBlk B = A.blks[tt];
     const BInst P = A.inst[T.inst];
@@ -754,16 +758,16 @@ __global__ void __launch_bounds__(256) k_batch_tet4_bpa(BatchElemArgs A) {
       }
       akx::put_entry<FE_W>(fe, (int64_t)A.bpos[B.off + k], float(fx), float(fy), float(fz));
     }
-    __syncthreads(); // the next tile overwrites the records and slots
   }
 }
 
 template <int LAW, int EPB>
 __global__ void __launch_bounds__(EPB * 8) k_batch_hex8_f32(BatchElemArgs A) {
   const int t = threadIdx.x, el = t >> 3, q = t & 7;
-  int64_t t0, t1;
-  batch_chunk(A.ntiles, A.nblocks, t0, t1);
-  for (int64_t tt = t0; tt < t1; ++tt) {
+  {
+    const int64_t tt = blockIdx.x; // one tile per workgroup
+    if (tt >= A.ntiles)
+      return;
     const BTile T = A.tiles[tt];
     const BInst P = A.inst[T.inst];
     const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
@@ -830,10 +834,11 @@ hipError_t launch_elements(int etype, int law, bool fp32, const ElemArgs &A,
 template <int LAW>
 static void launch_batch_law(int etype, int block, const BatchElemArgs &A,
                              hipStream_t s) {
-  dim3 grid((unsigned)A.nblocks);
+  dim3 grid((unsigned)A.ntiles);
   if (etype == 0) {
-    if (A.blks) // block-partial assembly: records, then 1024 corner slots
-      hipLaunchKernelGGL(k_batch_tet4_bpa<LAW>, grid, dim3(block),
+    if (A.blks) // block-partial assembly: one tile per workgroup; records,
+                // then 1024 corner slots
+      hipLaunchKernelGGL(k_batch_tet4_bpa<LAW>, dim3((unsigned)A.ntiles), dim3(block),
                          (size_t)A.bmaxu * 48 + 1024 * 12, s, A);
     else
       hipLaunchKernelGGL(k_batch_tet4_f32<LAW>, grid, dim3(block), 0, s, A);

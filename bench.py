@@ -348,7 +348,7 @@ def run_forms(args, plain, mesh, blocked, v0, dt, plain_value, plain_steps):
     forms["batch"].update({
         "workload": "512 Tet4 boxes of 5e3-4e5 elements (log-uniform, seed 4), aspect "
                     "U(1, 8), linear isotropic hardening with per-instance E, nu, sigma_y, "
-                    "h, FP32 form, persistent launches in a hipGraph, autotuned",
+                    "h, FP32 form, one tile per workgroup, in a hipGraph, autotuned",
         "instances": st["live"], "elements": st["elements"], "unit": "elem*qp/s",
         "steps": steps, "vs_plain_fp64": batch["fp32"]["value"] / plain_value,
         "fp64_same_workload": batch["fp64"],
@@ -626,14 +626,15 @@ def run_batch(args):
                                + ("linear isotropic hardening with per-instance E, nu, "
                                   "sigma_y, h" if law == PL else
                                   f"{law} with per-instance E, nu")
-                               + ", FP32 form, persistent launches in a hipGraph, autotuned",
+                               + ", FP32 form, one tile per workgroup, in a hipGraph, autotuned",
                    "instances": len(specs), "scale": args.scale,
                    "elements": int(st["elements"]),
                    "quadrature_points": int(st["quadrature_points"])},
         "roofline": {"bound": "hbm", "achieved": el["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": el["achieved_gbs"] / HBM_PEAK_GBS,
                      "traffic": None,
-                     "kernel": f"k_batch_{args.etype}_f32<{'plastic' if law == PL else law}>",
+                     "kernel": (f"k_batch_{'tet4_bpa' if args.etype == 'tet4' else 'hex8_f32'}"
+                                f"<{'plastic' if law == PL else law}>"),
                      "algorithmic_bytes_per_launch": el["bytes"],
                      "avg_kernel_ms": el["avg_ms"]},
         "forms": res,

@@ -371,7 +371,8 @@ int akx_batch_synchronize(akx_batch *b);
  * velocity, acceleration, internal force or nodes. */
 int akx_batch_get_field(akx_batch *b, int instance_id, int field, double *host);
 /* Launch shape: threads per block (64, 128, 256; Hex8 uses 8 lanes per
- * element), elements per tile, persistent blocks per CU, hipGraph on/off. */
+ * element), elements per tile, hipGraph on/off.  Every workgroup does one
+ * tile; blocks_per_cu (0..32) is kept for compatibility and has no effect. */
 int akx_batch_set_launch(akx_batch *b, int block, int tile, int blocks_per_cu,
                          int use_graph);
 int akx_batch_get_launch(akx_batch *b, int *block, int *tile,
