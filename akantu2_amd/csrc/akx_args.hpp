@@ -39,6 +39,17 @@ struct ElemArgs {
   const int32_t *bl_pos;
   const int32_t *pos;     // [(pos_offset + i) * nnpe + a] -> entry of the
                           // node-major elemental-force table
+  // FP64 Tet4 with partial-sum assembly (null otherwise), on the 256-element
+  // blocks of bl_nodes / lconn.  pcode [n][4]: >= 0 the table entry of the
+  // corner, < 0 the LDS slot -1 - pcode of a corner whose run (its node's
+  // corners in this block) is summed in the block.  The summed runs of block
+  // b are poff[b] .. poff[b + 1] - 1; run k sums slots pend[k - 1] (0 for
+  // the block's first) .. pend[k] - 1 into table entry ppos[k].  t_nblk:
+  // blocks of the launch; t_slots: LDS slots of the largest block.
+  const int32_t *pcode, *poff, *ppos;
+  const uint16_t *pend;
+  int64_t t_nblk;
+  int32_t t_slots;
   // Hex8, plain FP64: J^-1 and w det J of every Gauss point of the mesh,
   // [5][nel * 8] double2 {Ji00, Ji01} {Ji02, Ji10} {Ji11, Ji12} {Ji20, Ji21}
   // {Ji22, wJ}, or null
@@ -110,6 +121,19 @@ struct NodesSoA {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
       d[c] = u[(int64_t)c * stride + n];
+  }
+};
+
+// The four corners' {X, u} records, already read into registers (corner
+// a is node index a)
+struct NodesRegs4 {
+  double r[4][6];
+  __device__ __forceinline__ void load(int32_t n, double (&x)[3], double (&d)[3]) const {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x[c] = r[n][c];
+      d[c] = r[n][3 + c];
+    }
   }
 };
 

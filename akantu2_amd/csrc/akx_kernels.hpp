@@ -427,11 +427,12 @@ __device__ __forceinline__ void strict_law(const QpViewT<HT> &V, int64_t iq, CT 
 // over the Gauss points in ascending order.  Shared by the single-model
 // launch (one material) and the batched persistent launch.
 // ---------------------------------------------------------------------------
-template <int ET, int LAW, class CT, class FT, class HT, class NA, int W = 3, bool EXT = false>
+template <int ET, int LAW, class CT, class FT, class HT, class NA, int W = 3, bool EXT = false,
+          bool SLOTS = false>
 __device__ __forceinline__ void element_forces_one(
     const int32_t *conn_e, const NA &nodes_of,
     CT lambda, CT mu, CT sy, CT hh, const QpViewT<HT> &V, int64_t iq0,
-    const int32_t *pos_e, void *fe_out) {
+    const int32_t *pos_e, void *fe_out, double *slots = nullptr) {
   constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
   int32_t nodes[NN];
   {
@@ -509,9 +510,19 @@ __device__ __forceinline__ void element_forces_one(
     const int4 pv = p4[kk];
     const int32_t pn[4] = {pv.x, pv.y, pv.z, pv.w};
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4; ++j) {
+      if constexpr (SLOTS) { // a corner of its node's prefix: to its LDS slot
+        if (pn[j] < 0) {
+          double *o = slots + 3 * (-1 - pn[j]);
+          o[0] = double(re[4 * kk + j][0]);
+          o[1] = double(re[4 * kk + j][1]);
+          o[2] = double(re[4 * kk + j][2]);
+          continue;
+        }
+      }
       put_entry<W>(fe, (int64_t)pn[j], FT(re[4 * kk + j][0]), FT(re[4 * kk + j][1]),
                    FT(re[4 * kk + j][2]));
+    }
   }
 }
 
@@ -581,6 +592,80 @@ __global__ void __launch_bounds__(256) k_tet4_strict_lds(ElemArgs A) {
   element_forces_one<ET_TET4, LAW, CT, FT, double, NodesLDS, 3, EXT>(
       reinterpret_cast<const int32_t *>(&lc), NodesLDS{reinterpret_cast<const double *>(s_rec)},
       lambda, mu, sy, hh, qp_view(A, 1), i, reinterpret_cast<const int32_t *>(&pv), A.fe);
+}
+
+// Tet4 in the reference's operation order with partial-sum assembly.  The
+// gather sums node n's entries in order, f = (0 + -1 e_1) + -1 e_2 + ...
+// The corners of n in one 256-element block form a run of that order; a run
+// can be summed here, in LDS, from 0 and in the same order, into one entry P.
+// Rounding is symmetric, so for the node's first run 0 + -1 P is the
+// gather's own partial sum bit for bit (P = 0 included: both are +0):
+// summing each node's first run (prefix partials) keeps f_int's bits with
+// fewer entries.  Summing every run (block partials, opt-in) leaves one
+// entry per (block, node) and rounds otherwise.  The LDS first holds the
+// block's node records; once every lane has read its corners' records into
+// registers it holds the summed runs' slots.
+constexpr int PPA_TB = 256;
+
+template <int LAW, bool EXT = false>
+__global__ void __launch_bounds__(PPA_TB) k_tet4_ppa(ElemArgs A) {
+  extern __shared__ double2 s_rec[]; // [bl_maxu][3] records, later [t_slots][3] slots
+  const int64_t blk = xcd_block(A.nblocks_padded);
+  if (blk >= A.t_nblk) // padding block: no elements, no node list
+    return;
+  const int64_t i = blk * PPA_TB + threadIdx.x;
+  const bool active = i < A.n;
+  // the element's own streams first (lanes past the end read the last one)
+  const int64_t ic = active ? i : A.n - 1;
+  const ushort4 l = *reinterpret_cast<const ushort4 *>(A.lconn + ic * 4);
+  const int4 pv = *reinterpret_cast<const int4 *>(A.pcode + ic * 4);
+  double lambda = A.lambda, mu = A.mu, sy = A.sigma_y, hh = A.h;
+  if (A.lam_el) {
+    lambda = A.lam_el[ic];
+    mu = A.mu_el[ic];
+  }
+  if (A.sy_el)
+    sy = A.sy_el[ic];
+  if (A.h_el)
+    hh = A.h_el[ic];
+  stage_block_records(A, blk, s_rec);
+  __syncthreads();
+  NodesRegs4 rec;
+  {
+    const NodesLDS lds{reinterpret_cast<const double *>(s_rec)};
+    const uint16_t lcs[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      double x[3], d[3];
+      lds.load(lcs[a], x, d);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        rec.r[a][c] = x[c];
+        rec.r[a][3 + c] = d[c];
+      }
+    }
+  }
+  __syncthreads(); // the records are dead: the LDS now takes the slots
+  double *s_slot = reinterpret_cast<double *>(s_rec);
+  if (active) {
+    const int4 corners = make_int4(0, 1, 2, 3);
+    element_forces_one<ET_TET4, LAW, double, double, double, NodesRegs4, 3, EXT, true>(
+        reinterpret_cast<const int32_t *>(&corners), rec, lambda, mu, sy, hh, qp_view(A, 1),
+        i, reinterpret_cast<const int32_t *>(&pv), A.fe, s_slot);
+  }
+  __syncthreads();
+  const int32_t k0 = A.poff[blk], k1 = A.poff[blk + 1];
+  double *fe = reinterpret_cast<double *>(A.fe);
+  for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += blockDim.x) {
+    const int s0 = k == k0 ? 0 : A.pend[k - 1], s1 = A.pend[k];
+    double f0 = 0., f1 = 0., f2 = 0.;
+    for (int sl = s0; sl < s1; ++sl) {
+      f0 = f0 + s_slot[3 * sl];
+      f1 = f1 + s_slot[3 * sl + 1];
+      f2 = f2 + s_slot[3 * sl + 2];
+    }
+    put_entry<3>(fe, (int64_t)A.ppos[k], f0, f1, f2);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -517,6 +517,12 @@ struct Material {
   // offsets of every distinct node, table entry of every (block, node) partial
   DevBuf<uint16_t> lslot, bl_soff;
   DevBuf<int32_t> bl_pos;
+  // partial-sum assembly of the FP64 Tet4 pass: per-corner codes, per-block
+  // summed runs (slot ends, table entries)
+  DevBuf<int32_t> pcode, poff, ppos;
+  DevBuf<uint16_t> pend;
+  int64_t t_nblk = 0;
+  int32_t t_slots = 0;
   bool per_element = false;
   // materialised internals and plastic history [2][9][n*nqp] / [2][n*nqp]
   DevBuf<double> grad, stress; // materialised by compute_all_stresses
@@ -555,10 +561,18 @@ struct akx_model {
   // FP32 Tet4 block-partial assembly: the table holds one entry per (block,
   // distinct node of the block) instead of one per element corner
   bool bpa = false;
-  bool bpa_allowed = true; // akx_model_set_block_assembly
   DevBuf<int64_t> bp_off;
   DevBuf<uint8_t> bp_deg;
   int64_t bp_total = 0;
+  // FP64 Tet4 partial-sum assembly: prefix partials (same bits, fewer
+  // entries: each node's first-block sum, then one entry per later corner)
+  // or, opted into, block partials (one entry per (block, node))
+  bool ppa = false;
+  int assembly = AKX_ASSEMBLY_DEFAULT; // akx_model_set_block_assembly
+  DevBuf<int64_t> pp_off;
+  DevBuf<uint8_t> pp_deg;
+  int64_t pp_total = 0;
+  int64_t pp_entries = 0, pp_runs = 0; // table entries, summed runs
   DevBuf<unsigned char> staging;
   DevBuf<unsigned long long> scratch_u64;
   DevBuf<long long> scratch_i64;
@@ -714,12 +728,14 @@ void refresh_xu(akx_model *m, bool with_x) {
   HIPCHECK(hipGetLastError());
 }
 
-// FP32 Tet4 element pass: for each block of 256 consecutive elements of a
-// contiguous material, its distinct nodes in order of first use (at most
-// 1,024; every block's list padded with -1 to the longest) and each
-// element's corners as indices into them
-void build_block_nodes(akx_model *m, Material &mat) {
-  const int64_t n = mat.n, nb = (n + 255) / 256;
+// Tet4 element passes with staged node records: for each block of bs
+// consecutive positions of a contiguous material (position i holds element
+// order[i], or i without an order), its distinct nodes in order of first use
+// (at most 4 bs; every block's list padded with -1 to the longest) and each
+// position's corners as indices into them
+void build_block_nodes(akx_model *m, Material &mat, const std::vector<int32_t> *order = nullptr,
+                       int64_t bs = 256) {
+  const int64_t n = mat.n, nb = (n + bs - 1) / bs;
   std::vector<int32_t> off(nb + 1, 0), nodes;
   std::vector<uint16_t> lc(n * 4);
   std::vector<int64_t> stamp(m->nn, -1);
@@ -728,9 +744,10 @@ void build_block_nodes(akx_model *m, Material &mat) {
   int32_t maxu = 0;
   for (int64_t b = 0; b < nb; ++b) {
     const int64_t start = (int64_t)nodes.size();
-    for (int64_t i = b * 256; i < std::min<int64_t>(n, b * 256 + 256); ++i)
+    for (int64_t i = b * bs; i < std::min<int64_t>(n, b * bs + bs); ++i)
       for (int a = 0; a < 4; ++a) {
-        const int32_t node = m->conn_host[(mat.first + i) * 4 + a];
+        const int64_t e = order ? (int64_t)(*order)[i] : i;
+        const int32_t node = m->conn_host[(mat.first + e) * 4 + a];
         if (stamp[node] != b) {
           stamp[node] = b;
           local[node] = (int32_t)((int64_t)nodes.size() - start);
@@ -754,7 +771,7 @@ void build_block_nodes(akx_model *m, Material &mat) {
                      hipMemcpyHostToDevice));
   mat.bl_maxu = maxu;
   mat.bl_nodes_host = std::move(padded);
-  if (m->precision != AKX_PRECISION_FP32)
+  if (m->precision != AKX_PRECISION_FP32 || order)
     return;
   // block-partial assembly: the corners of distinct node k of a block take
   // consecutive LDS slots, in ascending element (then corner) order
@@ -827,6 +844,139 @@ void build_bpa_table(akx_model *m) {
   HIPCHECK(hipMemcpy(m->bp_deg.p, deg8.data(), deg8.size(), hipMemcpyHostToDevice));
 }
 
+// Prefix- or block-partial assembly of the FP64 Tet4 path (k_tet4_ppa) on
+// the 256-element blocks of the staged node records (build_block_nodes).
+// Node n's corners, in assembly order (materials in order, elements
+// ascending), come in runs: the corners in one block of one material.
+// full = false: the node's first run is summed in LDS in that order (the
+// gather then adds the same values in the same order: same bits) and every
+// later corner keeps an entry of its own.  full = true: every run is summed,
+// one entry per (block, node), added by the gather in block order (other
+// rounding).  The table keeps the node-major sliced-ELL layout.  Requires
+// every material contiguous and its block lists built.
+void build_ppa(akx_model *m, bool full) {
+  const int64_t TB = PPA_TB;
+  const size_t nm = m->mats.size();
+  std::vector<int32_t> nent(m->nn, 0), seen(m->nn, 0), base(m->nn, 0), cnt(m->nn, 0);
+  std::vector<int64_t> stamp(m->nn, -1), entry0(m->nn, 0);
+  std::vector<uint8_t> summed(m->nn, 0);
+  std::vector<int64_t> off;
+  std::vector<int32_t> nodes;
+  int32_t maxslots = 0;
+  // one traversal in assembly order; the second also writes the tables
+  auto traverse = [&](bool write, std::vector<std::vector<int32_t>> *code,
+                      std::vector<std::vector<int32_t>> *poff,
+                      std::vector<std::vector<uint16_t>> *pend,
+                      std::vector<std::vector<int32_t>> *ppos) {
+    std::fill(nent.begin(), nent.end(), 0);
+    std::fill(stamp.begin(), stamp.end(), -1);
+    int64_t gb = 0;
+    for (size_t k = 0; k < nm; ++k) {
+      const Material &mat = *m->mats[k];
+      const int64_t nb = (mat.n + TB - 1) / TB;
+      if (write) {
+        (*code)[k].assign(mat.n * 4, 0);
+        (*poff)[k].assign(nb + 1, 0);
+        (*pend)[k].clear();
+        (*ppos)[k].clear();
+      }
+      for (int64_t b = 0; b < nb; ++b, ++gb) {
+        const int64_t e0 = b * TB, e1 = std::min<int64_t>(mat.n, e0 + TB);
+        nodes.clear();
+        for (int64_t i = e0; i < e1; ++i)
+          for (int a = 0; a < 4; ++a) {
+            const int32_t node = m->conn_host[(mat.first + i) * 4 + a];
+            if (stamp[node] != gb) {
+              stamp[node] = gb;
+              cnt[node] = 0;
+              nodes.push_back(node);
+            }
+            cnt[node]++;
+          }
+        int32_t slots = 0;
+        for (int32_t node : nodes) {
+          summed[node] = full || nent[node] == 0;
+          seen[node] = 0;
+          entry0[node] = nent[node];
+          if (summed[node]) {
+            base[node] = slots;
+            slots += cnt[node];
+            if (write) {
+              (*pend)[k].push_back((uint16_t)slots);
+              (*ppos)[k].push_back(
+                  (int32_t)(off[node >> 8] + (int64_t)nent[node] * 256 + (node & 255)));
+            }
+            nent[node] += 1;
+          } else {
+            nent[node] += cnt[node];
+          }
+        }
+        maxslots = std::max(maxslots, slots);
+        if (!write)
+          continue;
+        (*poff)[k][b + 1] = (int32_t)(*pend)[k].size();
+        for (int64_t i = e0; i < e1; ++i)
+          for (int a = 0; a < 4; ++a) {
+            const int32_t node = m->conn_host[(mat.first + i) * 4 + a];
+            const int32_t j = seen[node]++;
+            (*code)[k][i * 4 + a] =
+                summed[node] ? -1 - (base[node] + j)
+                             : (int32_t)(off[node >> 8] + (entry0[node] + j) * 256 + (node & 255));
+          }
+      }
+    }
+  };
+  traverse(false, nullptr, nullptr, nullptr, nullptr);
+  const int64_t nslices = (m->nn + 255) / 256;
+  off.assign(nslices + 1, 0);
+  std::vector<uint8_t> deg8(m->nn, 0);
+  for (int64_t sl = 0; sl < nslices; ++sl) {
+    int w = 0;
+    for (int64_t nd = sl * 256; nd < std::min<int64_t>(m->nn, sl * 256 + 256); ++nd) {
+      require(nent[nd] <= 255, "a node has more than 255 table entries");
+      deg8[nd] = (uint8_t)nent[nd];
+      w = std::max<int>(w, nent[nd]);
+    }
+    off[sl + 1] = off[sl] + (int64_t)w * 256;
+  }
+  require(off[nslices] < (int64_t)INT32_MAX, "partial-sum table too large for int32");
+  std::vector<std::vector<int32_t>> code(nm), poff(nm), ppos(nm);
+  std::vector<std::vector<uint16_t>> pend(nm);
+  traverse(true, &code, &poff, &pend, &ppos);
+  for (size_t k = 0; k < nm; ++k) {
+    Material &mat = *m->mats[k];
+    mat.t_nblk = (mat.n + TB - 1) / TB;
+    mat.t_slots = maxslots;
+    const size_t np = pend[k].size();
+    mat.pcode.alloc(mat.n * 4);
+    mat.poff.alloc(poff[k].size());
+    mat.pend.alloc(std::max<size_t>(np, 1));
+    mat.ppos.alloc(std::max<size_t>(np, 1));
+    HIPCHECK(hipMemcpy(mat.pcode.p, code[k].data(), mat.n * 4 * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(mat.poff.p, poff[k].data(), poff[k].size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+    if (np > 0) {
+      HIPCHECK(hipMemcpy(mat.pend.p, pend[k].data(), np * sizeof(uint16_t),
+                         hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(mat.ppos.p, ppos[k].data(), np * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+    }
+  }
+  m->pp_total = off[nslices];
+  m->pp_entries = 0;
+  for (int64_t nd = 0; nd < m->nn; ++nd)
+    m->pp_entries += nent[nd];
+  m->pp_runs = 0;
+  for (size_t k = 0; k < nm; ++k)
+    m->pp_runs += (int64_t)pend[k].size();
+  m->pp_off.alloc(off.size());
+  m->pp_deg.alloc(std::max<int64_t>(m->nn, 1));
+  HIPCHECK(hipMemcpy(m->pp_off.p, off.data(), off.size() * sizeof(int64_t),
+                     hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(m->pp_deg.p, deg8.data(), deg8.size(), hipMemcpyHostToDevice));
+}
+
 // FP32 form: (float(lambda), float(mu)) pairs of a material with per-element
 // constants, the values its element pass rounds the FP64 pair to
 void fill_lame_f32(akx_model *m, Material &mat) {
@@ -870,6 +1020,14 @@ ElemArgs elem_args(akx_model *m, Material &mat, bool materialize) {
     A.lslot = mat.lslot.p;
     A.bl_soff = mat.bl_soff.p;
     A.bl_pos = mat.bl_pos.p;
+  }
+  if (m->ppa) {
+    A.pcode = mat.pcode.p;
+    A.poff = mat.poff.p;
+    A.pend = mat.pend.p;
+    A.ppos = mat.ppos.p;
+    A.t_nblk = mat.t_nblk;
+    A.t_slots = mat.t_slots;
   }
   A.sy_el = mat.sy_el.p;
   A.h_el = mat.h_el.p;
@@ -1041,6 +1199,36 @@ void launch_hex8_strict(akx_model *m, Material &mat, const ElemArgs &A) {
     launch_hex8_strict_x<false>(m, mat, A);
 }
 
+template <bool EXT>
+void launch_tet4_ppa_x(akx_model *m, Material &mat, const ElemArgs &A) {
+  dim3 grid((unsigned)A.nblocks_padded), block(PPA_TB);
+  const size_t lds = std::max((size_t)A.bl_maxu * 48, (size_t)A.t_slots * 24);
+  switch (mat.law) {
+  case AKX_LAW_ELASTIC:
+    hipLaunchKernelGGL((k_tet4_ppa<LAW_ELASTIC, EXT>), grid, block, lds, m->stream, A);
+    break;
+  case AKX_LAW_ELASTIC_FD:
+    hipLaunchKernelGGL((k_tet4_ppa<LAW_ELASTIC_FD, EXT>), grid, block, lds, m->stream, A);
+    break;
+  case AKX_LAW_NEOHOOKEAN:
+    hipLaunchKernelGGL((k_tet4_ppa<LAW_NEOHOOKEAN, EXT>), grid, block, lds, m->stream, A);
+    break;
+  case AKX_LAW_PLASTIC_LIH:
+    hipLaunchKernelGGL((k_tet4_ppa<LAW_PLASTIC, EXT>), grid, block, lds, m->stream, A);
+    break;
+  default: // user laws never take this path
+    throw Error(AKX_ERR_UNSUPPORTED, "prefix-partial pass for a user law");
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_tet4_ppa(akx_model *m, Material &mat, const ElemArgs &A) {
+  if (mat.ext())
+    launch_tet4_ppa_x<true>(m, mat, A);
+  else
+    launch_tet4_ppa_x<false>(m, mat, A);
+}
+
 void launch_element_pass(akx_model *m, bool materialize) {
   const bool strict = m->precision == AKX_PRECISION_FP64;
   for (auto &mp : m->mats) {
@@ -1069,8 +1257,16 @@ void launch_element_pass(akx_model *m, bool materialize) {
       int64_t nb = (mat.n + epb - 1) / epb;
       A.nblocks_padded = (nb + 7) / 8 * 8;
     }
+    if (strict && m->ppa) {
+      require(A.pcode && A.poff && A.pend && A.ppos && A.bl_nodes && A.lconn &&
+                  A.t_nblk == (mat.n + PPA_TB - 1) / PPA_TB,
+              "prefix-partial pass launched without its block tables");
+      A.nblocks_padded = (A.t_nblk + 7) / 8 * 8;
+    }
     m->timed(0, [&] {
-      if (strict) {
+      if (strict && m->ppa) {
+        launch_tet4_ppa(m, mat, A);
+      } else if (strict) {
         if (m->etype == AKX_TET4)
           launch_elem_law<ET_TET4, double, double>(m, mat, A);
         else
@@ -1140,13 +1336,22 @@ void flip_plastic_state(akx_model *m) {
       mp->cur = 1 - mp->cur;
 }
 
-GatherArgs gather_args(akx_model *m) {
+// per_corner: the per-corner table whatever the model's assembly (the
+// pipelined schedule's partial launches write it)
+GatherArgs gather_args(akx_model *m, bool per_corner = false) {
   GatherArgs G{};
   G.nn = m->nn;
   G.node0 = 0;
   G.node_end = m->nn;
-  G.ell_off = m->bpa ? m->bp_off.p : m->ell_off.p;
-  G.deg = m->bpa ? m->bp_deg.p : m->deg.p;
+  G.ell_off = m->ell_off.p;
+  G.deg = m->deg.p;
+  if (m->bpa && !per_corner) {
+    G.ell_off = m->bp_off.p;
+    G.deg = m->bp_deg.p;
+  } else if (m->ppa && !per_corner) {
+    G.ell_off = m->pp_off.p;
+    G.deg = m->pp_deg.p;
+  }
   G.fe = m->fe.p;
   G.fint = m->fint.p;
   G.fext = m->has_fext ? m->fext.p : nullptr;
@@ -1339,7 +1544,7 @@ void launch_element_part(akx_model *m, int p) {
 }
 
 void launch_gather_group(akx_model *m, int p, bool predict) {
-  GatherArgs G = gather_args(m);
+  GatherArgs G = gather_args(m, true);
   if (predict)
     G.fint = nullptr; // stored by the last step only (launch_gather)
   G.node0 = m->pipe_node[p];
@@ -1780,7 +1985,9 @@ int akx_model_set_block_assembly(akx_model *m, int enable) {
   return guard([&] {
     require(m != nullptr, "null model");
     require(!m->initialized, "the assembly must be chosen before initFull");
-    m->bpa_allowed = enable != 0;
+    require(enable >= AKX_ASSEMBLY_PER_CORNER && enable <= AKX_ASSEMBLY_BLOCK_PARTIALS,
+            "assembly must be AKX_ASSEMBLY_PER_CORNER, _DEFAULT or _BLOCK_PARTIALS");
+    m->assembly = enable;
   });
 }
 
@@ -1788,7 +1995,10 @@ int akx_model_get_block_assembly(akx_model *m, int *active) {
   return guard([&] {
     check_ready(m);
     require(active != nullptr, "null output");
-    *active = m->bpa ? 1 : 0;
+    *active = m->bpa ? AKX_ASSEMBLY_BLOCK_PARTIALS
+              : m->ppa ? (m->assembly == AKX_ASSEMBLY_BLOCK_PARTIALS ? AKX_ASSEMBLY_BLOCK_PARTIALS
+                                                                     : AKX_ASSEMBLY_PREFIX_PARTIALS)
+                       : AKX_ASSEMBLY_PER_CORNER;
   });
 }
 
@@ -1850,9 +2060,16 @@ int akx_model_init_full(akx_model *m) {
       for (auto &mp : m->mats) // Tet4 passes with staged node records
         if (mp->contiguous && mp->n > 0 && m->precision != AKX_PRECISION_FP64_FAST)
           build_block_nodes(m, *mp);
+      bool ppa = m->assembly != AKX_ASSEMBLY_PER_CORNER &&
+                 m->precision == AKX_PRECISION_FP64 && !m->has_user();
+      for (auto &mp : m->mats)
+        ppa = ppa && mp->contiguous && mp->n > 0 && mp->lconn.p != nullptr;
+      if (ppa)
+        build_ppa(m, m->assembly == AKX_ASSEMBLY_BLOCK_PARTIALS);
+      m->ppa = ppa;
     }
     build_gather_table(m);
-    m->bpa = m->bpa_allowed && m->etype == AKX_TET4 &&
+    m->bpa = m->assembly != AKX_ASSEMBLY_PER_CORNER && m->etype == AKX_TET4 &&
              m->precision == AKX_PRECISION_FP32 && m->nn > 0;
     for (auto &mp : m->mats)
       m->bpa = m->bpa && mp->contiguous && mp->n > 0 && mp->lslot.p != nullptr;
@@ -2997,6 +3214,11 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
       // slot offset (2 B) and table position (4 B)
       extra += -2.0 * m->nnpe * nel_all + 6.0 * partials;
       entries = partials;
+    } else if (m->ppa) {
+      // corner codes (4 B, as the positions); per summed run its slot end
+      // (2 B) and table position (4 B)
+      extra += 6.0 * (double)m->pp_runs;
+      entries = (double)m->pp_entries;
     }
     switch (kernel_class) {
     case 0: // element pass: connectivity + table entries, X and u once per

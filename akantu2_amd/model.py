@@ -663,16 +663,27 @@ class SolidMechanicsModel:
         N.check(N.lib().akx_model_set_pipeline(self._h, int(parts), C.byref(active)))
         return bool(active.value)
 
-    def setBlockAssembly(self, enable: bool):
-        """FP32 Tet4 block-partial assembly on or off (see
-        akx_model_set_block_assembly); before initFull."""
-        N.check(N.lib().akx_model_set_block_assembly(self._h, int(bool(enable))))
+    _ASSEMBLIES = {"per_corner": 0, "default": 1, "block_partials": 2}
 
-    def blockAssembly(self) -> bool:
-        """Whether the initialised model assembles by block partials."""
+    def setBlockAssembly(self, enable):
+        """Tet4 assembly (see akx_model_set_block_assembly), before initFull:
+        True / "default" (FP64: prefix partials, same bits; FP32: block
+        partials), False / "per_corner", or "block_partials" (also in FP64,
+        other rounding)."""
+        mode = self._ASSEMBLIES[enable] if isinstance(enable, str) else int(bool(enable))
+        N.check(N.lib().akx_model_set_block_assembly(self._h, mode))
+
+    def assembly(self) -> str:
+        """The assembly the initialised model uses: "per_corner",
+        "prefix_partials" or "block_partials"."""
         active = C.c_int(0)
         N.check(N.lib().akx_model_get_block_assembly(self._h, C.byref(active)))
-        return bool(active.value)
+        return ("per_corner", "prefix_partials", "block_partials")[active.value]
+
+    def blockAssembly(self) -> bool:
+        """Whether the initialised model sums element forces in blocks
+        (prefix or block partials)."""
+        return self.assembly() != "per_corner"
 
     # ----------------------------------------------------------- measurement
     def enableKernelTiming(self, enable=True):

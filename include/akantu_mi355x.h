@@ -283,18 +283,27 @@ int akx_model_get_energy_noalloc(akx_model *m, const char *energy_id, int materi
  * whether it applies.  parts = 0 or 1: the plain schedule (default). */
 int akx_model_set_pipeline(akx_model *m, int parts, int *active);
 
-/* Assembly of the FP32 Tet4 element pass, no counterpart in the reference
- * (call before akx_model_init_full).  enable != 0 (default): block-partial
- * assembly where it applies (AKX_PRECISION_FP32, Tet4, every material a
- * contiguous element range): each block of 256 elements sums the forces of
- * its elements on each of its distinct nodes in a fixed order, and the nodal
- * gather adds a node's block partials in ascending block order, so the
- * element-force table holds one entry per (block, node) instead of one per
- * element corner; results are bit-reproducible and within the FP32 form's
- * bound, but not bit-identical to the per-element order.  enable = 0: one
- * table entry per element corner, added in ascending element order (the
- * order of the FP64 path).  get_block_assembly reports whether the
- * initialised model uses it. */
+/* Assembly of the Tet4 element passes, no counterpart in the reference
+ * (call before akx_model_init_full).  It applies to Tet4 meshes whose
+ * materials are all contiguous element ranges (and, in FP64, built-in laws).
+ * A node's corners, in assembly order (materials in order, elements
+ * ascending), come in runs: its corners in one 256-element block.
+ * - AKX_ASSEMBLY_PER_CORNER: one table entry per element corner, added by
+ *   the nodal gather in assembly order.
+ * - AKX_ASSEMBLY_DEFAULT: in FP64, prefix partials: the block of a node's
+ *   first run sums it, in that order, into one entry and every later corner
+ *   keeps its own; the gather's sums keep their bits (same f_int as per
+ *   corner, fewer entries).  In FP32, block partials (below).
+ * - AKX_ASSEMBLY_BLOCK_PARTIALS: every run is summed in its block, one entry
+ *   per (block, node), added in block order: bit-reproducible, but rounded
+ *   otherwise than the per-corner order (FP64: opt-in; FP32: the default).
+ * get_block_assembly reports the assembly the initialised model uses:
+ * AKX_ASSEMBLY_PER_CORNER, AKX_ASSEMBLY_PREFIX_PARTIALS or
+ * AKX_ASSEMBLY_BLOCK_PARTIALS. */
+#define AKX_ASSEMBLY_PER_CORNER 0
+#define AKX_ASSEMBLY_DEFAULT 1
+#define AKX_ASSEMBLY_PREFIX_PARTIALS 1
+#define AKX_ASSEMBLY_BLOCK_PARTIALS 2
 int akx_model_set_block_assembly(akx_model *m, int enable);
 int akx_model_get_block_assembly(akx_model *m, int *active);
 
