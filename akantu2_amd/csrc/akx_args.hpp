@@ -155,19 +155,22 @@ __device__ __forceinline__ void put_entry(FT *fe, int64_t entry, FT a, FT b, FT 
   }
 }
 
-// Tet4 passes with staged records: copies the {X, u} records of block blk's
-// distinct nodes into LDS (three 16-byte loads and stores per node).  The
-// caller synchronises the block before reading them.
+// Tet4 passes with staged records: builds the {X, u} records of block blk's
+// distinct nodes in LDS from the [3][nn] arrays X and u (six 8-byte loads
+// per node; a block's list holds runs of consecutive node ids, so they
+// coalesce), three 16-byte LDS stores per node.  Reading X and u rather than
+// the global records lets the gathers skip the records' u, whose 24-byte
+// half-record stores cost the FP32 gather of 60 M Tet4 a quarter of its
+// time.  The caller synchronises the block before reading them.
 __device__ __forceinline__ void stage_block_records(const ElemArgs &A, int64_t blk,
                                                     double2 *s) {
   const int32_t *list = A.bl_nodes + blk * A.bl_maxu;
   for (int k = threadIdx.x; k < A.bl_maxu; k += blockDim.x) {
-    const int32_t node = list[k];
+    const int64_t node = list[k];
     if (node >= 0) {
-      const double2 *p = reinterpret_cast<const double2 *>(A.xu + 6 * (int64_t)node);
-      s[3 * k] = p[0];
-      s[3 * k + 1] = p[1];
-      s[3 * k + 2] = p[2];
+      s[3 * k] = make_double2(A.X[node], A.X[A.nn + node]);
+      s[3 * k + 1] = make_double2(A.X[2 * A.nn + node], A.u[node]);
+      s[3 * k + 2] = make_double2(A.u[A.nn + node], A.u[2 * A.nn + node]);
     }
   }
 }

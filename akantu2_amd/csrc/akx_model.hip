@@ -554,6 +554,12 @@ struct akx_model {
   DevBuf<int32_t> conn, pos;
   DevBuf<double2> jw;      // Hex8 plain FP64: J^-1, w det J [5][nel * 8]
   DevBuf<double> xu;       // Tet4: [nn][6] {X, u} node records of the element pass
+  // the records' u is kept current (by the predictor and the gathers) only
+  // while a pass that reads the records runs: the per-element Tet4 passes
+  // (filtered materials, user laws, the FP64 fast form, the pipelined
+  // schedule's partial launches); the staged passes read X and u
+  bool xu_static = false;
+  bool records_live() const { return xu_static || (pipe_parts > 1 && pipe_ok); }
   DevBuf<int64_t> ell_off; // [nn/256 + 1]
   DevBuf<uint8_t> deg;     // [nn]
   int64_t ell_total = 0;   // entries of the node-major table
@@ -1358,7 +1364,7 @@ GatherArgs gather_args(akx_model *m, bool per_corner = false) {
   G.mass = m->mass.p;
   G.blocked = m->blocked.p;
   G.u = m->u.p;
-  G.xu = m->xu.p;
+  G.xu = m->records_live() ? m->xu.p : nullptr;
   G.v = m->v.p;
   G.a = m->a.p;
   G.cu = m->cu.p;
@@ -1504,6 +1510,8 @@ void plan_pipeline(akx_model *m) {
   make(m->ev_gather[0]);
   make(m->ev_gather[1]);
   m->pipe_ok = true;
+  if (!m->xu_static) // the partial launches read the records: bring u in
+    refresh_xu(m, false);
 }
 
 void launch_element_part(akx_model *m, int p) {
@@ -2060,6 +2068,9 @@ int akx_model_init_full(akx_model *m) {
       for (auto &mp : m->mats) // Tet4 passes with staged node records
         if (mp->contiguous && mp->n > 0 && m->precision != AKX_PRECISION_FP64_FAST)
           build_block_nodes(m, *mp);
+      m->xu_static = m->precision == AKX_PRECISION_FP64_FAST;
+      for (auto &mp : m->mats)
+        m->xu_static = m->xu_static || (mp->n > 0 && (!mp->contiguous || mp->user()));
       bool ppa = m->assembly != AKX_ASSEMBLY_PER_CORNER &&
                  m->precision == AKX_PRECISION_FP64 && !m->has_user();
       for (auto &mp : m->mats)
@@ -3234,7 +3245,7 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
     case 1: { // gather + update of a step followed by another: degree, r_e
               // read, mass, blocked, a/v read+write, u read+write (fused
               // predictor); the last step of a call also writes f_int
-      double per_node = 4.0 + s_node /*M*/ + 1.0 + (m->xu.p ? 3 * s_node : 0.) /*u records*/ +
+      double per_node = 4.0 + s_node /*M*/ + 1.0 + (m->records_live() ? 3 * s_node : 0.) /*u records*/ +
                         2 * 3 * s_node /*a*/ + 2 * 3 * s_node /*v*/ +
                         2 * 3 * s_node /*u*/ + (m->has_fext ? 3 * s_node : 0.);
       if (m->precision == AKX_PRECISION_FP32)
