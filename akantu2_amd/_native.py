@@ -65,6 +65,7 @@ def _declare(lib):
         "akx_model_set_eigen_grad_u": (C.c_int, [_vp, C.c_int, _vp]),
         "akx_model_set_thermal": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
         "akx_model_set_stress_callback": (C.c_int, [_vp, COMPUTE_STRESS_FN, _vp]),
+        "akx_model_set_device_stress": (C.c_int, [_vp, C.c_int, C.c_char_p, _vp, C.c_int]),
         "akx_fe_gradient_on_integration_points": (C.c_int, [_vp, _vp, C.c_int, _vp, _i64,
                                                             _vp]),
         "akx_fe_compute_btd": (C.c_int, [_vp, _vp, C.c_int, _vp, _i64, _vp]),

@@ -45,7 +45,7 @@ def build_library(verbose: bool = False) -> str:
         logs = list(ex.map(lambda j: _run(j[1]), jobs))
     tmp = LIB + ".tmp"
     logs.append(_run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp,
-                      *[j[0] for j in jobs]]))
+                      *[j[0] for j in jobs], "-lhiprtc"]))
     if verbose:
         print("".join(logs))
     os.replace(tmp, LIB)

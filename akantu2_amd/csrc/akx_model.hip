@@ -12,6 +12,7 @@
 #include "akx_kernels.hpp"
 
 #include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
 
 #include <algorithm>
 #include <cmath>
@@ -496,6 +497,20 @@ using namespace akx;
 // ---------------------------------------------------------------------------
 // Model
 // ---------------------------------------------------------------------------
+// A user law compiled for the device at run time (akx_model_set_device_stress)
+struct DeviceLaw {
+  hipModule_t module = nullptr;
+  hipFunction_t kernel = nullptr;
+  DevBuf<double> params;
+  DeviceLaw() = default;
+  DeviceLaw(const DeviceLaw &) = delete;
+  DeviceLaw &operator=(const DeviceLaw &) = delete;
+  ~DeviceLaw() {
+    if (module)
+      (void)hipModuleUnload(module);
+  }
+};
+
 struct Material {
   int law = 0;
   double rho = 0, E = 0, nu = 0, sigma_y = 0, h = 0;
@@ -536,6 +551,8 @@ struct Material {
   bool ext() const { return eig.p != nullptr || sth_now.p != nullptr; }
   // a law computed by the caller (the computeStress plugin)
   bool user() const { return law == AKX_LAW_USER || law == AKX_LAW_USER_FD; }
+  // the caller's law compiled for the device, or null (host callback)
+  std::unique_ptr<DeviceLaw> dlaw;
 };
 
 struct akx_model {
@@ -1293,12 +1310,13 @@ void launch_element_pass(akx_model *m, bool materialize) {
 void user_stress_phase(akx_model *m) {
   if (!m->has_user())
     return;
-  require(m->stress_fn != nullptr,
-          "a user material needs a computeStress callback (akx_model_set_stress_callback)");
   for (int k = 0; k < (int)m->mats.size(); ++k) {
     Material &mat = *m->mats[k];
     if (!mat.user() || mat.n == 0)
       continue;
+    require(mat.dlaw || m->stress_fn != nullptr,
+            "a user material needs a computeStress callback (akx_model_set_stress_callback) "
+            "or a device law (akx_model_set_device_stress)");
     const int64_t nq = mat.n * m->nqp;
     ElemArgs A = elem_args(m, mat, false);
     A.grad_u = mat.grad.p;
@@ -1324,6 +1342,15 @@ void user_stress_phase(akx_model *m) {
                            dim3((unsigned)A.nblocks_padded), dim3(TPB), 0, m->stream, A);
     }
     HIPCHECK(hipGetLastError());
+    if (mat.dlaw) { // the law on the device: no host round trip
+      const double *gp = mat.grad.p, *pp = mat.dlaw->params.p;
+      double *sp = mat.stress.p;
+      long long nql = nq;
+      void *args[] = {&gp, &sp, &nql, &pp};
+      HIPCHECK(hipModuleLaunchKernel(mat.dlaw->kernel, (unsigned)grid_for(nq), 1, 1, TPB, 1, 1,
+                                     0, m->stream, args, nullptr));
+      continue;
+    }
     m->user_grad.resize(9 * nq);
     m->user_stress.assign(9 * nq, 0.);
     download_soa(m, mat.grad.p, m->user_grad.data(), nq, 9);
@@ -1334,6 +1361,72 @@ void user_stress_phase(akx_model *m) {
                   "computeStress of material " + std::to_string(k) + " failed");
     upload_soa(m, m->user_stress.data(), mat.stress.p, nq, 9);
   }
+}
+
+// The kernel around a device law: one thread per Gauss point; grad u and the
+// stress are [9][nq] on the device (column-major 3x3 per point)
+const char *k_user_stress_source = R"(
+extern "C" __global__ void __launch_bounds__(256)
+akx_user_stress_kernel(const double *grad, double *stress, long long nq,
+                       const double *params) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq)
+    return;
+  double g[9], s[9];
+  for (int c = 0; c < 9; ++c) {
+    g[c] = grad[c * nq + q];
+    s[c] = 0.;
+  }
+  akx_compute_stress(g, s, params);
+  for (int c = 0; c < 9; ++c)
+    stress[c * nq + q] = s[c];
+}
+)";
+
+// Compiles the caller's law with hiprtc for the model's device, with the
+// plain path's floating-point rules (no contraction: every multiply and add
+// rounds on its own), and loads it.
+std::unique_ptr<DeviceLaw> compile_device_law(akx_model *m, const char *source,
+                                              const double *params, int nparams) {
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, m->device));
+  std::string arch(prop.gcnArchName);
+  arch = "--offload-arch=" + arch.substr(0, arch.find(':'));
+  const std::string src = std::string(source) + "\n" + k_user_stress_source;
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "akx_device_law.hip", 0, nullptr, nullptr) !=
+      HIPRTC_SUCCESS)
+    throw Error(AKX_ERR_HIP, "hiprtcCreateProgram failed");
+  const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult rc = hiprtcCompileProgram(prog, 8, opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    std::string log;
+    if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 0) {
+      log.resize(n);
+      (void)hiprtcGetProgramLog(prog, &log[0]);
+    }
+    (void)hiprtcDestroyProgram(&prog);
+    throw Error(AKX_ERR_INVALID, std::string("the device law did not compile (") +
+                                     hiprtcGetErrorString(rc) + "):\n" + log);
+  }
+  size_t size = 0;
+  std::vector<char> code;
+  if (hiprtcGetCodeSize(prog, &size) == HIPRTC_SUCCESS && size > 0) {
+    code.resize(size);
+    if (hiprtcGetCode(prog, code.data()) != HIPRTC_SUCCESS)
+      code.clear();
+  }
+  (void)hiprtcDestroyProgram(&prog);
+  require(!code.empty(), "the device law compiled to no code");
+  auto law = std::make_unique<DeviceLaw>();
+  HIPCHECK(hipModuleLoadData(&law->module, code.data()));
+  HIPCHECK(hipModuleGetFunction(&law->kernel, law->module, "akx_user_stress_kernel"));
+  law->params.alloc(std::max(nparams, 1));
+  if (nparams > 0)
+    HIPCHECK(hipMemcpy(law->params.p, params, nparams * sizeof(double), hipMemcpyHostToDevice));
+  return law;
 }
 
 void flip_plastic_state(akx_model *m) {
@@ -2487,6 +2580,24 @@ int akx_model_compute_all_stresses(akx_model *m) {
     launch_element_pass(m, true);
     flip_plastic_state(m);
     HIPCHECK(hipStreamSynchronize(m->stream));
+  });
+}
+
+int akx_model_set_device_stress(akx_model *m, int material_id, const char *source,
+                                const double *params, int nparams) {
+  return guard([&] {
+    require(m != nullptr, "null model");
+    require(material_id >= 0 && material_id < (int)m->mats.size(), "no such material");
+    Material &mat = *m->mats[material_id];
+    require(mat.user(), "a device law replaces the computeStress of a user material");
+    require(nparams >= 0 && nparams <= 4096 && (nparams == 0 || params != nullptr),
+            "device law parameters: 0 to 4096 values");
+    m->set_device();
+    std::unique_ptr<DeviceLaw> law;
+    if (source != nullptr)
+      law = compile_device_law(m, source, params, nparams);
+    HIPCHECK(hipStreamSynchronize(m->stream)); // the law it replaces may be running
+    mat.dlaw = std::move(law);
   });
 }
 

@@ -216,6 +216,28 @@ class Material:
     def initMaterial(self):
         """Called once at initFull (Material::initMaterial)."""
 
+    # A law run on the device (akx_model_set_device_stress; the reference's
+    # plugin runs on the host).  A subclass that sets device_stress_source --
+    # HIP C++ defining
+    #   __device__ void akx_compute_stress(const double *grad_u,
+    #                                      double *stress, const double *params)
+    # for one Gauss point (3x3 column-major, as getGradU / getStress) -- has
+    # it compiled at initFull, after initMaterial, with the values
+    # deviceStressParams() returns; its computeStress is then not called.
+    device_stress_source = None
+
+    def deviceStressParams(self):
+        return ()
+
+    def setDeviceStress(self, source, params=()):
+        """Compile and use a device law for this material (None: back to
+        computeStress)."""
+        p = np.ascontiguousarray(params, np.float64).reshape(-1)
+        N.check(N.lib().akx_model_set_device_stress(
+            self._model._h, self.id, None if source is None else source.encode(),
+            _p(p) if p.size else None, int(p.size)))
+        self._model._stresses_fresh = False
+
     def computeStress(self, el_type, ghost_type=_not_ghost):
         raise NotImplementedError("computeStress of a user material")
 
@@ -491,6 +513,8 @@ class SolidMechanicsModel:
         for mat in self.materials:
             if mat.law in (N.LAW_USER, N.LAW_USER_FD):
                 mat.initMaterial()
+                if mat.device_stress_source is not None:
+                    mat.setDeviceStress(mat.device_stress_source, mat.deviceStressParams())
         self._initialized = True
 
     def getStableTimeStep(self) -> float:

@@ -170,6 +170,24 @@ typedef int (*akx_compute_stress_fn)(void *user, int material_id, int64_t nb_qp,
                                      const double *grad_u, double *stress);
 int akx_model_set_stress_callback(akx_model *m, akx_compute_stress_fn fn, void *user);
 
+/* computeStress of one user material run on the device: no counterpart in
+ * the reference, whose plugin runs on the host (the callback above, which
+ * costs a device-host round trip of the material's grad u and stress per
+ * evaluation).  source is HIP C++ defining
+ *   __device__ void akx_compute_stress(const double *grad_u, double *stress,
+ *                                      const double *params);
+ * for one Gauss point: grad_u and stress are 3x3, column-major as for the
+ * callback (stress is zero on entry), params the nparams values given here
+ * (copied).  The source is compiled for the model's GPU at this call
+ * (hiprtc, -O3, no floating-point contraction: every multiply and add
+ * rounds on its own, as on the plain path); a compile error returns
+ * AKX_ERR_INVALID with the compiler's log in akx_last_error.  From then on
+ * the material's stresses are computed on the device, the callback is not
+ * called for it.  Calling again replaces the law; source NULL returns the
+ * material to the callback. */
+int akx_model_set_device_stress(akx_model *m, int material_id, const char *source,
+                                const double *params, int nparams);
+
 /* FEEngine / DOFManager calls on caller fields, run on the device in the
  * reference's operation order (the one of the element passes).  filter
  * lists elements in order (NULL: all, ascending).  Per-point arrays are

@@ -171,6 +171,12 @@ public:
   // Gauss point): sigma_th = -E / (1 - 2 nu) alpha delta_T
   void setThermal(double alpha, double delta_T);
   void setThermal(double alpha, const Array<double> &delta_T);
+  // a user material's computeStress run on the device (no counterpart in
+  // the reference): HIP C++ defining
+  //   __device__ void akx_compute_stress(const double *grad_u,
+  //                                      double *stress, const double *params)
+  // compiled at this call; the ConstitutiveLaw is then not called
+  void setDeviceStress(const std::string &source, const std::vector<double> &params = {});
 
 private:
   SolidMechanicsModel *model;
@@ -449,6 +455,13 @@ inline void Material::setThermal(double alpha, const Array<double> &delta_T) {
   if (delta_T.size() != n * model->nqp)
     throw Exception(AKX_ERR_INVALID, "delta_T needs one value per Gauss point");
   check(akx_model_set_thermal(model->h, id, alpha, delta_T.data()));
+}
+
+inline void Material::setDeviceStress(const std::string &source,
+                                      const std::vector<double> &params) {
+  check(akx_model_set_device_stress(model->h, id, source.c_str(),
+                                    params.empty() ? nullptr : params.data(),
+                                    (int)params.size()));
 }
 
 } // namespace akantu_mi355x
