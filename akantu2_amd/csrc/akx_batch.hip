@@ -336,33 +336,44 @@ template <int ET> __global__ void k_batch_mass(BatchInitArgs A) {
 }
 
 // Lumped-mass contributions of one instance under block-partial assembly
-// (Tet4, FP32 batches): block b of 256 elements puts each corner's m_e(a)
-// into an LDS slot (the corners of one distinct node take consecutive slots
-// in ascending element order), then one thread per distinct node adds its
-// slots in that order and writes the block's partial to the node's table
-// entry; k_batch_gather_mass adds the partials in ascending block order.
+// (FP32 batches): block b (BPA_TET4 / BPA_HEX8 elements) puts each corner's m_e(a) (the
+// sums of k_batch_mass) into an LDS slot (the corners of one distinct node
+// take consecutive slots in ascending element order), then one thread per
+// distinct node adds its slots in that order and writes the block's partial
+// to the node's table entry; k_batch_gather_mass adds the partials in
+// ascending block order.
+template <int ET>
 __global__ void __launch_bounds__(256) k_batch_mass_bpa(BatchInitArgs A, const BBlk *blks,
                                                         const uint16_t *ls,
                                                         const uint16_t *bsend,
                                                         const int32_t *bpos) {
-  __shared__ double s_m[1024];
+  constexpr int NN = Elem<ET>::nnpe, NQ = Elem<ET>::nqp;
+  constexpr int BS = ET == ET_TET4 ? BPA_TET4 : BPA_HEX8; // elements per block
+  __shared__ double s_m[BS * NN];
   const BBlk B = blks[blockIdx.x];
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < A.n) {
+  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if ((int)threadIdx.x < BS && i < A.n) {
     const int64_t e = A.first + i;
-    double X[4][3];
-    batch_load_X<ET_TET4>(A, e, X);
-    double detJ;
-    const double wJ = batch_wj<ET_TET4>(A, X, 0, detJ);
-    if (detJ < 0.)
-      atomicMin(A.first_bad, (long long)i);
-    // the single Gauss point: m[n] = 0 + (rho N_n) wJ with N_n = 1/4
-    const double mn = 0. + (A.rho * .25) * wJ;
-    const ushort4 sl = *reinterpret_cast<const ushort4 *>(ls + e * 4);
-    s_m[sl.x] = mn;
-    s_m[sl.y] = mn;
-    s_m[sl.z] = mn;
-    s_m[sl.w] = mn;
+    double X[NN][3];
+    batch_load_X<ET>(A, e, X);
+    double m[NN];
+#pragma unroll
+    for (int n = 0; n < NN; ++n)
+      m[n] = 0.;
+    for (int q = 0; q < NQ; ++q) {
+      double detJ;
+      const double wJ = batch_wj<ET>(A, X, q, detJ);
+      if (detJ < 0.)
+        atomicMin(A.first_bad, (long long)(i * NQ + q));
+#pragma unroll
+      for (int n = 0; n < NN; ++n) {
+        const double N = (ET == ET_TET4) ? .25 : A.hex_tab[HEX_TAB_N + q * 8 + n];
+        m[n] += (A.rho * N) * wJ;
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < NN; ++n)
+      s_m[ls[e * NN + n]] = m[n];
   }
   __syncthreads();
   for (int k = threadIdx.x; k < B.count; k += blockDim.x) {
@@ -507,7 +518,7 @@ struct HostInst {
   int64_t ell0 = 0, nell = 0;            // force-table entries
   int64_t steps = 0;                     // steps done
   // block-partial assembly (Tet4, FP32): the instance's block-entry range
-  // and the distinct-node count of each of its 256-element blocks
+  // and the distinct-node count of each of its blocks
   int64_t bent0 = 0, nbent = 0;
   std::vector<int32_t> blk_cnt;
 };
@@ -542,12 +553,12 @@ struct akx_batch {
   // range and its table entry
   bool bpa = false;
   int64_t bcap = 0;
-  DevBuf<uint16_t> lc_t, ls_t;
+  DevBuf<uint16_t> lc_t, ls_t, ls_h; // local corners (Tet4), slots (Tet4, Hex8)
   DevBuf<int32_t> bnode, bpos;
   DevBuf<uint16_t> bsend;
   FreeList fl_bent;
   DevBuf<BBlk> d_blks; // parallel to d_tiles
-  int32_t bmaxu_live = 0;
+  int32_t bmaxu_live = 0; // most distinct nodes of a live Tet4 block (staged)
   std::vector<HostInst> inst;
   // device tables
   DevBuf<BInst> d_inst; // MAX_INSTANCES entries, address fixed
@@ -630,13 +641,15 @@ void rebuild_lists(akx_batch *b) {
     auto &G = groups[I.etype * 4 + I.law];
     auto &GB = gblks[I.etype * 4 + I.law];
     if (!I.blk_cnt.empty()) { // block-partial assembly: one tile per block
+      const int64_t bs = I.etype == AKX_TET4 ? BPA_TET4 : BPA_HEX8;
       int64_t off = I.bent0;
       for (size_t k = 0; k < I.blk_cnt.size(); ++k) {
-        const int64_t f = (int64_t)k * 256;
-        G.push_back({I.elem0 + f, (int32_t)std::min<int64_t>(256, I.nel - f), (int32_t)id});
+        const int64_t f = (int64_t)k * bs;
+        G.push_back({I.elem0 + f, (int32_t)std::min<int64_t>(bs, I.nel - f), (int32_t)id});
         GB.push_back({off, I.blk_cnt[k], 0});
         off += I.blk_cnt[k];
-        b->bmaxu_live = std::max(b->bmaxu_live, I.blk_cnt[k]);
+        if (I.etype == AKX_TET4)
+          b->bmaxu_live = std::max(b->bmaxu_live, I.blk_cnt[k]);
       }
     } else {
       for (int64_t f = 0; f < I.nel; f += b->tile) {
@@ -700,6 +713,13 @@ BatchElemArgs elem_args(akx_batch *b, int etype, int64_t t0, int64_t t1, int par
     A.bmaxu = b->bmaxu_live;
     require(A.blks && A.bnode && A.bsend && A.bpos && A.lc && A.ls && A.bmaxu > 0 &&
                 A.bmaxu <= 1024,
+            "block-partial pass launched without its block tables");
+  } else if (b->bpa) { // Hex8: corner slots only (no staged records)
+    A.blks = b->d_blks.p + t0;
+    A.bsend = b->bsend.p;
+    A.bpos = b->bpos.p;
+    A.ls = b->ls_h.p;
+    require(A.blks && A.bsend && A.bpos && A.ls,
             "block-partial pass launched without its block tables");
   }
   return A;
@@ -897,6 +917,8 @@ int64_t footprint_bytes(int precision, int etype, int law, int64_t nn, int64_t n
   if (f32 && etype == AKX_TET4) // block partials: corners and slots, and at most
                                 // two distinct-node entries per element
     bytes += (double)nel * (4 * 2 * 2 + 2 * (4 + 4 + 2));
+  else if (f32) // Hex8: slots, at most five distinct-node entries per element
+    bytes += (double)nel * (8 * 2 + 5 * (4 + 4 + 2));
   if (law == AKX_LAW_PLASTIC_LIH) // two buffers of the history
     bytes += (double)nel * nqp * 2 * (f32 ? 7 * 4 : 28 * 8);
   return (int64_t)bytes;
@@ -973,14 +995,19 @@ int akx_batch_create(int device, int precision, int64_t node_capacity,
       b->conn_t.alloc(4 * b->tcap);
       b->pos_t.alloc(4 * b->tcap);
     }
-    // FP32 Tet4 instances assemble by block partials; a block of 256 box
-    // elements has about 0.7 distinct nodes per element, so twice the
-    // element capacity holds the distinct-node entries of any box mesh
-    b->bpa = b->fp32() && b->tcap > 0;
+    // FP32 instances assemble by block partials; a block of BPA_TET4 / BPA_HEX8 box
+    // elements has about 0.7 (Tet4) and at most 4.03 (Hex8: part of one row
+    // of cells) distinct nodes per element, so twice the Tet4 and five times
+    // the Hex8 capacity hold the distinct-node entries of any box mesh
+    b->bpa = b->fp32() && (b->tcap > 0 || b->hexcap > 0);
     if (b->bpa) {
-      b->bcap = 2 * b->tcap + 1024;
-      b->lc_t.alloc(4 * b->tcap);
-      b->ls_t.alloc(4 * b->tcap);
+      b->bcap = 2 * b->tcap + 5 * b->hexcap + 2048;
+      if (b->tcap) {
+        b->lc_t.alloc(4 * b->tcap);
+        b->ls_t.alloc(4 * b->tcap);
+      }
+      if (b->hexcap)
+        b->ls_h.alloc(8 * b->hexcap);
       b->bnode.alloc(b->bcap);
       b->bpos.alloc(b->bcap);
       b->bsend.alloc(b->bcap);
@@ -1065,27 +1092,28 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
     I.nel = nb_elements;
     for (int64_t i = 0; i < nb_elements * nnpe; ++i)
       require(conn[i] >= 0 && conn[i] < nb_nodes, "connectivity refers to a missing node");
-    // block-partial assembly (FP32 Tet4): per 256-element block its distinct
+    // block-partial assembly (FP32): per block of bs elements its distinct
     // nodes in order of first use, each corner's index among them and its
     // LDS slot (the corners of one distinct node take consecutive slots in
     // ascending element order); the table then holds one entry per (block,
     // distinct node) and a node's degree is the number of blocks touching it
-    const bool bpa = b->bpa && element_type == AKX_TET4;
-    const int64_t nblk = bpa ? (nb_elements + 255) / 256 : 0;
+    const bool bpa = b->bpa;
+    const int64_t bs = element_type == AKX_TET4 ? BPA_TET4 : BPA_HEX8; // per block
+    const int64_t nblk = bpa ? (nb_elements + bs - 1) / bs : 0;
     std::vector<int32_t> bl_nodes; // instance-local node ids, blocks in order
     std::vector<uint16_t> lc, ls, send;
     if (bpa) {
-      lc.resize(nb_elements * 4);
-      ls.resize(nb_elements * 4);
+      lc.resize(nb_elements * nnpe);
+      ls.resize(nb_elements * nnpe);
       bl_nodes.reserve(nb_elements);
       I.blk_cnt.resize(nblk);
       std::vector<int64_t> stamp(nb_nodes, -1);
       std::vector<int32_t> local(nb_nodes, 0);
-      std::vector<uint16_t> cnt(1025);
+      std::vector<uint16_t> cnt(bs * nnpe + 1);
       for (int64_t bk = 0; bk < nblk; ++bk) {
-        const int64_t e0 = bk * 256, e1 = std::min<int64_t>(nb_elements, e0 + 256);
+        const int64_t e0 = bk * bs, e1 = std::min<int64_t>(nb_elements, e0 + bs);
         const int64_t start = (int64_t)bl_nodes.size();
-        for (int64_t i = e0 * 4; i < e1 * 4; ++i) {
+        for (int64_t i = e0 * nnpe; i < e1 * nnpe; ++i) {
           const int32_t nd = conn[i];
           if (stamp[nd] != bk) {
             stamp[nd] = bk;
@@ -1097,13 +1125,13 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
         const int32_t nloc = (int32_t)((int64_t)bl_nodes.size() - start);
         I.blk_cnt[bk] = nloc;
         std::fill(cnt.begin(), cnt.begin() + nloc + 1, 0);
-        for (int64_t i = e0 * 4; i < e1 * 4; ++i)
+        for (int64_t i = e0 * nnpe; i < e1 * nnpe; ++i)
           cnt[lc[i] + 1]++;
         for (int32_t k = 0; k < nloc; ++k) { // slot ends (inclusive prefix)
           cnt[k + 1] = (uint16_t)(cnt[k + 1] + cnt[k]);
           send.push_back(cnt[k + 1]);
         }
-        for (int64_t i = e0 * 4; i < e1 * 4; ++i) // cnt[k] now = start of k's slots
+        for (int64_t i = e0 * nnpe; i < e1 * nnpe; ++i) // cnt[k]: start of k's slots
           ls[i] = cnt[lc[i]]++;
       }
       I.nbent = (int64_t)bl_nodes.size();
@@ -1200,11 +1228,16 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
       if (!bpa)
         HIPCHECK(hipMemcpy(dpos + I.elem0 * nnpe, gpos.data(), gpos.size() * 4,
                            hipMemcpyHostToDevice));
-      if (bpa) {
+      if (bpa && element_type == AKX_TET4) {
         HIPCHECK(hipMemcpy(b->lc_t.p + I.elem0 * 4, lc.data(), lc.size() * 2,
                            hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(b->ls_t.p + I.elem0 * 4, ls.data(), ls.size() * 2,
                            hipMemcpyHostToDevice));
+      } else if (bpa) {
+        HIPCHECK(hipMemcpy(b->ls_h.p + I.elem0 * 8, ls.data(), ls.size() * 2,
+                           hipMemcpyHostToDevice));
+      }
+      if (bpa) {
         HIPCHECK(hipMemcpy(b->bnode.p + I.bent0, gbnode.data(), gbnode.size() * 4,
                            hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(b->bpos.p + I.bent0, gbpos.data(), gbpos.size() * 4,
@@ -1299,11 +1332,19 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
         }
         iblk.alloc(nblk);
         HIPCHECK(hipMemcpy(iblk.p, hb.data(), hb.size() * sizeof(BBlk), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_batch_mass_bpa, dim3((unsigned)nblk), dim3(256), 0, b->stream, A,
-                           (const BBlk *)iblk.p, (const uint16_t *)b->ls_t.p,
-                           (const uint16_t *)b->bsend.p, (const int32_t *)b->bpos.p);
-        hipLaunchKernelGGL(k_batch_dt<ET_TET4>, dim3(gridn(I.nel)), dim3(TPB), 0,
-                           b->stream, A);
+        if (element_type == AKX_TET4) {
+          hipLaunchKernelGGL(k_batch_mass_bpa<ET_TET4>, dim3((unsigned)nblk), dim3(256), 0,
+                             b->stream, A, (const BBlk *)iblk.p, (const uint16_t *)b->ls_t.p,
+                             (const uint16_t *)b->bsend.p, (const int32_t *)b->bpos.p);
+          hipLaunchKernelGGL(k_batch_dt<ET_TET4>, dim3(gridn(I.nel)), dim3(TPB), 0,
+                             b->stream, A);
+        } else {
+          hipLaunchKernelGGL(k_batch_mass_bpa<ET_HEX8>, dim3((unsigned)nblk), dim3(256), 0,
+                             b->stream, A, (const BBlk *)iblk.p, (const uint16_t *)b->ls_h.p,
+                             (const uint16_t *)b->bsend.p, (const int32_t *)b->bpos.p);
+          hipLaunchKernelGGL(k_batch_dt<ET_HEX8>, dim3(gridn(I.nel)), dim3(TPB), 0,
+                             b->stream, A);
+        }
       } else if (element_type == AKX_TET4) {
         hipLaunchKernelGGL(k_batch_mass<ET_TET4>, dim3(gridn(I.nel)), dim3(TPB), 0,
                            b->stream, A);
@@ -1603,9 +1644,10 @@ int akx_batch_get_kernel_bytes(akx_batch *b, int kernel_class, double *bytes) {
       const bool bpa = !I.blk_cnt.empty();
       const double entries = bpa ? (double)I.nbent : corners;
       if (kernel_class == 0) {
-        if (bpa) // local corners + slots (2 B each), per entry node id, slot
-                 // end and table position, the partial written
-          total += corners * (2 + 2) + entries * (4 + 2 + 4 + 3 * fs) + nn * 2 * 3 * 8;
+        if (bpa) // local corners (Tet4) + slots (2 B each), per entry node id
+                 // (Tet4), slot end and table position, the partial written
+          total += corners * (I.etype == AKX_TET4 ? 4 : 2) +
+                   entries * ((I.etype == AKX_TET4 ? 4 : 0) + 2 + 4 + 3 * fs) + nn * 2 * 3 * 8;
         else
           total += corners * (4 + 4 + 3 * fs) + nn * 2 * 3 * 8;
         if (I.law == AKX_LAW_PLASTIC_LIH) // history read and written

@@ -76,6 +76,11 @@ struct BatchElemArgs {
 
 constexpr int HEX_TAB_N = 192, HEX_TAB_W = 256, HEX_TAB_SIZE = 264;
 
+// elements per block of the block-partial assembly: 256 Tet4; 192 Hex8, whose
+// 1,536 corner slots (18 KB of LDS) leave room for eight 256-lane
+// workgroups per CU (256 elements' 24 KB allowed six)
+constexpr int BPA_TET4 = 256, BPA_HEX8 = 192;
+
 // values per elemental-force entry in a batch's table.  3: a zero-padded
 // 4-wide entry (one vector store per corner) measured slower, its extra
 // bytes cost more than the saved cache accesses.

@@ -389,18 +389,25 @@ __device__ __forceinline__ float hex_sf_force(float R, float qx, float qy, float
 
 // All 64 lanes of the wave call it (cross-lane steps); lanes 8 e .. 8 e + 7
 // are element e, lane bits b = l % 8.  Lanes of elements beyond the range
-// pass active = false and compute on zeros.
-template <int LAW, class FT, class VT, class NA, int W = 3>
+// pass active = false and compute on zeros.  SLOT: the node's force goes to
+// LDS slot ls_e[node] of `slots` (block-partial assembly) instead of table
+// entry pos_e[node] of fe.
+template <int LAW, class FT, class VT, class NA, int W = 3, bool SLOT = false>
 __device__ __forceinline__ void hex8_group_sf(int b, bool active, const int32_t *conn_e,
                                               const NA &nodes_of, float lambda, float mu,
                                               float sy, float hh, const VT &V, int64_t iq,
-                                              const int32_t *pos_e, FT *fe) {
+                                              const int32_t *pos_e, FT *fe,
+                                              float *slots = nullptr,
+                                              const uint16_t *ls_e = nullptr) {
   const int node = b ^ ((b >> 1) & 1);
   double x[3] = {0., 0., 0.}, d[3] = {0., 0., 0.};
   int32_t pq = 0;
   if (active) {
     nodes_of.load(conn_e[node], x, d);
-    pq = pos_e[node];
+    if constexpr (SLOT)
+      pq = ls_e[node];
+    else
+      pq = pos_e[node];
   }
   const float R = hex_r_vgpr();
   // relative to node 0 (lane bits 0): exact FP64 difference, rounded once
@@ -445,8 +452,16 @@ __device__ __forceinline__ void hex8_group_sf(int b, bool active, const int32_t 
     }
     F[k] = hex_sf_force(R, q[0], q[1], q[2]);
   }
-  if (active)
-    akx::put_entry<W>(fe, (int64_t)pq, FT(F[0]), FT(F[1]), FT(F[2]));
+  if (active) {
+    if constexpr (SLOT) {
+      float *o = slots + 3 * pq;
+      o[0] = F[0];
+      o[1] = F[1];
+      o[2] = F[2];
+    } else {
+      akx::put_entry<W>(fe, (int64_t)pq, FT(F[0]), FT(F[1]), FT(F[2]));
+    }
+  }
 }
 
 template <int LAW, class FT>
@@ -782,6 +797,46 @@ __global__ void __launch_bounds__(EPB * 8) k_batch_hex8_f32(BatchElemArgs A) {
   }
 }
 
+// Batched FP32 Hex8 with block-partial assembly: each tile is one block of at
+// most BPA_HEX8 consecutive elements of one instance and each workgroup (256
+// lanes, 32 elements at a time) does one tile.  The lanes put their nodes'
+// forces in LDS slots (the corners of one distinct node of the block take
+// consecutive slots in ascending element order), then one thread per
+// distinct node adds its slots in that order (FP64, rounded once) and
+// writes one table entry per (block, node).
+template <int LAW>
+__global__ void __launch_bounds__(256) k_batch_hex8_bpa(BatchElemArgs A) {
+  __shared__ float s_f[akx::BPA_HEX8 * 8 * 3]; // one slot per element corner of the block
+  const int t = threadIdx.x, el = t >> 3, q = t & 7;
+  const int64_t tt = blockIdx.x;
+  if (tt >= A.ntiles)
+    return;
+  const BTile T = A.tiles[tt];
+  const BBlk B = A.blks[tt];
+  const BInst P = A.inst[T.inst];
+  const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
+  for (int base = 0; base < T.count; base += 32) {
+    const bool active = base + el < T.count;
+    const int64_t e = T.first + (active ? base + el : 0);
+    hex8_group_sf<LAW, float, QpCompact, NodesXU, FE_W, true>(
+        q, active, A.conn + e * 8, NodesXU{A.xu}, float(P.lambda), float(P.mu),
+        float(P.sigma_y), float(P.h), V, A.hoff + e * 8 + q, nullptr, nullptr, s_f,
+        A.ls + e * 8);
+  }
+  __syncthreads();
+  float *fe = reinterpret_cast<float *>(A.fe);
+  for (int k = t; k < B.count; k += blockDim.x) {
+    const int s0 = k > 0 ? (int)A.bsend[B.off + k - 1] : 0, s1 = A.bsend[B.off + k];
+    double fx = 0., fy = 0., fz = 0.;
+    for (int sl = s0; sl < s1; ++sl) {
+      fx += (double)s_f[3 * sl];
+      fy += (double)s_f[3 * sl + 1];
+      fz += (double)s_f[3 * sl + 2];
+    }
+    akx::put_entry<3>(fe, (int64_t)A.bpos[B.off + k], float(fx), float(fy), float(fz));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -842,6 +897,8 @@ static void launch_batch_law(int etype, int block, const BatchElemArgs &A,
                          (size_t)A.bmaxu * 48 + 1024 * 12, s, A);
     else
       hipLaunchKernelGGL(k_batch_tet4_f32<LAW>, grid, dim3(block), 0, s, A);
+  } else if (A.blks) { // block-partial assembly: 256 lanes, LDS corner slots
+    hipLaunchKernelGGL(k_batch_hex8_bpa<LAW>, grid, dim3(256), 0, s, A);
   } else if (block == 64) {
     hipLaunchKernelGGL((k_batch_hex8_f32<LAW, 8>), grid, dim3(64), 0, s, A);
   } else if (block == 128) {
