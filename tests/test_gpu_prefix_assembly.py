@@ -105,6 +105,17 @@ def test_prefix_assembly_shuffled_element_order():
     _same_bits(on, off, "shuffled")
 
 
+def test_prefix_assembly_large_shuffled_mesh():
+    """A shuffled 153,600-element mesh: a block's 1,024 corners fall on up to
+    1,020 distinct nodes (48 KB of records, then the slots in the same LDS)."""
+    mesh, blocked, v0 = workload("_tetrahedron_4", (80, 20, 16), jitter=0.1, seed=68)
+    perm = np.random.default_rng(68).permutation(mesh.nb_elements)
+    mesh = dataclasses.replace(mesh, conn=np.ascontiguousarray(mesh.conn[perm]))
+    on = _tet4(mesh, blocked, v0, "elastic", True)
+    off = _tet4(mesh, blocked, v0, "elastic", False, dt=on.getTimeStep())
+    _same_bits(on, off, "shared LDS", counts=(1, 4))
+
+
 def test_prefix_assembly_pipelined_schedule():
     """The pipelined schedule's partial launches write the per-corner table;
     a model with prefix-partial assembly switches between the two."""
