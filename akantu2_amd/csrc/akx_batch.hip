@@ -107,6 +107,7 @@ struct BatchGatherArgs {
   const void *fe;
   double *fint, *v, *a; // [3][ncap]
   double *xu;           // [ncap][6] node records {X, u}
+  double *u;            // [3][ncap] displacement of the instances with urec = 0
   const double *mass;
   const uint8_t *blocked;
   float *cu, *cv; // compensation terms (FP32 form) or null
@@ -126,6 +127,7 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
     if ((int)threadIdx.x >= SL.count)
       return;
     const double dt = G.inst[SL.inst].dt;
+    const bool urec = G.inst[SL.inst].urec != 0;
     const int64_t n = SL.node0 + threadIdx.x;
     const FT *p = reinterpret_cast<const FT *>(G.fe) + (G.ell_off[n >> 8] + (n & 255)) * FE_W;
     double f[3] = {0., 0., 0.};
@@ -155,14 +157,21 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
     const double M = G.mass[n];
     const uint8_t blk = G.blocked[n];
     const double dd = .5 * dt; // beta * dt
-    // u of the node's record (u0 at 48 n + 24 bytes, then an aligned pair),
-    // read and written whole; blocked components keep their value
+    // u of the node's record (u0 at 48 n + 24 bytes, then an aligned pair)
+    // or of the [3][ncap] array, read and written whole; blocked components
+    // keep their value
     double us[3] = {0., 0., 0.};
     if constexpr (PREDICT) {
-      const double2 u12 = *reinterpret_cast<const double2 *>(G.xu + 6 * n + 4);
-      us[0] = G.xu[6 * n + 3];
-      us[1] = u12.x;
-      us[2] = u12.y;
+      if (urec) {
+        const double2 u12 = *reinterpret_cast<const double2 *>(G.xu + 6 * n + 4);
+        us[0] = G.xu[6 * n + 3];
+        us[1] = u12.x;
+        us[2] = u12.y;
+      } else {
+        us[0] = G.u[n];
+        us[1] = G.u[G.ncap + n];
+        us[2] = G.u[2 * G.ncap + n];
+      }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -203,8 +212,14 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
       G.v[o] = v;
     }
     if constexpr (PREDICT) {
-      G.xu[6 * n + 3] = us[0];
-      *reinterpret_cast<double2 *>(G.xu + 6 * n + 4) = make_double2(us[1], us[2]);
+      if (urec) {
+        G.xu[6 * n + 3] = us[0];
+        *reinterpret_cast<double2 *>(G.xu + 6 * n + 4) = make_double2(us[1], us[2]);
+      } else {
+        G.u[n] = us[0];
+        G.u[G.ncap + n] = us[1];
+        G.u[2 * G.ncap + n] = us[2];
+      }
     }
   }
 }
@@ -220,10 +235,20 @@ __global__ void __launch_bounds__(256) k_batch_predictor(BatchGatherArgs G) {
     if ((int)threadIdx.x >= SL.count)
       return;
     const double dt = G.inst[SL.inst].dt;
+    const bool urec = G.inst[SL.inst].urec != 0;
     const int64_t n = SL.node0 + threadIdx.x;
     const uint8_t blk = G.blocked[n];
-    const double2 u12 = *reinterpret_cast<const double2 *>(G.xu + 6 * n + 4);
-    double us[3] = {G.xu[6 * n + 3], u12.x, u12.y};
+    double us[3];
+    if (urec) {
+      const double2 u12 = *reinterpret_cast<const double2 *>(G.xu + 6 * n + 4);
+      us[0] = G.xu[6 * n + 3];
+      us[1] = u12.x;
+      us[2] = u12.y;
+    } else {
+      us[0] = G.u[n];
+      us[1] = G.u[G.ncap + n];
+      us[2] = G.u[2 * G.ncap + n];
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       if ((blk >> c) & 1)
@@ -245,8 +270,14 @@ __global__ void __launch_bounds__(256) k_batch_predictor(BatchGatherArgs G) {
       us[c] = u;
       G.v[o] = v;
     }
-    G.xu[6 * n + 3] = us[0];
-    *reinterpret_cast<double2 *>(G.xu + 6 * n + 4) = make_double2(us[1], us[2]);
+    if (urec) {
+      G.xu[6 * n + 3] = us[0];
+      *reinterpret_cast<double2 *>(G.xu + 6 * n + 4) = make_double2(us[1], us[2]);
+    } else {
+      G.u[n] = us[0];
+      G.u[G.ncap + n] = us[1];
+      G.u[2 * G.ncap + n] = us[2];
+    }
   }
 }
 
@@ -517,6 +548,7 @@ struct HostInst {
   int64_t elem0 = 0, nel = 0;            // in the arena of its element type
   int64_t ell0 = 0, nell = 0;            // force-table entries
   int64_t steps = 0;                     // steps done
+  bool urec = true;                      // u in the node records (BInst::urec)
   // block-partial assembly (Tet4, FP32): the instance's block-entry range
   // and the distinct-node count of each of its blocks
   int64_t bent0 = 0, nbent = 0;
@@ -536,6 +568,7 @@ struct akx_batch {
   int64_t ncap = 0, tcap = 0, hexcap = 0, ellcap = 0;
   // arenas
   DevBuf<double> xu; // [ncap][6] node records {X, u}
+  DevBuf<double> bx, bu; // [3][ncap] X and u (FP32 batches with Tet4 capacity)
   DevBuf<double> v, a, fint, mass;
   DevBuf<uint8_t> blocked, deg;
   DevBuf<float> cu, cv;
@@ -620,7 +653,8 @@ void upload_inst(akx_batch *b) {
   std::vector<BInst> ti(std::max<size_t>(b->inst.size(), 1));
   for (size_t id = 0; id < b->inst.size(); ++id) {
     const HostInst &I = b->inst[id];
-    ti[id] = BInst{I.lambda, I.mu, I.sy, I.h, I.rho, I.dt, (int32_t)(I.steps & 1), I.law};
+    ti[id] = BInst{I.lambda, I.mu, I.sy, I.h, I.rho, I.dt, (int32_t)(I.steps & 1), I.law,
+                   I.urec ? 1 : 0, 0};
   }
   sync(b);
   HIPCHECK(hipMemcpy(b->d_inst.p, ti.data(), ti.size() * sizeof(BInst),
@@ -694,6 +728,8 @@ BatchElemArgs elem_args(akx_batch *b, int etype, int64_t t0, int64_t t1, int par
   A.conn = etype == AKX_TET4 ? b->conn_t.p : b->conn_h.p;
   A.pos = etype == AKX_TET4 ? b->pos_t.p : b->pos_h.p;
   A.xu = b->xu.p;
+  A.X = b->bx.p;
+  A.u = b->bu.p;
   A.ncap = b->ncap;
   A.fe = b->fe.p;
   for (int s = 0; s < 2; ++s)
@@ -775,6 +811,7 @@ BatchGatherArgs gather_args(akx_batch *b) {
   G.fe = b->fe.p;
   G.fint = b->fint.p;
   G.xu = b->xu.p;
+  G.u = b->bu.p;
   G.v = b->v.p;
   G.a = b->a.p;
   G.mass = b->mass.p;
@@ -1005,6 +1042,8 @@ int akx_batch_create(int device, int precision, int64_t node_capacity,
       if (b->tcap) {
         b->lc_t.alloc(4 * b->tcap);
         b->ls_t.alloc(4 * b->tcap);
+        b->bx.alloc(3 * b->ncap);
+        b->bu.alloc(3 * b->ncap);
       }
       if (b->hexcap)
         b->ls_h.alloc(8 * b->hexcap);
@@ -1083,6 +1122,8 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
     HostInst I;
     I.etype = element_type;
     I.law = law;
+    I.urec = !(b->fp32() && element_type == AKX_TET4);
+    require(I.urec || (b->bx.p && b->bu.p), "FP32 Tet4 instance without its X and u arrays");
     I.rho = params[0];
     lame(params[1], params[2], I.lambda, I.mu);
     I.sy = params[3];
@@ -1262,8 +1303,17 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
                          (const double *)b->staging.p, b->xu.p, (int64_t)1, (int64_t)6,
                          I.node0, nb_nodes);
       HIPCHECK(hipGetLastError());
-      for (auto *f : {&b->v, &b->a, &b->fint})
+      if (!I.urec) { // X and u = 0 in the [3][ncap] arrays
         for (int c = 0; c < 3; ++c)
+          hipLaunchKernelGGL((k_batch_fill<double>), dim3(gridn(I.nn_pad)), dim3(TPB), 0,
+                             b->stream, b->bx.p + c * b->ncap + I.node0, I.nn_pad, 0.);
+        hipLaunchKernelGGL(k_batch_put3, dim3(gridn(3 * nb_nodes)), dim3(TPB), 0, b->stream,
+                           (const double *)b->staging.p, b->bx.p, b->ncap, (int64_t)1,
+                           I.node0, nb_nodes);
+        HIPCHECK(hipGetLastError());
+      }
+      for (auto *f : {&b->v, &b->a, &b->fint, &b->bu})
+        for (int c = 0; c < 3 && f->p; ++c)
           hipLaunchKernelGGL((k_batch_fill<double>), dim3(gridn(I.nn_pad)), dim3(TPB), 0,
                              b->stream, f->p + c * b->ncap + I.node0, I.nn_pad, 0.);
       if (b->fp32())
@@ -1459,9 +1509,13 @@ int akx_batch_get_field(akx_batch *b, int id, int field, double *host) {
     int64_t comp_stride = b->ncap, node_stride = 1;
     switch (field) {
     case AKX_FIELD_DISPLACEMENT:
-      src = b->xu.p + 3;
-      comp_stride = 1;
-      node_stride = 6;
+      if (I.urec) {
+        src = b->xu.p + 3;
+        comp_stride = 1;
+        node_stride = 6;
+      } else {
+        src = b->bu.p;
+      }
       break;
     case AKX_FIELD_VELOCITY: src = b->v.p; break;
     case AKX_FIELD_ACCELERATION: src = b->a.p; break;

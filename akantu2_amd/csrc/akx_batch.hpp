@@ -22,6 +22,12 @@ struct BInst { // one per instance id
   double lambda, mu, sigma_y, h, rho, dt;
   int32_t parity_base; // steps done before the current solve call, mod 2
   int32_t law;
+  // where the instance's displacement lives: 1 in the {X, u} node records
+  // (the passes that read corners from them), 0 in the [3][ncap] array u
+  // (FP32 Tet4: its block pass builds the records in LDS from X and u, and
+  // the gathers then store u whole instead of half-records)
+  int32_t urec;
+  int32_t pad;
 };
 
 struct BTile {
@@ -61,6 +67,7 @@ struct BatchElemArgs {
   const int32_t *conn; // [elem][nnpe] arena node ids
   const int32_t *pos;  // [elem][nnpe] table entries
   const double *xu;    // [ncap][6] node records {X, u}
+  const double *X, *u; // [3][ncap] coordinates and displacement (BInst::urec = 0)
   int64_t ncap;
   void *fe;            // [entry][FE_W] elemental forces (FP64 or FP32)
   // plastic history, two buffers (ping-pong by step parity).  FP64 form:

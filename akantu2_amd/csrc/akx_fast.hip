@@ -736,12 +736,13 @@ __global__ void __launch_bounds__(256) k_batch_tet4_bpa(BatchElemArgs A) {
     const BBlk B = A.blks[tt];
     const BInst P = A.inst[T.inst];
     const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
+    // the records of the block's distinct nodes, built from X and u (FP32
+    // Tet4 instances keep u in the [3][ncap] array, BInst::urec = 0)
     for (int k = threadIdx.x; k < B.count; k += blockDim.x) {
-      const double2 *p =
-          reinterpret_cast<const double2 *>(A.xu + 6 * (int64_t)A.bnode[B.off + k]);
-      s_xu[3 * k] = p[0];
-      s_xu[3 * k + 1] = p[1];
-      s_xu[3 * k + 2] = p[2];
+      const int64_t nd = A.bnode[B.off + k];
+      s_xu[3 * k] = make_double2(A.X[nd], A.X[A.ncap + nd]);
+      s_xu[3 * k + 1] = make_double2(A.X[2 * A.ncap + nd], A.u[nd]);
+      s_xu[3 * k + 2] = make_double2(A.u[A.ncap + nd], A.u[2 * A.ncap + nd]);
     }
     __syncthreads();
     for (int j = threadIdx.x; j < T.count; j += blockDim.x) {
