@@ -348,9 +348,12 @@ int akx_calibration_copy(int device, int64_t bytes, int repeats, double *ms);
  * An `akx_batch` holds many independent instances (each one what the
  * reference runs as its own SolidMechanicsModel with one material, explicit
  * lumped mass and zero external force) in device arenas sized at creation,
- * and advances all live instances together: per step one persistent element
- * launch per (element type, law) group present and one persistent nodal
- * gather + update launch, captured as a hipGraph of two steps.  The
+ * and advances all live instances together: per step one element launch per
+ * (element type, law) group present (one tile of elements per workgroup)
+ * and one nodal gather + update launch (one 256-node slice per workgroup),
+ * captured as a hipGraph of two steps.  FP32 batches assemble by block
+ * partials (one force-table entry per (element block, node), added in block
+ * order).  The
  * reference has no batch object; per instance the calls stand for
  *   SolidMechanicsModel(mesh) + addMaterial + initFull   -> add_instance
  *   getStableTimeStep / setTimeStep                      -> get/set_time_step
