@@ -480,6 +480,51 @@ __global__ void __launch_bounds__(256) k_hex8_sf(ElemArgs A) {
       i * 8 + b, A.pos + (A.pos_offset + (active ? i : 0)) * 8, reinterpret_cast<FT *>(A.fe));
 }
 
+// FP32 Hex8 with block-partial assembly: each workgroup (256 lanes, 32
+// elements at a time through the sum-factorised body) does one block of
+// BPA_HEX8 consecutive elements; the lanes put their nodes' forces in LDS
+// slots (the corners of one distinct node of the block take consecutive
+// slots in ascending element order), then one thread per distinct node adds
+// its slots in that order (FP64, rounded once) and writes one table entry
+// per (block, node).  The gather adds a node's partials in block order.
+template <int LAW>
+__global__ void __launch_bounds__(256) k_hex8_bpa(ElemArgs A) {
+  __shared__ float s_f[akx::BPA_HEX8 * 8 * 3];
+  const int t = threadIdx.x, el = t >> 3, b = t & 7;
+  const int64_t blk = xcd_block(A.nblocks_padded);
+  const int64_t i0 = blk * akx::BPA_HEX8;
+  if (i0 >= A.n) // padding block: no elements, no node list
+    return;
+  for (int base = 0; base < akx::BPA_HEX8; base += HEX_EPB) {
+    const int64_t i = i0 + base + el;
+    const bool active = base + el < akx::BPA_HEX8 && i < A.n;
+    const int64_t ic = active ? i : i0;
+    float lambda = 0.f, mu = 0.f, sy = 0.f, hh = 0.f;
+    if (active)
+      element_params(A, i, lambda, mu, sy, hh);
+    hex8_group_sf<LAW, float, QpViewT<double>, NodesSoA, 3, true>(
+        b, active, A.conn + (A.first + ic) * 8, NodesSoA{A.X, A.u, A.nn}, lambda, mu, sy,
+        hh, qp_view(A, 8), ic * 8 + b, nullptr, nullptr, s_f, A.lslot + ic * 8);
+  }
+  __syncthreads();
+  const uint16_t *soff = A.bl_soff + blk * (A.bl_maxu + 1);
+  const int32_t *dst = A.bl_pos + blk * A.bl_maxu;
+  float *fe = reinterpret_cast<float *>(A.fe);
+  for (int k = t; k < A.bl_maxu; k += blockDim.x) {
+    const int32_t d = dst[k];
+    if (d < 0)
+      continue;
+    const int s0 = soff[k], s1 = soff[k + 1];
+    double fx = 0., fy = 0., fz = 0.;
+    for (int sl = s0; sl < s1; ++sl) {
+      fx += (double)s_f[3 * sl];
+      fy += (double)s_f[3 * sl + 1];
+      fz += (double)s_f[3 * sl + 2];
+    }
+    akx::put_entry<3>(fe, (int64_t)d, float(fx), float(fy), float(fz));
+  }
+}
+
 template <int LAW, class CT, class FT>
 __global__ void __launch_bounds__(256) k_hex8_fast(ElemArgs A) {
   __shared__ CT sN[HEX_EPB][8][6];     // X (0..2) and u (3..5) of the 8 nodes
@@ -845,7 +890,9 @@ hipError_t upload_hex8_dnds(const double *dnds) {
   return hipMemcpyToSymbol(HIP_SYMBOL(c_dnds), dnds, sizeof(double) * 192);
 }
 
-int64_t elements_per_block(int etype) { return etype == 1 ? HEX_EPB : 256; }
+int64_t elements_per_block(int etype, bool bpa) {
+  return etype == 1 ? (bpa ? akx::BPA_HEX8 : HEX_EPB) : 256;
+}
 
 template <int L, class CT, class FT>
 static void launch_one(int etype, const ElemArgs &A, hipStream_t s) {
@@ -862,8 +909,12 @@ static void launch_one(int etype, const ElemArgs &A, hipStream_t s) {
       }
     hipLaunchKernelGGL((k_tet4_fast<L, CT, FT>), grid, block, 0, s, A);
   }
-  else if constexpr (sizeof(CT) == 4) // FP32: sum-factorised Hex8
-    hipLaunchKernelGGL((k_hex8_sf<L, FT>), grid, block, 0, s, A);
+  else if constexpr (sizeof(CT) == 4) { // FP32: sum-factorised Hex8
+    if (A.lslot) // block-partial assembly
+      hipLaunchKernelGGL((k_hex8_bpa<L>), grid, block, 0, s, A);
+    else
+      hipLaunchKernelGGL((k_hex8_sf<L, FT>), grid, block, 0, s, A);
+  }
   else
     hipLaunchKernelGGL((k_hex8_fast<L, CT, FT>), grid, block, 0, s, A);
 }

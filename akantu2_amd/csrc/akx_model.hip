@@ -751,16 +751,18 @@ void refresh_xu(akx_model *m, bool with_x) {
   HIPCHECK(hipGetLastError());
 }
 
-// Tet4 element passes with staged node records: for each block of bs
-// consecutive positions of a contiguous material (position i holds element
-// order[i], or i without an order), its distinct nodes in order of first use
-// (at most 4 bs; every block's list padded with -1 to the longest) and each
-// position's corners as indices into them
-void build_block_nodes(akx_model *m, Material &mat, const std::vector<int32_t> *order = nullptr,
-                       int64_t bs = 256) {
+// Element passes with block lists: for each block of bs consecutive
+// elements of a contiguous material, its distinct nodes in order of first
+// use (every block's list padded with -1 to the longest) and each element's
+// corners as indices into them (the Tet4 passes stage these nodes' records
+// in LDS).  In FP32 also, for block-partial assembly, the LDS slot of every
+// corner: the corners of distinct node k of a block take consecutive slots
+// in ascending element (then corner) order.
+void build_block_nodes(akx_model *m, Material &mat, int64_t bs = 256) {
+  const int nnpe = m->nnpe;
   const int64_t n = mat.n, nb = (n + bs - 1) / bs;
   std::vector<int32_t> off(nb + 1, 0), nodes;
-  std::vector<uint16_t> lc(n * 4);
+  std::vector<uint16_t> lc(n * nnpe);
   std::vector<int64_t> stamp(m->nn, -1);
   std::vector<int32_t> local(m->nn, 0);
   nodes.reserve(n);
@@ -768,15 +770,14 @@ void build_block_nodes(akx_model *m, Material &mat, const std::vector<int32_t> *
   for (int64_t b = 0; b < nb; ++b) {
     const int64_t start = (int64_t)nodes.size();
     for (int64_t i = b * bs; i < std::min<int64_t>(n, b * bs + bs); ++i)
-      for (int a = 0; a < 4; ++a) {
-        const int64_t e = order ? (int64_t)(*order)[i] : i;
-        const int32_t node = m->conn_host[(mat.first + e) * 4 + a];
+      for (int a = 0; a < nnpe; ++a) {
+        const int32_t node = m->conn_host[(mat.first + i) * nnpe + a];
         if (stamp[node] != b) {
           stamp[node] = b;
           local[node] = (int32_t)((int64_t)nodes.size() - start);
           nodes.push_back(node);
         }
-        lc[i * 4 + a] = (uint16_t)local[node];
+        lc[i * nnpe + a] = (uint16_t)local[node];
       }
     require((int64_t)nodes.size() < (int64_t)INT32_MAX, "block node lists too large");
     off[b + 1] = (int32_t)nodes.size();
@@ -787,30 +788,28 @@ void build_block_nodes(akx_model *m, Material &mat, const std::vector<int32_t> *
     std::copy(nodes.begin() + off[b], nodes.begin() + off[b + 1],
               padded.begin() + b * maxu);
   mat.bl_nodes.alloc(padded.size());
-  mat.lconn.alloc(n * 4);
+  mat.lconn.alloc(n * nnpe);
   HIPCHECK(hipMemcpy(mat.bl_nodes.p, padded.data(), padded.size() * sizeof(int32_t),
                      hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(mat.lconn.p, lc.data(), lc.size() * sizeof(uint16_t),
                      hipMemcpyHostToDevice));
   mat.bl_maxu = maxu;
   mat.bl_nodes_host = std::move(padded);
-  if (m->precision != AKX_PRECISION_FP32 || order)
+  if (m->precision != AKX_PRECISION_FP32)
     return;
-  // block-partial assembly: the corners of distinct node k of a block take
-  // consecutive LDS slots, in ascending element (then corner) order
-  std::vector<uint16_t> soff((size_t)nb * (maxu + 1), 0), slot(n * 4);
+  std::vector<uint16_t> soff((size_t)nb * (maxu + 1), 0), slot(n * nnpe);
   std::vector<uint16_t> cnt(maxu + 1);
   for (int64_t b = 0; b < nb; ++b) {
-    const int64_t e0 = b * 256, e1 = std::min<int64_t>(n, b * 256 + 256);
+    const int64_t e0 = b * bs, e1 = std::min<int64_t>(n, b * bs + bs);
     std::fill(cnt.begin(), cnt.end(), 0);
-    for (int64_t i = e0 * 4; i < e1 * 4; ++i)
+    for (int64_t i = e0 * nnpe; i < e1 * nnpe; ++i)
       cnt[lc[i]]++;
     uint16_t *o = &soff[b * (maxu + 1)];
     o[0] = 0;
     for (int k = 0; k < maxu; ++k)
       o[k + 1] = (uint16_t)(o[k] + cnt[k]);
     std::fill(cnt.begin(), cnt.end(), 0);
-    for (int64_t i = e0 * 4; i < e1 * 4; ++i)
+    for (int64_t i = e0 * nnpe; i < e1 * nnpe; ++i)
       slot[i] = (uint16_t)(o[lc[i]] + cnt[lc[i]]++);
   }
   mat.lslot.alloc(slot.size());
@@ -1276,7 +1275,7 @@ void launch_element_pass(akx_model *m, bool materialize) {
       int64_t nb = (mat.n + HEX_STRICT_EPB - 1) / HEX_STRICT_EPB;
       A.nblocks_padded = (nb + 7) / 8 * 8;
     } else if (!strict) {
-      int64_t epb = akxf::elements_per_block(m->etype);
+      int64_t epb = akxf::elements_per_block(m->etype, m->bpa);
       int64_t nb = (mat.n + epb - 1) / epb;
       A.nblocks_padded = (nb + 7) / 8 * 8;
     }
@@ -2172,9 +2171,14 @@ int akx_model_init_full(akx_model *m) {
         build_ppa(m, m->assembly == AKX_ASSEMBLY_BLOCK_PARTIALS);
       m->ppa = ppa;
     }
+    if (m->etype == AKX_HEX8 && m->precision == AKX_PRECISION_FP32 && m->nn > 0 &&
+        m->assembly != AKX_ASSEMBLY_PER_CORNER)
+      for (auto &mp : m->mats) // FP32 Hex8 block partials
+        if (mp->contiguous && mp->n > 0)
+          build_block_nodes(m, *mp, akxf::elements_per_block(AKX_HEX8, true));
     build_gather_table(m);
-    m->bpa = m->assembly != AKX_ASSEMBLY_PER_CORNER && m->etype == AKX_TET4 &&
-             m->precision == AKX_PRECISION_FP32 && m->nn > 0;
+    m->bpa = m->assembly != AKX_ASSEMBLY_PER_CORNER && m->precision == AKX_PRECISION_FP32 &&
+             m->nn > 0;
     for (auto &mp : m->mats)
       m->bpa = m->bpa && mp->contiguous && mp->n > 0 && mp->lslot.p != nullptr;
     if (m->bpa)
@@ -3315,8 +3319,8 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
       nel_all += mp->n;
       if (mp->per_element) // lambda, mu per element: FP64, or the FP32 pair
         extra += mp->n * (mp->lm_f.p ? 2.0 * 4.0 : 2.0 * 8.0);
-      if (mp->lconn.p) // 2-byte local corners instead of 4-byte node ids, plus
-                       // the blocks' node lists
+      if (mp->lconn.p && m->etype == AKX_TET4) // staged records: 2-byte local
+        // corners instead of 4-byte node ids, plus the blocks' node lists
         extra += -2.0 * m->nnpe * mp->n + 4.0 * (double)mp->bl_nodes.n;
       if (mp->law == AKX_LAW_PLASTIC_LIH)
         extra += (double)mp->n * m->nqp * (2 * (9 + 9 + 9 + 1) * 8.0); // read + write history
