@@ -690,10 +690,10 @@ class SolidMechanicsModel:
     _ASSEMBLIES = {"per_corner": 0, "default": 1, "block_partials": 2}
 
     def setBlockAssembly(self, enable):
-        """Tet4 assembly (see akx_model_set_block_assembly), before initFull:
-        True / "default" (FP64: prefix partials, same bits; FP32: block
-        partials), False / "per_corner", or "block_partials" (also in FP64,
-        other rounding)."""
+        """Element-force assembly (see akx_model_set_block_assembly), before
+        initFull: True / "default" (Tet4 FP64: prefix partials, same bits;
+        FP32 Tet4 and Hex8: block partials), False / "per_corner", or
+        "block_partials" (also for Tet4 in FP64, other rounding)."""
         mode = self._ASSEMBLIES[enable] if isinstance(enable, str) else int(bool(enable))
         N.check(N.lib().akx_model_set_block_assembly(self._h, mode))
 

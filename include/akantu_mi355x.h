@@ -301,11 +301,12 @@ int akx_model_get_energy_noalloc(akx_model *m, const char *energy_id, int materi
  * whether it applies.  parts = 0 or 1: the plain schedule (default). */
 int akx_model_set_pipeline(akx_model *m, int parts, int *active);
 
-/* Assembly of the Tet4 element passes, no counterpart in the reference
- * (call before akx_model_init_full).  It applies to Tet4 meshes whose
- * materials are all contiguous element ranges (and, in FP64, built-in laws).
- * A node's corners, in assembly order (materials in order, elements
- * ascending), come in runs: its corners in one 256-element block.
+/* Assembly of the element passes, no counterpart in the reference (call
+ * before akx_model_init_full).  It applies to models whose materials are all
+ * contiguous element ranges: Tet4 (in FP64 with built-in laws) and, in FP32,
+ * Hex8.  A node's corners, in assembly order (materials in order, elements
+ * ascending), come in runs: its corners in one block of elements (256 Tet4,
+ * 192 Hex8).
  * - AKX_ASSEMBLY_PER_CORNER: one table entry per element corner, added by
  *   the nodal gather in assembly order.
  * - AKX_ASSEMBLY_DEFAULT: in FP64, prefix partials: the block of a node's
