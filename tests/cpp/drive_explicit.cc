@@ -8,7 +8,9 @@
 //
 // <law name> "local_elastic" is a ConstitutiveLaw written here (the
 // reference's Material subclass overriding computeStress), the small-strain
-// elastic law in the reference's operation order.
+// elastic law in the reference's operation order; "device_elastic" is the
+// same material with the law given as device code (Material::setDeviceStress),
+// whose host computeStress must then never run.
 //
 // With "batch" the same simulation runs as one instance of an FP64 Batch
 // (include/akantu_mi355x.hpp), beside a second instance (the same mesh at
@@ -39,6 +41,7 @@ public:
   LocalElastic(double E, double nu)
       : lambda(nu * E / ((1. + nu) * (1. - 2. * nu))), mu(E / (2. * (1. + nu))) {}
   void computeStress(const Array<double> &grad_u, Array<double> &stress) override {
+    ++calls;
     for (std::int64_t q = 0; q < grad_u.size(); ++q) {
       auto g = [&](int i, int j) { return grad_u(q, i + 3 * j); };
       const double tr = g(0, 0) + g(1, 1) + g(2, 2), two_mu = 2. * mu;
@@ -50,7 +53,20 @@ public:
     }
   }
   double lambda, mu;
+  long calls = 0;
 };
+
+// the same law for one Gauss point, as device code (params: lambda, mu)
+const char *kDeviceElastic = R"(
+__device__ void akx_compute_stress(const double *g, double *s, const double *p) {
+  const double tr = g[0] + g[4] + g[8], two_mu = 2. * p[1];
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) {
+      const double eps = .5 * (g[j + 3 * i] + g[i + 3 * j]);
+      s[i + 3 * j] = i == j ? two_mu * eps + p[0] * tr : two_mu * eps;
+    }
+}
+)";
 
 template <class T> static std::vector<T> load(const std::string &path) {
   std::ifstream f(path, std::ios::binary | std::ios::ate);
@@ -121,9 +137,12 @@ int main(int argc, char **argv) {
     }
     SolidMechanicsModel model(mesh);
     LocalElastic local(P[1], P[2]);
-    if (law == "local_elastic")
+    if (law == "local_elastic") {
       model.addMaterial(local, {P[0], P[1], P[2]}, fd);
-    else
+    } else if (law == "device_elastic") {
+      Material &mat = model.addMaterial(local, {P[0], P[1], P[2]}, fd);
+      mat.setDeviceStress(kDeviceElastic, {local.lambda, local.mu});
+    } else
       model.addMaterial(law, {P[0], P[1], P[2], P[3], P[4]}, fd);
     model.initFull(_explicit_lumped_mass);
     auto &blocked = model.getBlockedDOFs();
@@ -155,6 +174,8 @@ int main(int argc, char **argv) {
     auto stress = model.getMaterial(0).getStress();
     save(dir + "/stress.f64", stress.data(), (size_t)stress.size() * 9);
     save(dir + "/dt.f64", &dt, 1);
+    if (law == "device_elastic" && local.calls != 0)
+      throw std::runtime_error("the host computeStress ran beside the device law");
   } catch (const JacobianException &e) {
     std::cerr << "jacobian: " << e.what() << "\n";
     return 3;

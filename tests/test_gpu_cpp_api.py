@@ -20,6 +20,8 @@ DRIVER = os.path.join(ROOT, "tests", "cpp", "drive_explicit")
     ("_hexahedron_8", "elastic", 1, (8, 4, 4), 200),
     ("_hexahedron_8", "local_elastic", 0, (6, 4, 3), 150),  # a ConstitutiveLaw
     ("_tetrahedron_4", "local_elastic", 0, (10, 3, 3), 300),
+    ("_hexahedron_8", "device_elastic", 0, (6, 4, 3), 150),  # Material::setDeviceStress
+    ("_tetrahedron_4", "device_elastic", 0, (10, 3, 3), 300),
 ])
 def test_cpp_driver_matches_oracle(tmp_path, etype, law, fd, shape, steps):
     assert os.path.exists(DRIVER), "build() compiles tests/cpp/drive_explicit"
@@ -34,7 +36,7 @@ def test_cpp_driver_matches_oracle(tmp_path, etype, law, fd, shape, steps):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     oracle_law = "elastic_fd" if (law == "elastic" and fd) else law
-    if law == "local_elastic":  # the driver's own ConstitutiveLaw
+    if law in ("local_elastic", "device_elastic"):  # the driver's own law
         oracle_law = "elastic"
     o, dt = make_oracle(mesh, blocked, v0, oracle_law)
     assert np.fromfile(os.path.join(d, "dt.f64"))[0] == dt
