@@ -3,7 +3,8 @@ the host's cores (OpenMP; its results do not depend on the thread count).
 
 - 200 x 100 x 100 jittered Hex8, neo-Hookean (2,000,000 elements,
   16,000,000 Gauss points): the plain FP64 path against the oracle after 3
-  steps (bar 1e-12) and the FP32 form within 1e-5;
+  steps (bar 1e-12) and the FP32 form within 1e-5; over 1,000 steps, the
+  gather's internal forces against the oracle's at every 100th step;
 - 1000 x 100 x 100 x 6 jittered Tet4 with per-element E, nu (60,000,000
   elements): the plain FP64 path bit for bit after 2 steps, the FP32 form
   within 1e-5; after the precision form's 2,000 steps, the FP32 form's L2 and
@@ -83,6 +84,30 @@ def test_hex8_neohookean_200x100x100(hex8_full, precision, tol):
         assert max(errs.values()) <= tol
         print("bit-identical f_int:", np.array_equal(g.getInternalForce(), o.f_int), flush=True)
     assert tr <= 1e-9
+
+
+@pytest.mark.timeout(600)
+def test_hex8_gather_checked_every_100_steps(hex8_full):
+    """The benchmark's run: 1,000 steps of the 2 M-element neo-Hookean box,
+    the deterministic gather assembly checked against the oracle at every
+    100th step -- the oracle's internal forces recomputed from the GPU's
+    displacement at that step, within 1e-12 (the last bit of the device's
+    and glibc's log aside) -- and the forces still summing to zero."""
+    mesh, blocked, v0, o = hex8_full
+    g, dt = make_gpu(mesh, blocked, v0, "neohookean", dt=o.dt)
+    chk = _oracle(mesh, blocked, v0, "neohookean", dt=o.dt)
+    for k in range(10):
+        g.solveSteps(100)
+        u, f = g.getDisplacement(), g.getInternalForce()
+        chk.u[:] = u
+        chk.internal_forces()
+        e = relerr(f, chk.f_int)
+        diff = int(np.count_nonzero(f != chk.f_int))
+        tr = _translation_residual(f)
+        print(f"step {100 * (k + 1)}: f_int {e:.2e} ({diff} of {f.size} values differ), "
+              f"sum f_int {tr:.1e}, max |u| {np.abs(u).max():.3e}", flush=True)
+        assert e <= TOL and tr <= 1e-9
+        assert np.isfinite(u).all() and np.abs(u).max() > 0
 
 
 def test_hex8_full_size_runs_are_bit_identical(hex8_full):
