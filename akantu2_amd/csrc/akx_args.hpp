@@ -1,8 +1,10 @@
 // akx_args.hpp -- plain argument structs shared by the element kernels of
 // the strict (reference operation order) and fast translation units.
 #pragma once
+#ifndef __HIPCC_RTC__ // run-time compilation provides these
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace akx {
 

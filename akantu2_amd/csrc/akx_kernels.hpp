@@ -18,9 +18,13 @@
 
 #include "akx_args.hpp"
 #include "akx_math.hpp"
+#ifndef __HIPCC_RTC__ // run-time compilation provides these
 #include <stdint.h>
+#endif
 
+#ifndef __HIPCC_RTC__ // run-time compilation provides these
 #include <cmath>
+#endif
 
 namespace akx {
 
@@ -38,8 +42,15 @@ constexpr int LAW_PLASTIC = 3;
 constexpr int LAW_USER_GRAD = 4;
 constexpr int LAW_USER = 5;
 constexpr int LAW_USER_FD = 6;
+// a user law compiled with the element pass at run time (akx_model_set_device_
+// stress): the program defines akx_compute_stress and akx_law_params and
+// AKX_DEVICE_LAW before including this header; sigma (LAW_DEVICE) or PK2
+// (LAW_DEVICE_FD) of one Gauss point from grad u, column-major
+constexpr int LAW_DEVICE = 7;
+constexpr int LAW_DEVICE_FD = 8;
 template <int LAW> constexpr bool finite_law() {
-  return LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN || LAW == LAW_USER_FD;
+  return LAW == LAW_ELASTIC_FD || LAW == LAW_NEOHOOKEAN || LAW == LAW_USER_FD ||
+         LAW == LAW_DEVICE_FD;
 }
 
 template <int ET> struct Elem;
@@ -66,6 +77,7 @@ extern __constant__ Hex8Tables c_hex8;
 // points +-1/sqrt(3), xi fastest (element_class_hexahedron_8_inline_impl.hh
 // :111-188, gauss_integration.cc:42-43); host code, shared by the
 // single-model and batched engines so both upload the same bits.
+#ifndef __HIPCC_RTC__ // host code
 inline void fill_hex8_tables(Hex8Tables &t) {
   double pos[2] = {-1. / std::sqrt(3.), 1. / std::sqrt(3.)};
   for (int q = 0; q < 8; ++q) {
@@ -113,6 +125,7 @@ inline void fill_hex8_tables(Hex8Tables &t) {
         t.dnds_f[q][i][n] = (float)d[i][n];
   }
 }
+#endif
 
 // blockIdx -> logical block so that each of the 8 XCDs walks a contiguous
 // range of elements/nodes (blocks are dealt round-robin to the XCDs; the
@@ -384,6 +397,25 @@ __device__ __forceinline__ void strict_law(const QpViewT<HT> &V, int64_t iq, CT 
       for (int ii = 0; ii < 3; ++ii)
         S[ii][jj] = CT(V.stress[(int64_t)(ii + 3 * jj) * V.nq_all + iq]);
     return;
+#ifdef AKX_DEVICE_LAW
+  } else if constexpr (LAW == LAW_DEVICE || LAW == LAW_DEVICE_FD) {
+    // the caller's law, inline: grad u and the stress column-major, as the
+    // host callback sees them (the stress is zero on entry)
+    double gc[9], sc[9];
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii) {
+        gc[ii + 3 * jj] = double(g[ii][jj]);
+        sc[ii + 3 * jj] = 0.;
+      }
+    akx_compute_stress(gc, sc, akx_law_params);
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+      for (int ii = 0; ii < 3; ++ii)
+        S[ii][jj] = CT(sc[ii + 3 * jj]);
+#endif
   } else {
     CT pg[3][3], ps[3][3], pe[3][3], eps[3][3], iso;
 #pragma unroll

@@ -14,7 +14,9 @@
 // out: that changes at most the sign of a zero result, never a value.
 #pragma once
 
+#ifndef __HIPCC_RTC__ // run-time compilation provides these
 #include <hip/hip_runtime.h>
+#endif
 
 #define AKX_HD __host__ __device__ __forceinline__
 

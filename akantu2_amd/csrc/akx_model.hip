@@ -497,11 +497,12 @@ using namespace akx;
 // ---------------------------------------------------------------------------
 // Model
 // ---------------------------------------------------------------------------
-// A user law compiled for the device at run time (akx_model_set_device_stress)
+// A user law compiled at run time into the material's element pass
+// (akx_model_set_device_stress): the pass without and with the eigen-gradient
+// terms
 struct DeviceLaw {
   hipModule_t module = nullptr;
-  hipFunction_t kernel = nullptr;
-  DevBuf<double> params;
+  hipFunction_t kernel[2] = {nullptr, nullptr};
   DeviceLaw() = default;
   DeviceLaw(const DeviceLaw &) = delete;
   DeviceLaw &operator=(const DeviceLaw &) = delete;
@@ -1251,6 +1252,8 @@ void launch_tet4_ppa(akx_model *m, Material &mat, const ElemArgs &A) {
     launch_tet4_ppa_x<false>(m, mat, A);
 }
 
+void launch_device_law(akx_model *m, Material &mat, ElemArgs A, bool materialize);
+
 void launch_element_pass(akx_model *m, bool materialize) {
   const bool strict = m->precision == AKX_PRECISION_FP64;
   for (auto &mp : m->mats) {
@@ -1286,7 +1289,10 @@ void launch_element_pass(akx_model *m, bool materialize) {
       A.nblocks_padded = (A.t_nblk + 7) / 8 * 8;
     }
     m->timed(0, [&] {
-      if (strict && m->ppa) {
+      if (mat.dlaw) {
+        require(strict, "device laws run on the plain FP64 path");
+        launch_device_law(m, mat, A, materialize);
+      } else if (strict && m->ppa) {
         launch_tet4_ppa(m, mat, A);
       } else if (strict) {
         if (m->etype == AKX_TET4)
@@ -1313,7 +1319,9 @@ void user_stress_phase(akx_model *m) {
     Material &mat = *m->mats[k];
     if (!mat.user() || mat.n == 0)
       continue;
-    require(mat.dlaw || m->stress_fn != nullptr,
+    if (mat.dlaw) // the law runs inside the element pass
+      continue;
+    require(m->stress_fn != nullptr,
             "a user material needs a computeStress callback (akx_model_set_stress_callback) "
             "or a device law (akx_model_set_device_stress)");
     const int64_t nq = mat.n * m->nqp;
@@ -1341,15 +1349,6 @@ void user_stress_phase(akx_model *m) {
                            dim3((unsigned)A.nblocks_padded), dim3(TPB), 0, m->stream, A);
     }
     HIPCHECK(hipGetLastError());
-    if (mat.dlaw) { // the law on the device: no host round trip
-      const double *gp = mat.grad.p, *pp = mat.dlaw->params.p;
-      double *sp = mat.stress.p;
-      long long nql = nq;
-      void *args[] = {&gp, &sp, &nql, &pp};
-      HIPCHECK(hipModuleLaunchKernel(mat.dlaw->kernel, (unsigned)grid_for(nq), 1, 1, TPB, 1, 1,
-                                     0, m->stream, args, nullptr));
-      continue;
-    }
     m->user_grad.resize(9 * nq);
     m->user_stress.assign(9 * nq, 0.);
     download_soa(m, mat.grad.p, m->user_grad.data(), nq, 9);
@@ -1362,40 +1361,57 @@ void user_stress_phase(akx_model *m) {
   }
 }
 
-// The kernel around a device law: one thread per Gauss point; grad u and the
-// stress are [9][nq] on the device (column-major 3x3 per point)
-const char *k_user_stress_source = R"(
-extern "C" __global__ void __launch_bounds__(256)
-akx_user_stress_kernel(const double *grad, double *stress, long long nq,
-                       const double *params) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq)
-    return;
-  double g[9], s[9];
-  for (int c = 0; c < 9; ++c) {
-    g[c] = grad[c * nq + q];
-    s[c] = 0.;
-  }
-  akx_compute_stress(g, s, params);
-  for (int c = 0; c < 9; ++c)
-    stress[c * nq + q] = s[c];
-}
-)";
+#include "akx_rtc_sources.inc"
 
-// Compiles the caller's law with hiprtc for the model's device, with the
-// plain path's floating-point rules (no contraction: every multiply and add
-// rounds on its own), and loads it.
-std::unique_ptr<DeviceLaw> compile_device_law(akx_model *m, const char *source,
-                                              const double *params, int nparams) {
+// Compiles the caller's law into the material's element pass with hiprtc for
+// the model's GPU -- the library's own kernel headers, the plain path's
+// floating-point rules (-O3, no contraction: every multiply and add rounds on
+// its own) -- for the pass without and with the eigen-gradient terms, and
+// loads it.  The parameters become constants of the program (hexadecimal
+// literals: exact).
+std::unique_ptr<DeviceLaw> compile_device_law(akx_model *m, const Material &mat,
+                                              const char *source, const double *params,
+                                              int nparams) {
   hipDeviceProp_t prop;
   HIPCHECK(hipGetDeviceProperties(&prop, m->device));
   std::string arch(prop.gcnArchName);
   arch = "--offload-arch=" + arch.substr(0, arch.find(':'));
-  const std::string src = std::string(source) + "\n" + k_user_stress_source;
+  const int law = mat.law == AKX_LAW_USER_FD ? LAW_DEVICE_FD : LAW_DEVICE;
+  std::string src =
+      "typedef __hip_internal::int8_t int8_t;\n"
+      "typedef __hip_internal::uint8_t uint8_t;\n"
+      "typedef __hip_internal::int16_t int16_t;\n"
+      "typedef __hip_internal::uint16_t uint16_t;\n"
+      "typedef __hip_internal::int32_t int32_t;\n"
+      "typedef __hip_internal::uint32_t uint32_t;\n"
+      "typedef __hip_internal::int64_t int64_t;\n"
+      "typedef __hip_internal::uint64_t uint64_t;\n"
+      "#define AKX_DEVICE_LAW 1\n";
+  src += "#line 1 \"device law\"\n";
+  src += source;
+  src += "\n#line 1 \"akx device law program\"\n";
+  src += "__constant__ double akx_law_params[" + std::to_string(std::max(nparams, 1)) + "] = {";
+  for (int k = 0; k < std::max(nparams, 1); ++k) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%a", nparams > 0 ? params[k] : 0.);
+    src += (k ? ", " : "") + std::string(buf);
+  }
+  src += "};\n#include \"akx_kernels.hpp\"\nnamespace akx {\n__constant__ Hex8Tables c_hex8;\n}\n";
+  std::string names[2];
+  for (int ext = 0; ext < 2; ++ext)
+    names[ext] = m->etype == AKX_HEX8
+                     ? "akx::k_hex8_strict<" + std::to_string(law) + ", " +
+                           (ext ? "true" : "false") + ">"
+                     : "akx::k_element_forces<0, " + std::to_string(law) + ", double, double, " +
+                           (ext ? "true" : "false") + ">";
+  const std::string table = "&akx::c_hex8";
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "akx_device_law.hip", 0, nullptr, nullptr) !=
-      HIPRTC_SUCCESS)
+  if (hiprtcCreateProgram(&prog, src.c_str(), "akx_device_law.hip", 3, k_rtc_headers,
+                          k_rtc_header_names) != HIPRTC_SUCCESS)
     throw Error(AKX_ERR_HIP, "hiprtcCreateProgram failed");
+  for (auto &nm : names)
+    (void)hiprtcAddNameExpression(prog, nm.c_str());
+  (void)hiprtcAddNameExpression(prog, table.c_str());
   const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off",
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
   const hiprtcResult rc = hiprtcCompileProgram(prog, 8, opts);
@@ -1410,6 +1426,17 @@ std::unique_ptr<DeviceLaw> compile_device_law(akx_model *m, const char *source,
     throw Error(AKX_ERR_INVALID, std::string("the device law did not compile (") +
                                      hiprtcGetErrorString(rc) + "):\n" + log);
   }
+  std::string lowered[3];
+  for (int k = 0; k < 3; ++k) {
+    const char *low = nullptr;
+    if (hiprtcGetLoweredName(prog, k < 2 ? names[k].c_str() : table.c_str(), &low) !=
+            HIPRTC_SUCCESS ||
+        !low) {
+      (void)hiprtcDestroyProgram(&prog);
+      throw Error(AKX_ERR_HIP, "the device law program lost an instantiation");
+    }
+    lowered[k] = low;
+  }
   size_t size = 0;
   std::vector<char> code;
   if (hiprtcGetCodeSize(prog, &size) == HIPRTC_SUCCESS && size > 0) {
@@ -1419,13 +1446,29 @@ std::unique_ptr<DeviceLaw> compile_device_law(akx_model *m, const char *source,
   }
   (void)hiprtcDestroyProgram(&prog);
   require(!code.empty(), "the device law compiled to no code");
-  auto law = std::make_unique<DeviceLaw>();
-  HIPCHECK(hipModuleLoadData(&law->module, code.data()));
-  HIPCHECK(hipModuleGetFunction(&law->kernel, law->module, "akx_user_stress_kernel"));
-  law->params.alloc(std::max(nparams, 1));
-  if (nparams > 0)
-    HIPCHECK(hipMemcpy(law->params.p, params, nparams * sizeof(double), hipMemcpyHostToDevice));
-  return law;
+  auto dl = std::make_unique<DeviceLaw>();
+  HIPCHECK(hipModuleLoadData(&dl->module, code.data()));
+  for (int ext = 0; ext < 2; ++ext)
+    HIPCHECK(hipModuleGetFunction(&dl->kernel[ext], dl->module, lowered[ext].c_str()));
+  hipDeviceptr_t tab = nullptr;
+  size_t tab_bytes = 0;
+  HIPCHECK(hipModuleGetGlobal(&tab, &tab_bytes, dl->module, lowered[2].c_str()));
+  Hex8Tables t;
+  fill_hex8_tables(t);
+  require(tab_bytes == sizeof(t), "the device law program's Hex8 table has another size");
+  HIPCHECK(hipMemcpy((void *)tab, &t, sizeof(t), hipMemcpyHostToDevice));
+  return dl;
+}
+
+// The element pass of a material whose law was compiled in
+void launch_device_law(akx_model *m, Material &mat, ElemArgs A, bool materialize) {
+  A.grad_u = materialize ? mat.grad.p : nullptr; // grad u and the stress, when asked
+  A.stress = mat.stress.p;
+  void *args[] = {&A};
+  const unsigned grid = (unsigned)A.nblocks_padded;
+  const unsigned block = m->etype == AKX_HEX8 ? 64 : TPB;
+  HIPCHECK(hipModuleLaunchKernel(mat.dlaw->kernel[mat.ext() ? 1 : 0], grid, 1, 1, block, 1, 1,
+                                 0, m->stream, args, nullptr));
 }
 
 void flip_plastic_state(akx_model *m) {
@@ -1537,7 +1580,7 @@ void plan_pipeline(akx_model *m) {
   m->pipe_planned = true;
   m->pipe_ok = false;
   const int P = m->pipe_parts;
-  if (P < 2 || m->mats.size() != 1 || m->bpa || m->mats[0]->ext())
+  if (P < 2 || m->mats.size() != 1 || m->bpa || m->mats[0]->ext() || m->mats[0]->dlaw)
     return;
   Material &mat = *m->mats[0];
   if (!mat.contiguous || mat.law == AKX_LAW_PLASTIC_LIH || mat.first != 0 ||
@@ -2597,9 +2640,10 @@ int akx_model_set_device_stress(akx_model *m, int material_id, const char *sourc
     require(nparams >= 0 && nparams <= 4096 && (nparams == 0 || params != nullptr),
             "device law parameters: 0 to 4096 values");
     m->set_device();
+    require(m->etype >= 0, "set the mesh first");
     std::unique_ptr<DeviceLaw> law;
     if (source != nullptr)
-      law = compile_device_law(m, source, params, nparams);
+      law = compile_device_law(m, mat, source, params, nparams);
     HIPCHECK(hipStreamSynchronize(m->stream)); // the law it replaces may be running
     mat.dlaw = std::move(law);
   });

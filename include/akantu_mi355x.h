@@ -178,13 +178,15 @@ int akx_model_set_stress_callback(akx_model *m, akx_compute_stress_fn fn, void *
  *                                      const double *params);
  * for one Gauss point: grad_u and stress are 3x3, column-major as for the
  * callback (stress is zero on entry), params the nparams values given here
- * (copied).  The source is compiled for the model's GPU at this call
- * (hiprtc, -O3, no floating-point contraction: every multiply and add
- * rounds on its own, as on the plain path); a compile error returns
- * AKX_ERR_INVALID with the compiler's log in akx_last_error.  From then on
- * the material's stresses are computed on the device, the callback is not
- * called for it.  Calling again replaces the law; source NULL returns the
- * material to the callback. */
+ * (copied).  At this call the source is compiled into the material's
+ * element pass for the model's GPU (hiprtc, the library's own kernel
+ * headers, -O3, no floating-point contraction: every multiply and add
+ * rounds on its own, as on the plain path): the law runs per Gauss point
+ * inside the pass, between grad u and the force terms.  A compile error
+ * returns AKX_ERR_INVALID with the compiler's log in akx_last_error.  From
+ * then on the callback is not called for the material.  Calling again
+ * replaces the law; source NULL returns the material to the callback.  The
+ * pipelined schedule does not apply to a model with a device law. */
 int akx_model_set_device_stress(akx_model *m, int material_id, const char *source,
                                 const double *params, int nparams);
 
