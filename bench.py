@@ -59,6 +59,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_hex8_neohookean_f64.json"
 FLOPS_FILE = os.path.join(ROOT, "profiles", "flops_hex8_neohookean_200x100x100.json")
 PL = "plastic_linear_isotropic_hardening"
 TIMING_STEPS = 100  # steps of the event-timed per-kernel pass
+RAMP_STEPS = 40     # untimed steps first: the GPU clocks rise after the idle set-up
 
 
 def build_c2(shape=SHAPE):
@@ -151,11 +152,16 @@ def run_plain(args):
     schedule = "plain"
     if args.pipeline > 1 and model.setPipeline(args.pipeline):
         schedule = f"pipelined, {args.pipeline} parts"
-    # per-kernel times first: a separate pass of TIMING_STEPS steps with HIP
-    # events around every launch (plain schedule), so the timed loop below
-    # carries no event overhead.  Running it first also brings the GPU clocks
-    # to their steady state: after the idle set-up, the element passes of the
-    # first ~20 steps run up to 25 % slower while the clocks ramp.
+    # The copy calibration first: run between the kernel-timing pass and the
+    # timed steps, it left the element passes of the timed steps 7 % slower.
+    copy_gbs = copy_bandwidth()
+    # After the idle set-up the element passes of the first ~20 steps run up
+    # to 25 % slower while the GPU clocks ramp: RAMP_STEPS untimed steps, then
+    # the per-kernel times, a separate pass of TIMING_STEPS steps with HIP
+    # events around every launch (plain schedule), so that the timed loop
+    # below carries no event overhead.
+    model.solveSteps(RAMP_STEPS)
+    model.synchronize()
     model.enableKernelTiming(True)
     model.solveSteps(TIMING_STEPS)
     model.synchronize()
@@ -166,7 +172,6 @@ def run_plain(args):
         if n:
             kern[name] = {"avg_ms": ms / n, "launches": n, "bytes": model.kernelBytes(k)}
     model.enableKernelTiming(False)
-    copy_gbs = copy_bandwidth()
     model.solveSteps(args.warmup)
     model.synchronize()
     t0 = time.perf_counter()
@@ -264,7 +269,7 @@ def run_plain(args):
         print("# forms: FP32 form, 512-instance batch, serving stream", file=sys.stderr,
               flush=True)
         out["forms"] = run_forms(args, model, mesh, blocked, v0, dt, value,
-                                 TIMING_STEPS + args.warmup + steps)
+                                 RAMP_STEPS + TIMING_STEPS + args.warmup + steps)
         out["north_star"] = north_star_summary(out)
     return out
 
@@ -301,6 +306,8 @@ def run_forms(args, plain, mesh, blocked, v0, dt, plain_value, plain_steps):
     m.getBlockedDOFs()[:] = blocked
     m.getVelocity()[:] = v0
     m.setTimeStep(dt)
+    m.solveSteps(RAMP_STEPS)
+    m.synchronize()
     m.enableKernelTiming(True)
     m.solveSteps(TIMING_STEPS)
     m.synchronize()
@@ -312,13 +319,17 @@ def run_forms(args, plain, mesh, blocked, v0, dt, plain_value, plain_steps):
     m.enableKernelTiming(False)
     m.solveSteps(args.warmup + steps)
     m.synchronize()
-    assert TIMING_STEPS + args.warmup + steps == plain_steps
+    assert RAMP_STEPS + TIMING_STEPS + args.warmup + steps == plain_steps
     err = {"u": rel_errors(m.getDisplacement(), plain.getDisplacement()),
            "v": rel_errors(m.getVelocity(), plain.getVelocity()),
            "f_int": rel_errors(m.getInternalForce(), plain.getInternalForce())}
-    # the form's throughput over at least 200 steps: a 20-step window of a
-    # 0.25 ms step is a few milliseconds of wall clock, within the noise
+    # the form's throughput over at least 200 steps (a 20-step window of a
+    # 0.25 ms step is a few milliseconds of wall clock, within the noise),
+    # after a few untimed steps: the first call after the getters above
+    # releases their host copies
     timed = max(steps, 200)
+    m.solveSteps(max(args.warmup, 1))
+    m.synchronize()
     t0 = time.perf_counter()
     m.solveSteps(timed)
     m.synchronize()
