@@ -180,6 +180,32 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
         G.fint[o] = f[c];
       if ((blk >> c) & 1)
         continue;
+      if constexpr (COMP) {
+        // FP32 form: between two steps of one call the velocity array holds
+        // the half-step velocity w = v + dt/2 a (with its compensation), so
+        // the acceleration array is neither read nor written until the
+        // call's last step.  a = f / M is a + (f - M a) / M of the reference
+        // (zero external force) without the cancelling terms.  The
+        // velocity v = w + dt/2 a and the next w = v + dt/2 a are the two
+        // compensated additions that a call boundary performs too (here,
+        // then in the predictor), so splitting a run of steps over calls
+        // keeps every bit; u += dt w is the predictor's u + dt v + dt^2/2 a.
+        const double a = f[c] / M;
+        double w = G.v[o];
+        float cw = G.cv[o];
+        kahan_add(w, cw, dd * a); // v at the end of the step
+        if constexpr (PREDICT) {
+          float cu = G.cu[o];
+          kahan_add(w, cw, dd * a);
+          kahan_add(us[c], cu, dt * w);
+          G.cu[o] = cu;
+        } else {
+          G.a[o] = a;
+        }
+        G.v[o] = w;
+        G.cv[o] = cw;
+        continue;
+      }
       double a = G.a[o], v = G.v[o];
       double r = 0.;
       r += 1. * 0.; // zero external force
@@ -189,24 +215,11 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
       r += M * cache;
       const double x = 1. * (r / M);
       a += 1. * x;
-      if constexpr (COMP) {
-        float cv = G.cv[o];
-        kahan_add(v, cv, dd * x);
-        if constexpr (PREDICT) {
-          float cu = G.cu[o];
-          const double dt_a = dt * a;
-          kahan_add(us[c], cu, dt * v + (.5 * dt) * dt_a);
-          kahan_add(v, cv, dt_a);
-          G.cu[o] = cu;
-        }
-        G.cv[o] = cv;
-      } else {
-        v += dd * x;
-        if constexpr (PREDICT) {
-          const double dt_a = dt * a;
-          us[c] = us[c] + (dt * v + (.5 * dt) * dt_a);
-          v = v + dt_a;
-        }
+      v += dd * x;
+      if constexpr (PREDICT) {
+        const double dt_a = dt * a;
+        us[c] = us[c] + (dt * v + (.5 * dt) * dt_a);
+        v = v + dt_a;
       }
       G.a[o] = a;
       G.v[o] = v;
@@ -258,9 +271,11 @@ __global__ void __launch_bounds__(256) k_batch_predictor(BatchGatherArgs G) {
       double v = G.v[o], u = us[c];
       const double dt_a = dt * a;
       if constexpr (COMP) {
+        // the velocity array takes the half-step velocity w = v + dt/2 a,
+        // which the gathers of this call advance (k_batch_gather)
         float cu = G.cu[o], cv = G.cv[o];
-        kahan_add(u, cu, dt * v + (.5 * dt) * dt_a);
-        kahan_add(v, cv, dt_a);
+        kahan_add(v, cv, (.5 * dt) * a);
+        kahan_add(u, cu, dt * v);
         G.cu[o] = cu;
         G.cv[o] = cv;
       } else {
@@ -568,7 +583,7 @@ struct akx_batch {
   int64_t ncap = 0, tcap = 0, hexcap = 0, ellcap = 0;
   // arenas
   DevBuf<double> xu; // [ncap][6] node records {X, u}
-  DevBuf<double> bx, bu; // [3][ncap] X and u (FP32 batches with Tet4 capacity)
+  DevBuf<double> bx, bu; // [3][ncap] X and u (FP32 batches)
   DevBuf<double> v, a, fint, mass;
   DevBuf<uint8_t> blocked, deg;
   DevBuf<float> cu, cv;
@@ -948,7 +963,9 @@ int64_t footprint_bytes(int precision, int etype, int law, int64_t nn, int64_t n
   const bool f32 = precision == AKX_PRECISION_FP32;
   const double sf = f32 ? 4 : 8;
   const int64_t nn_pad = (nn + 255) / 256 * 256;
-  double bytes = (double)nn_pad * (5 * 3 * 8 + 8 + 1 + 1 + (f32 ? 2 * 3 * 4 : 0)) +
+  // node records, v, a, f_int, mass, blocked bits, degree; FP32: the X and
+  // u arrays and the two compensation terms
+  double bytes = (double)nn_pad * (5 * 3 * 8 + 8 + 1 + 1 + (f32 ? 2 * 3 * 8 + 2 * 3 * 4 : 0)) +
                  (double)nn_pad / 256 * 8;
   bytes += (double)nel * nnpe * (4 + 4 + 1.25 * FE_W * sf);
   if (f32 && etype == AKX_TET4) // block partials: corners and slots, and at most
@@ -1042,9 +1059,9 @@ int akx_batch_create(int device, int precision, int64_t node_capacity,
       if (b->tcap) {
         b->lc_t.alloc(4 * b->tcap);
         b->ls_t.alloc(4 * b->tcap);
-        b->bx.alloc(3 * b->ncap);
-        b->bu.alloc(3 * b->ncap);
       }
+      b->bx.alloc(3 * b->ncap);
+      b->bu.alloc(3 * b->ncap);
       if (b->hexcap)
         b->ls_h.alloc(8 * b->hexcap);
       b->bnode.alloc(b->bcap);
@@ -1122,8 +1139,8 @@ int akx_batch_add_instance(akx_batch *b, int element_type, int law, const double
     HostInst I;
     I.etype = element_type;
     I.law = law;
-    I.urec = !(b->fp32() && element_type == AKX_TET4);
-    require(I.urec || (b->bx.p && b->bu.p), "FP32 Tet4 instance without its X and u arrays");
+    I.urec = !b->fp32();
+    require(I.urec || (b->bx.p && b->bu.p), "FP32 instance without its X and u arrays");
     I.rho = params[0];
     lame(params[1], params[2], I.lambda, I.mu);
     I.sy = params[3];
@@ -1698,18 +1715,21 @@ int akx_batch_get_kernel_bytes(akx_batch *b, int kernel_class, double *bytes) {
       const bool bpa = !I.blk_cnt.empty();
       const double entries = bpa ? (double)I.nbent : corners;
       if (kernel_class == 0) {
-        if (bpa) // local corners (Tet4) + slots (2 B each), per entry node id
+        if (bpa) // per corner the local corner (Tet4, 2 B) or the node id
+                 // (Hex8, 4 B) and the slot (2 B); per entry the node id
                  // (Tet4), slot end and table position, the partial written
-          total += corners * (I.etype == AKX_TET4 ? 4 : 2) +
+          total += corners * (I.etype == AKX_TET4 ? 4 : 6) +
                    entries * ((I.etype == AKX_TET4 ? 4 : 0) + 2 + 4 + 3 * fs) + nn * 2 * 3 * 8;
         else
           total += corners * (4 + 4 + 3 * fs) + nn * 2 * 3 * 8;
         if (I.law == AKX_LAW_PLASTIC_LIH) // history read and written
           total += (double)I.nel * nqp * 2 * (b->fp32() ? 7 : 28) * hs;
       } else {
-        // a step followed by another (f_int is written by the last one only)
-        total += entries * 3 * fs + nn * (1 + 1 + 8 + 3 * 3 * 2 * 8 +
-                                          (b->fp32() ? 2 * 3 * 2 * 4 : 0));
+        // a step followed by another (f_int is written by the last one
+        // only): u, v and a read and written; FP32: u, the half-step
+        // velocity and their compensation terms
+        total += entries * 3 * fs + nn * (1 + 1 + 8 + (b->fp32() ? 2 * 3 * 2 * (8 + 4)
+                                                                  : 3 * 3 * 2 * 8));
       }
     }
     *bytes = total;

@@ -23,9 +23,10 @@ struct BInst { // one per instance id
   int32_t parity_base; // steps done before the current solve call, mod 2
   int32_t law;
   // where the instance's displacement lives: 1 in the {X, u} node records
-  // (the passes that read corners from them), 0 in the [3][ncap] array u
-  // (FP32 Tet4: its block pass builds the records in LDS from X and u, and
-  // the gathers then store u whole instead of half-records)
+  // (FP64: its passes read corners from them), 0 in the [3][ncap] array u
+  // (FP32: the Tet4 block pass builds the records in LDS from X and u, the
+  // Hex8 pass reads its corners from them, and the gathers store u whole
+  // instead of half-records)
   int32_t urec;
   int32_t pad;
 };

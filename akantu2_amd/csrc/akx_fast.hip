@@ -835,8 +835,8 @@ __global__ void __launch_bounds__(EPB * 8) k_batch_hex8_f32(BatchElemArgs A) {
     for (int base = 0; base < T.count; base += EPB) {
       const bool active = base + el < T.count;
       const int64_t e = T.first + (active ? base + el : 0);
-      hex8_group_sf<LAW, float, QpCompact, NodesXU, FE_W>(
-          q, active, A.conn + e * 8, NodesXU{A.xu}, float(P.lambda),
+      hex8_group_sf<LAW, float, QpCompact, NodesSoA, FE_W>(
+          q, active, A.conn + e * 8, NodesSoA{A.X, A.u, A.ncap}, float(P.lambda),
           float(P.mu), float(P.sigma_y), float(P.h), V, A.hoff + e * 8 + q,
           A.pos + e * 8, reinterpret_cast<float *>(A.fe));
     }
@@ -864,8 +864,8 @@ __global__ void __launch_bounds__(256) k_batch_hex8_bpa(BatchElemArgs A) {
   for (int base = 0; base < T.count; base += 32) {
     const bool active = base + el < T.count;
     const int64_t e = T.first + (active ? base + el : 0);
-    hex8_group_sf<LAW, float, QpCompact, NodesXU, FE_W, true>(
-        q, active, A.conn + e * 8, NodesXU{A.xu}, float(P.lambda), float(P.mu),
+    hex8_group_sf<LAW, float, QpCompact, NodesSoA, FE_W, true>(
+        q, active, A.conn + e * 8, NodesSoA{A.X, A.u, A.ncap}, float(P.lambda), float(P.mu),
         float(P.sigma_y), float(P.h), V, A.hoff + e * 8 + q, nullptr, nullptr, s_f,
         A.ls + e * 8);
   }

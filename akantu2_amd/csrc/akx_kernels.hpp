@@ -988,6 +988,35 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
       continue;
     const int64_t o = (int64_t)c * A.nn + n;
     double fx = A.fext ? A.fext[o] : 0.;
+    if constexpr (COMP) {
+      // FP32 form: between two steps of one call the velocity array holds
+      // the half-step velocity w = v + dt/2 a (with its compensation), so
+      // the acceleration array is neither read nor written until the
+      // call's last step.  a = (f_ext + f_int) / M is the reference's
+      // a + (f_ext + f_int - M a) / M without the cancelling terms.  The
+      // velocity v = w + dt/2 a and the next w = v + dt/2 a are the two
+      // compensated additions that a call boundary performs too (here, then
+      // in k_predictor), so splitting a run of steps over calls keeps every
+      // bit; u += dt w is the predictor's u + dt v + dt^2/2 a.
+      const double a = (fx + f[c]) / M;
+      double w = A.v[o];
+      float cw = A.cv[o];
+      kahan_add(w, cw, dd * a); // v at the end of the step
+      if constexpr (PREDICT) {
+        double u = A.u[o];
+        float cu = A.cu[o];
+        kahan_add(w, cw, dd * a);
+        kahan_add(u, cu, A.dt * w);
+        A.u[o] = u;
+        un[c] = u;
+        A.cu[o] = cu;
+      } else {
+        A.a[o] = a;
+      }
+      A.v[o] = w;
+      A.cv[o] = cw;
+      continue;
+    }
     double a = A.a[o], v = A.v[o];
     double r = 0.;
     r += 1. * fx;
@@ -997,29 +1026,13 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
     r += M * cache;
     const double x = 1. * (r / M);
     a += 1. * x;
-    if constexpr (COMP) {
-      float cv = A.cv[o];
-      kahan_add(v, cv, dd * x);
-      if constexpr (PREDICT) {
-        double u = A.u[o];
-        float cu = A.cu[o];
-        const double dt_a = A.dt * a;
-        kahan_add(u, cu, A.dt * v + (.5 * A.dt) * dt_a);
-        kahan_add(v, cv, dt_a);
-        A.u[o] = u;
-        un[c] = u;
-        A.cu[o] = cu;
-      }
-      A.cv[o] = cv;
-    } else {
-      v += dd * x;
-      if constexpr (PREDICT) {
-        const double dt_a = A.dt * a;
-        const double u = A.u[o] + (A.dt * v + (.5 * A.dt) * dt_a);
-        A.u[o] = u;
-        un[c] = u;
-        v = v + dt_a;
-      }
+    v += dd * x;
+    if constexpr (PREDICT) {
+      const double dt_a = A.dt * a;
+      const double u = A.u[o] + (A.dt * v + (.5 * A.dt) * dt_a);
+      A.u[o] = u;
+      un[c] = u;
+      v = v + dt_a;
     }
     A.a[o] = a;
     A.v[o] = v;
@@ -1054,9 +1067,11 @@ __global__ void __launch_bounds__(256) k_predictor(GatherArgs A) {
     double v = A.v[o], u = A.u[o];
     const double dt_a = A.dt * a;
     if constexpr (COMP) {
+      // the velocity array takes the half-step velocity w = v + dt/2 a,
+      // which the gathers of this call advance (k_gather_update)
       float cu = A.cu[o], cv = A.cv[o];
-      kahan_add(u, cu, A.dt * v + (.5 * A.dt) * dt_a);
-      kahan_add(v, cv, dt_a);
+      kahan_add(v, cv, (.5 * A.dt) * a);
+      kahan_add(u, cu, A.dt * v);
       A.cu[o] = cu;
       A.cv[o] = cv;
     } else {

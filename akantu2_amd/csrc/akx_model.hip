@@ -3403,11 +3403,15 @@ int akx_model_get_kernel_bytes(akx_model *m, int kernel_class, double *bytes) {
       break;
     case 1: { // gather + update of a step followed by another: degree, r_e
               // read, mass, blocked, a/v read+write, u read+write (fused
-              // predictor); the last step of a call also writes f_int
+              // predictor); the last step of a call also writes f_int.  The
+              // FP32 form reads and writes no a between the steps of a call
+              // (its velocity array holds the half-step velocity), and its
+              // compensation terms
+      const bool f32 = m->precision == AKX_PRECISION_FP32;
       double per_node = 4.0 + s_node /*M*/ + 1.0 + (m->records_live() ? 3 * s_node : 0.) /*u records*/ +
-                        2 * 3 * s_node /*a*/ + 2 * 3 * s_node /*v*/ +
+                        (f32 ? 0. : 2 * 3 * s_node) /*a*/ + 2 * 3 * s_node /*v*/ +
                         2 * 3 * s_node /*u*/ + (m->has_fext ? 3 * s_node : 0.);
-      if (m->precision == AKX_PRECISION_FP32)
+      if (f32)
         per_node += 2 * 2 * 3 * 4.0; // compensation terms read + write
       *bytes = entries * (3.0 * s_f) + m->nn * (per_node - 3.0);
       break;

@@ -5,7 +5,8 @@
   element types, all four laws, odd step splits (history ping-pong parity)
   and graph / no-graph launches.
 - An instance gives the same bits alone and in the mix (FP64 and FP32), and
-  after other instances were removed or admitted around it.
+  after other instances were removed or admitted around it; a run of steps
+  split over calls gives the bits of one call (FP64 and FP32).
 - FP32 batch (the precision form): <= 1e-5 relative displacement error
   against the FP64 oracle on the plasticity workload of the batching form.
 - The autotuner leaves the simulation state untouched.
@@ -99,6 +100,26 @@ def test_instance_same_alone_and_in_mix(precision):
         for f in a:
             assert np.array_equal(a[f], m[f]), (k, f)
         alone.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_step_split_over_calls_keeps_bits(precision):
+    """A run of steps split over several calls gives the bits of one call.
+    The FP32 form keeps the half-step velocity between the steps of a call
+    and the velocity between calls: the call boundary performs the same two
+    compensated additions as a step inside a call."""
+    one, ids1 = _batch(precision, SPECS)
+    one.solveStep(44)
+    split, ids2 = _batch(precision, SPECS)
+    for n in (37, 1, 2, 1, 3):
+        split.solveStep(n)
+    for k in range(len(SPECS)):
+        a, m = _fields(one, ids1[k]), _fields(split, ids2[k])
+        for f in a:
+            assert np.array_equal(a[f], m[f]), (k, f)
+    assert np.abs(a["acceleration"]).max() > 0
+    one.close()
+    split.close()
 
 
 def test_admission_and_retirement_leave_residents_unchanged():

@@ -196,3 +196,29 @@ def test_fp32_hex8_block_assembly(law, amp, mat):
         assert tr <= 1e-5
         for get in ("getDisplacement", "getVelocity", "getInternalForce"):
             assert np.array_equal(getattr(bpa, get)(), getattr(again, get)()), (get, n)
+
+
+@pytest.mark.parametrize("etype,law", [("_tetrahedron_4", "elastic"),
+                                       ("_hexahedron_8", "neohookean")])
+def test_fp32_steps_split_over_calls_keep_bits(etype, law):
+    """The FP32 form keeps the half-step velocity w = v + dt/2 a between the
+    steps of one call and (v, a) between calls; the call boundary performs
+    the same two compensated additions as a step inside a call, so a run of
+    steps split over calls gives the bits of one call (with an external
+    force, and blocked DOFs)."""
+    mesh, blocked, v0 = workload(etype, (10, 4, 3), jitter=0.1, seed=21)
+    fext = 1e-5 * np.random.default_rng(21).uniform(-1, 1, (mesh.nb_nodes, 3))
+    one = _fp32_model(mesh, blocked, v0, True, law=law)
+    split = _fp32_model(mesh, blocked, v0, True, dt=one.getTimeStep(), law=law)
+    for g in (one, split):
+        g.getExternalForce()[:] = fext
+    one.solveSteps(47)
+    for n in (30, 1, 2, 1, 13):
+        split.solveSteps(n)
+    for get in ("getDisplacement", "getVelocity", "getAcceleration", "getInternalForce"):
+        assert np.array_equal(getattr(one, get)(), getattr(split, get)()), get
+    # blocked components keep their velocity and acceleration
+    fixed = np.asarray(blocked, dtype=bool)
+    assert fixed.shape == (mesh.nb_nodes, 3) and fixed.any()
+    assert np.all(one.getVelocity()[fixed] == np.asarray(v0)[fixed])
+    assert np.all(one.getAcceleration()[fixed] == 0.0)
