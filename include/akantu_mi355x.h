@@ -75,7 +75,11 @@ extern "C" {
                                 reproduced bit for bit except the last bit
                                 of log() */
 #define AKX_PRECISION_FP32 1 /* grad u, stress, B^T sigma in FP32; nodal
-                                update in FP64 with compensated summation */
+                                update in FP64 with compensated summation.
+                                Between the steps of one solve call the
+                                velocity field holds v + dt/2 a and the
+                                acceleration field is not updated; both are
+                                v and a again when the call returns */
 #define AKX_PRECISION_FP64_FAST 2 /* FP64 with FMA and the regrouped
                                 contraction r_e = sum_q Q_q dN/dxi_q; equal to
                                 the plain path to rounding per evaluation */
