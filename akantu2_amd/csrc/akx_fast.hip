@@ -697,6 +697,17 @@ __global__ void __launch_bounds__(256) k_tet4_bpa(ElemArgs A) {
     ls = *reinterpret_cast<const ushort4 *>(A.lslot + i * 4);
     element_params(A, i, lambda, mu, sy, hh);
   }
+  // this lane's first distinct node of the summing phase, read ahead so
+  // that its loads overlap the staging instead of following the barrier
+  const uint16_t *soff = A.bl_soff + blk * (A.bl_maxu + 1);
+  const int32_t *dst = A.bl_pos + blk * A.bl_maxu;
+  int32_t d_first = -1;
+  int s0_first = 0, s1_first = 0;
+  if ((int)threadIdx.x < A.bl_maxu) {
+    d_first = dst[threadIdx.x];
+    s0_first = soff[threadIdx.x];
+    s1_first = soff[threadIdx.x + 1];
+  }
   akx::stage_block_records(A, blk, s_xu);
   __syncthreads();
   float *s_f = reinterpret_cast<float *>(s_xu + 3 * A.bl_maxu);
@@ -715,14 +726,13 @@ __global__ void __launch_bounds__(256) k_tet4_bpa(ElemArgs A) {
     }
   }
   __syncthreads();
-  const uint16_t *soff = A.bl_soff + blk * (A.bl_maxu + 1);
-  const int32_t *dst = A.bl_pos + blk * A.bl_maxu;
   float *fe = reinterpret_cast<float *>(A.fe);
   for (int k = threadIdx.x; k < A.bl_maxu; k += blockDim.x) {
-    const int32_t d = dst[k];
+    const bool first = k == (int)threadIdx.x;
+    const int32_t d = first ? d_first : dst[k];
     if (d < 0)
       continue;
-    const int s0 = soff[k], s1 = soff[k + 1];
+    const int s0 = first ? s0_first : soff[k], s1 = first ? s1_first : soff[k + 1];
     double fx = 0., fy = 0., fz = 0.;
     for (int sl = s0; sl < s1; ++sl) {
       fx += (double)s_f[3 * sl];
@@ -781,8 +791,17 @@ __global__ void __launch_bounds__(256) k_batch_tet4_bpa(BatchElemArgs A) {
     const BBlk B = A.blks[tt];
     const BInst P = A.inst[T.inst];
     const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
+    // this lane's first distinct node of the summing phase, read ahead so
+    // that its loads overlap the staging instead of following the barriers
+    const int t = threadIdx.x;
+    int s0_first = 0, s1_first = 0, pos_first = 0;
+    if (t < B.count) {
+      s0_first = t > 0 ? (int)A.bsend[B.off + t - 1] : 0;
+      s1_first = A.bsend[B.off + t];
+      pos_first = A.bpos[B.off + t];
+    }
     // the records of the block's distinct nodes, built from X and u (FP32
-    // Tet4 instances keep u in the [3][ncap] array, BInst::urec = 0)
+    // instances keep u in the [3][ncap] array, BInst::urec = 0)
     for (int k = threadIdx.x; k < B.count; k += blockDim.x) {
       const int64_t nd = A.bnode[B.off + k];
       s_xu[3 * k] = make_double2(A.X[nd], A.X[A.ncap + nd]);
@@ -809,15 +828,17 @@ __global__ void __launch_bounds__(256) k_batch_tet4_bpa(BatchElemArgs A) {
     }
     __syncthreads();
     for (int k = threadIdx.x; k < B.count; k += blockDim.x) {
-      const int s0 = k > 0 ? (int)A.bsend[B.off + k - 1] : 0;
-      const int s1 = A.bsend[B.off + k];
+      const bool first = k == t;
+      const int s0 = first ? s0_first : (int)A.bsend[B.off + k - 1];
+      const int s1 = first ? s1_first : (int)A.bsend[B.off + k];
+      const int pos = first ? pos_first : A.bpos[B.off + k];
       double fx = 0., fy = 0., fz = 0.;
       for (int sl = s0; sl < s1; ++sl) {
         fx += (double)s_f[3 * sl];
         fy += (double)s_f[3 * sl + 1];
         fz += (double)s_f[3 * sl + 2];
       }
-      akx::put_entry<FE_W>(fe, (int64_t)A.bpos[B.off + k], float(fx), float(fy), float(fz));
+      akx::put_entry<FE_W>(fe, (int64_t)pos, float(fx), float(fy), float(fz));
     }
   }
 }

@@ -660,6 +660,17 @@ __global__ void __launch_bounds__(PPA_TB) k_tet4_ppa(ElemArgs A) {
     sy = A.sy_el[ic];
   if (A.h_el)
     hh = A.h_el[ic];
+  // this lane's first run of the summing phase, read ahead so that its
+  // loads overlap the staging instead of following the barriers
+  const int32_t k0 = A.poff[blk], k1 = A.poff[blk + 1];
+  const int32_t k_first = k0 + (int32_t)threadIdx.x;
+  int s0_first = 0, s1_first = 0;
+  int32_t pos_first = 0;
+  if (k_first < k1) {
+    s0_first = k_first == k0 ? 0 : A.pend[k_first - 1];
+    s1_first = A.pend[k_first];
+    pos_first = A.ppos[k_first];
+  }
   stage_block_records(A, blk, s_rec);
   __syncthreads();
   NodesRegs4 rec;
@@ -686,17 +697,18 @@ __global__ void __launch_bounds__(PPA_TB) k_tet4_ppa(ElemArgs A) {
         i, reinterpret_cast<const int32_t *>(&pv), A.fe, s_slot);
   }
   __syncthreads();
-  const int32_t k0 = A.poff[blk], k1 = A.poff[blk + 1];
   double *fe = reinterpret_cast<double *>(A.fe);
-  for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += blockDim.x) {
-    const int s0 = k == k0 ? 0 : A.pend[k - 1], s1 = A.pend[k];
+  for (int32_t k = k_first; k < k1; k += blockDim.x) {
+    const bool first = k == k_first;
+    const int s0 = first ? s0_first : A.pend[k - 1], s1 = first ? s1_first : A.pend[k];
+    const int32_t pos = first ? pos_first : A.ppos[k];
     double f0 = 0., f1 = 0., f2 = 0.;
     for (int sl = s0; sl < s1; ++sl) {
       f0 = f0 + s_slot[3 * sl];
       f1 = f1 + s_slot[3 * sl + 1];
       f2 = f2 + s_slot[3 * sl + 2];
     }
-    put_entry<3>(fe, (int64_t)A.ppos[k], f0, f1, f2);
+    put_entry<3>(fe, (int64_t)pos, f0, f1, f2);
   }
 }
 
