@@ -218,7 +218,7 @@ def run_plain(args):
         tflops = flops / (el["avg_ms"] * 1e-3) / 1e12
         roofline = {"bound": "fp32_valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "k_hex8_sf<neo-Hookean> (FP32, sum-factorised)",
+                    "kernel": "k_hex8_bpa<neo-Hookean> (FP32, sum-factorised, block partials)",
                     "compute": "FP32 VALU; packed FP32 instructions are not used, so the "
                                f"ceiling is {FP32_VALU_TFLOPS} TFLOP/s "
                                f"({tflops / FP32_VALU_TFLOPS:.0%} of it)",
