@@ -2158,9 +2158,6 @@ int akx_model_init_full(akx_model *m) {
         mp->grad.alloc(9 * mp->n * m->nqp);
         mp->stress.alloc(9 * mp->n * m->nqp);
       }
-    int64_t nslots = 0;
-    for (auto &mp : m->mats)
-      nslots += mp->n * m->nnpe;
     // IntegratorGauss::checkJacobians
     {
       InitArgs A = init_args(m, nullptr);
