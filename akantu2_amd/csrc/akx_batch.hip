@@ -134,10 +134,10 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
     const int deg = G.deg[n];
     for (int d = 0; d < deg; ++d, p += 256 * FE_W) {
       double e0, e1, e2;
-      if constexpr (FE_W == 3) {
-        e0 = p[0];
-        e1 = p[1];
-        e2 = p[2];
+      if constexpr (FE_W == 3) { // read once per step: non-temporal
+        e0 = __builtin_nontemporal_load(p);
+        e1 = __builtin_nontemporal_load(p + 1);
+        e2 = __builtin_nontemporal_load(p + 2);
       } else if constexpr (sizeof(FT) == 4) {
         const float4 v = *reinterpret_cast<const float4 *>(p);
         e0 = v.x;
