@@ -744,9 +744,12 @@ __global__ void __launch_bounds__(256) k_tet4_bpa(ElemArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// Batched FP32 form: persistent over the tiles of one (element type, law)
-// group; per-instance parameters, compact FP32 plastic state (ping-pong by
-// step parity).
+// Batched FP32 form, one tile of one (element type, law) group per
+// workgroup; per-instance parameters, compact FP32 plastic state (ping-pong
+// by step parity), corners read from the X and u arrays.  The per-corner
+// kernels (k_batch_tet4_f32, k_batch_hex8_f32) serve groups without block
+// tables; FP32 batches build block tables for every group, so their tiles
+// run the block-partial kernels below.
 // ---------------------------------------------------------------------------
 template <int LAW>
 __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
@@ -759,8 +762,8 @@ __global__ void __launch_bounds__(256) k_batch_tet4_f32(BatchElemArgs A) {
     const QpCompact V = batch_compact_view(A, (P.parity_base ^ A.parity) & 1);
     for (int j = threadIdx.x; j < T.count; j += blockDim.x) {
       const int64_t e = T.first + j;
-      tet4_one<LAW, float, float, QpCompact, NodesXU, FE_W>(
-          *reinterpret_cast<const int4 *>(A.conn + e * 4), NodesXU{A.xu},
+      tet4_one<LAW, float, float, QpCompact, NodesSoA, FE_W>(
+          *reinterpret_cast<const int4 *>(A.conn + e * 4), NodesSoA{A.X, A.u, A.ncap},
                                          float(P.lambda), float(P.mu), float(P.sigma_y),
                                          float(P.h), V, A.hoff + e, A.pos + e * 4,
                                          reinterpret_cast<float *>(A.fe));
