@@ -156,7 +156,6 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
     }
     const double M = G.mass[n];
     const uint8_t blk = G.blocked[n];
-    const double dd = .5 * dt; // beta * dt
     // u of the node's record (u0 at 48 n + 24 bytes, then an aligned pair)
     // or of the [3][ncap] array, read and written whole; blocked components
     // keep their value
@@ -180,49 +179,24 @@ __global__ void __launch_bounds__(256) k_batch_gather(BatchGatherArgs G) {
         G.fint[o] = f[c];
       if ((blk >> c) & 1)
         continue;
-      if constexpr (COMP) {
-        // FP32 form: between two steps of one call the velocity array holds
-        // the half-step velocity w = v + dt/2 a (with its compensation), so
-        // the acceleration array is neither read nor written until the
-        // call's last step.  a = f / M is a + (f - M a) / M of the reference
-        // (zero external force) without the cancelling terms.  The
-        // velocity v = w + dt/2 a and the next w = v + dt/2 a are the two
-        // compensated additions that a call boundary performs too (here,
-        // then in the predictor), so splitting a run of steps over calls
-        // keeps every bit; u += dt w is the predictor's u + dt v + dt^2/2 a.
-        const double a = f[c] / M;
-        double w = G.v[o];
-        float cw = G.cv[o];
-        kahan_add(w, cw, dd * a); // v at the end of the step
-        if constexpr (PREDICT) {
-          float cu = G.cu[o];
-          kahan_add(w, cw, dd * a);
-          kahan_add(us[c], cu, dt * w);
+      if constexpr (COMP) { // FP32 form: half-step velocity (akx_math.hpp)
+        double w = G.v[o], a;
+        float cw = G.cv[o], cu = 0.f;
+        if constexpr (PREDICT)
+          cu = G.cu[o];
+        half_step_gather<PREDICT>(f[c], M, dt, w, cw, us[c], cu, a);
+        if constexpr (PREDICT)
           G.cu[o] = cu;
-        } else {
+        else
           G.a[o] = a;
-        }
         G.v[o] = w;
         G.cv[o] = cw;
-        continue;
+      } else { // reference order, zero external force (akx_math.hpp)
+        double a = G.a[o], v = G.v[o];
+        reference_gather<PREDICT>(0., f[c], M, dt, a, v, us[c]);
+        G.a[o] = a;
+        G.v[o] = v;
       }
-      double a = G.a[o], v = G.v[o];
-      double r = 0.;
-      r += 1. * 0.; // zero external force
-      r += 1. * f[c];
-      double cache = 0.;
-      cache += -1. * a;
-      r += M * cache;
-      const double x = 1. * (r / M);
-      a += 1. * x;
-      v += dd * x;
-      if constexpr (PREDICT) {
-        const double dt_a = dt * a;
-        us[c] = us[c] + (dt * v + (.5 * dt) * dt_a);
-        v = v + dt_a;
-      }
-      G.a[o] = a;
-      G.v[o] = v;
     }
     if constexpr (PREDICT) {
       if (urec) {
@@ -269,18 +243,13 @@ __global__ void __launch_bounds__(256) k_batch_predictor(BatchGatherArgs G) {
       const int64_t o = (int64_t)c * G.ncap + n;
       const double a = G.a[o];
       double v = G.v[o], u = us[c];
-      const double dt_a = dt * a;
-      if constexpr (COMP) {
-        // the velocity array takes the half-step velocity w = v + dt/2 a,
-        // which the gathers of this call advance (k_batch_gather)
+      if constexpr (COMP) { // the velocity array takes the half-step velocity
         float cu = G.cu[o], cv = G.cv[o];
-        kahan_add(v, cv, (.5 * dt) * a);
-        kahan_add(u, cu, dt * v);
+        half_step_predict(a, dt, v, cv, u, cu);
         G.cu[o] = cu;
         G.cv[o] = cv;
       } else {
-        u = u + (dt * v + (.5 * dt) * dt_a);
-        v = v + dt_a;
+        reference_predict(a, dt, v, u);
       }
       us[c] = u;
       G.v[o] = v;

@@ -930,6 +930,8 @@ __global__ void __launch_bounds__(64) k_hex8_strict(ElemArgs A) {
 //   x      = 1 * (r / M)                  (unblocked DOFs only)
 //   a     += 1 * x ; v += (beta dt) * x    (u += 0 * x is the identity)
 //   [predictor] u += dt v + (.5 dt)(dt a) ; v = v + dt a
+// (reference_gather / reference_predict in akx_math.hpp; the FP32 form runs
+// half_step_gather / half_step_predict there: the half-step velocity).
 // References: dof_manager.cc:65-120 (scatter order), solid_mechanics_model.cc
 // :235-249, integration_scheme_2nd_order.cc:70-93, dof_manager_default.cc
 // :260-278, non_linear_solver_lumped.cc:76-89, newmark-beta.cc:47-77,164-197.
@@ -950,13 +952,6 @@ struct GatherArgs {
   double dt;
   int64_t nblocks_padded;
 };
-
-__device__ __forceinline__ void kahan_add(double &s, float &c, double inc) {
-  double y = inc - double(c);
-  double t = s + y;
-  c = float((t - s) - y);
-  s = t;
-}
 
 template <class FT, bool UPDATE, bool PREDICT, bool COMP>
 __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
@@ -983,7 +978,6 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
     return;
   const double M = A.mass[n];
   const uint8_t blk = A.blocked[n];
-  const double dd = .5 * A.dt; // beta * dt
   double f[3] = {f0, f1, f2};
   // u after the predictor (blocked components keep theirs), for the record
   double un[3] = {0., 0., 0.};
@@ -999,26 +993,16 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
     if ((blk >> c) & 1)
       continue;
     const int64_t o = (int64_t)c * A.nn + n;
-    double fx = A.fext ? A.fext[o] : 0.;
-    if constexpr (COMP) {
-      // FP32 form: between two steps of one call the velocity array holds
-      // the half-step velocity w = v + dt/2 a (with its compensation), so
-      // the acceleration array is neither read nor written until the
-      // call's last step.  a = (f_ext + f_int) / M is the reference's
-      // a + (f_ext + f_int - M a) / M without the cancelling terms.  The
-      // velocity v = w + dt/2 a and the next w = v + dt/2 a are the two
-      // compensated additions that a call boundary performs too (here, then
-      // in k_predictor), so splitting a run of steps over calls keeps every
-      // bit; u += dt w is the predictor's u + dt v + dt^2/2 a.
-      const double a = (fx + f[c]) / M;
-      double w = A.v[o];
-      float cw = A.cv[o];
-      kahan_add(w, cw, dd * a); // v at the end of the step
+    const double fx = A.fext ? A.fext[o] : 0.;
+    if constexpr (COMP) { // FP32 form: half-step velocity (akx_math.hpp)
+      double w = A.v[o], a, u = 0.;
+      float cw = A.cv[o], cu = 0.f;
       if constexpr (PREDICT) {
-        double u = A.u[o];
-        float cu = A.cu[o];
-        kahan_add(w, cw, dd * a);
-        kahan_add(u, cu, A.dt * w);
+        u = A.u[o];
+        cu = A.cu[o];
+      }
+      half_step_gather<PREDICT>(fx + f[c], M, A.dt, w, cw, u, cu, a);
+      if constexpr (PREDICT) {
         A.u[o] = u;
         un[c] = u;
         A.cu[o] = cu;
@@ -1027,27 +1011,18 @@ __global__ void __launch_bounds__(256) k_gather_update(GatherArgs A) {
       }
       A.v[o] = w;
       A.cv[o] = cw;
-      continue;
+    } else { // reference order (akx_math.hpp)
+      double a = A.a[o], v = A.v[o], u = 0.;
+      if constexpr (PREDICT)
+        u = A.u[o];
+      reference_gather<PREDICT>(fx, f[c], M, A.dt, a, v, u);
+      if constexpr (PREDICT) {
+        A.u[o] = u;
+        un[c] = u;
+      }
+      A.a[o] = a;
+      A.v[o] = v;
     }
-    double a = A.a[o], v = A.v[o];
-    double r = 0.;
-    r += 1. * fx;
-    r += 1. * f[c];
-    double cache = 0.;
-    cache += -1. * a;
-    r += M * cache;
-    const double x = 1. * (r / M);
-    a += 1. * x;
-    v += dd * x;
-    if constexpr (PREDICT) {
-      const double dt_a = A.dt * a;
-      const double u = A.u[o] + (A.dt * v + (.5 * A.dt) * dt_a);
-      A.u[o] = u;
-      un[c] = u;
-      v = v + dt_a;
-    }
-    A.a[o] = a;
-    A.v[o] = v;
   }
   if constexpr (PREDICT) {
     if (A.xu) { // u0 at 48 n + 24 bytes, then an aligned pair
@@ -1077,18 +1052,13 @@ __global__ void __launch_bounds__(256) k_predictor(GatherArgs A) {
     const int64_t o = (int64_t)c * A.nn + n;
     const double a = A.a[o];
     double v = A.v[o], u = A.u[o];
-    const double dt_a = A.dt * a;
-    if constexpr (COMP) {
-      // the velocity array takes the half-step velocity w = v + dt/2 a,
-      // which the gathers of this call advance (k_gather_update)
+    if constexpr (COMP) { // the velocity array takes the half-step velocity
       float cu = A.cu[o], cv = A.cv[o];
-      kahan_add(v, cv, (.5 * A.dt) * a);
-      kahan_add(u, cu, A.dt * v);
+      half_step_predict(a, A.dt, v, cv, u, cu);
       A.cu[o] = cu;
       A.cv[o] = cv;
     } else {
-      u = u + (A.dt * v + (.5 * A.dt) * dt_a);
-      v = v + dt_a;
+      reference_predict(a, A.dt, v, u);
     }
     A.u[o] = u;
     un[c] = u;

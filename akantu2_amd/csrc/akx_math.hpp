@@ -442,4 +442,73 @@ AKX_HD void plastic_lih_total(R lambda, R mu, R sigma_y, R h, const R (&g)[3][3]
     pe_new[k] = pe[k] + c * dev[vi[k]][vj[k]];
 }
 
+// ---------------------------------------------------------------------------
+// Nodal update of one free DOF component (blocked components are not
+// touched).  Reference order, the plain FP64 path: newmark-beta.cc:47-77
+// (predictor) and :164-197 (corrector) with alpha = 0, beta = 1/2, the
+// residual of solid_mechanics_model.cc:235-249 and the lumped solve of
+// non_linear_solver_lumped.cc:76-89:
+//   gather     r = (0 + f_ext) + f_int + M (0 + -a); x = r / M;
+//              a += x; v += (dt / 2) x
+//   predictor  u += dt v + (dt / 2)(dt a); v += dt a
+// FP32 form: the same central-difference scheme with compensated FP64
+// state (FP32 compensation terms).  Between the steps of one solve call the
+// velocity holds the half-step velocity w = v + dt/2 a, so the acceleration
+// is neither read nor stored until the call's last step:
+//   predictor  w = v + dt/2 a; u += dt w
+//   gather     a = (f_ext + f_int) / M (the reference's a + x without the
+//              cancelling terms); v = w + dt/2 a; then, when more steps
+//              follow in the call, w = v + dt/2 a and u += dt w
+// The predictor at the start of a call and the gather that ends the call
+// before it perform the same two compensated additions as a step inside a
+// call, so splitting a run of steps over calls keeps every bit.
+// ---------------------------------------------------------------------------
+AKX_HD void kahan_add(double &s, float &c, double inc) {
+  double y = inc - double(c);
+  double t = s + y;
+  c = float((t - s) - y);
+  s = t;
+}
+
+AKX_HD void reference_predict(double a, double dt, double &v, double &u) {
+  const double dt_a = dt * a;
+  u = u + (dt * v + (.5 * dt) * dt_a);
+  v = v + dt_a;
+}
+
+template <bool PREDICT>
+AKX_HD void reference_gather(double f_ext, double f_int, double M, double dt, double &a,
+                             double &v, double &u) {
+  double r = 0.;
+  r += 1. * f_ext;
+  r += 1. * f_int;
+  double cache = 0.;
+  cache += -1. * a;
+  r += M * cache;
+  const double x = 1. * (r / M);
+  a += 1. * x;
+  v += (.5 * dt) * x;
+  if constexpr (PREDICT)
+    reference_predict(a, dt, v, u);
+}
+
+AKX_HD void half_step_predict(double a, double dt, double &v, float &cv, double &u,
+                              float &cu) {
+  kahan_add(v, cv, (.5 * dt) * a);
+  kahan_add(u, cu, dt * v);
+}
+
+// w, cw: the half-step velocity on entry; the velocity (PREDICT = false)
+// or the next half-step velocity on return.  a: written.
+template <bool PREDICT>
+AKX_HD void half_step_gather(double f, double M, double dt, double &w, float &cw, double &u,
+                             float &cu, double &a) {
+  a = f / M;
+  kahan_add(w, cw, (.5 * dt) * a);
+  if constexpr (PREDICT) {
+    kahan_add(w, cw, (.5 * dt) * a);
+    kahan_add(u, cu, dt * w);
+  }
+}
+
 } // namespace AKX_MATH_NS
