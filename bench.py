@@ -62,7 +62,7 @@ TIMING_STEPS = 100  # steps of the event-timed per-kernel pass
 RAMP_STEPS = 40     # untimed steps first: the GPU clocks rise after the idle set-up
 
 
-def build_c2(shape=SHAPE):
+def build_hex8_box(shape=SHAPE):
     from akantu2_amd import mesh as M
     mesh = M.box_mesh("_hexahedron_8", *shape, jitter=0.1, seed=1)
     blocked = M.clamped_face_x0(mesh)
@@ -104,7 +104,7 @@ def cpu_baseline_plain(shape=(20, 100, 100), steps=10):
     first 20 cells in x of the 200 x 100 x 100 grid, on one thread and on
     the host's cores (OpenMP element and DOF loops; results identical)."""
     import oracle
-    mesh, blocked, v0 = build_c2(shape)
+    mesh, blocked, v0 = build_hex8_box(shape)
     o = oracle.OracleModel(oracle.HEX8, mesh.nodes, mesh.conn, "neohookean")
     o.init_full()
     o.blocked[:] = blocked
@@ -135,7 +135,7 @@ def run_plain(args):
     steps = args.steps or 1000
     shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else SHAPE
     t = time.perf_counter()
-    mesh, blocked, v0 = build_c2(shape)
+    mesh, blocked, v0 = build_hex8_box(shape)
     model = SolidMechanicsModel(mesh)
     model.addMaterial("neohookean", rho=1.0, E=1.0, nu=0.3)
     model.setPrecision(args.precision)
@@ -270,7 +270,7 @@ def run_plain(args):
               flush=True)
         out["forms"] = run_forms(args, model, mesh, blocked, v0, dt, value,
                                  RAMP_STEPS + TIMING_STEPS + args.warmup + steps)
-        out["north_star"] = north_star_summary(out)
+        out["form_targets"] = form_targets(out)
     return out
 
 
@@ -344,7 +344,7 @@ def run_forms(args, plain, mesh, blocked, v0, dt, plain_value, plain_steps):
     del m
     print(f"# forms: precision {1e3 * el / timed:.3f} ms/step", file=sys.stderr, flush=True)
 
-    specs = c4_instances(512)
+    specs = batch_instances(512)
     batch = {}
     for prec in ("fp32", "fp64"):
         t = time.perf_counter()
@@ -399,7 +399,7 @@ def run_forms(args, plain, mesh, blocked, v0, dt, plain_value, plain_steps):
     return forms
 
 
-def north_star_summary(out):
+def form_targets(out):
     """The target stated for the forms: the FP32 batched / served form at >= 3x
     the plain FP64 path's element*qp throughput, <= 1e-5 relative
     displacement error, >= 70 % of peak HBM bandwidth.  Ratios are given
@@ -423,7 +423,7 @@ def north_star_summary(out):
 # --------------------------------------------------------------------------
 # precision form: 60 M Tet4, per-element E and nu
 # --------------------------------------------------------------------------
-def build_c3(shape, seed=3):
+def build_tet4_box(shape, seed=3):
     from akantu2_amd import mesh as M
     mesh = M.box_mesh("_tetrahedron_4", *shape, jitter=0.1, seed=seed)
     rng = np.random.default_rng(seed)
@@ -464,7 +464,7 @@ def run_precision(args):
     steps = args.steps or 2000
     shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else (1000, 100, 100)
     t = time.perf_counter()
-    mesh, blocked, v0, E, nu = build_c3(shape)
+    mesh, blocked, v0, E, nu = build_tet4_box(shape)
     print(f"# mesh {time.perf_counter() - t:.1f} s, {mesh.nb_elements} Tet4",
           file=sys.stderr, flush=True)
     res, models = {}, {}
@@ -521,7 +521,7 @@ def cpu_baseline_precision(shape=(60, 10, 10), steps=200):
     """The oracle on a 36,000-Tet4 box of the same workload (timed, 1 thread),
     and the FP32 form's per-field error against it on that box."""
     import oracle
-    mesh, blocked, v0, E, nu = build_c3(shape)
+    mesh, blocked, v0, E, nu = build_tet4_box(shape)
     o = oracle.OracleModel(oracle.TET4, mesh.nodes, mesh.conn, "elastic", E_el=E, nu_el=nu)
     o.init_full()
     o.blocked[:] = blocked
@@ -555,7 +555,7 @@ def cpu_baseline_precision(shape=(60, 10, 10), steps=200):
 ETYPES = {"tet4": "_tetrahedron_4", "hex8": "_hexahedron_8"}
 
 
-def c4_instances(n=512, seed=4, scale=1.0, etype="tet4", law=PL):
+def batch_instances(n=512, seed=4, scale=1.0, etype="tet4", law=PL):
     """Instance descriptions: cells, material, seed (scale shrinks the sizes
     for a quick run).  The batching workload is Tet4 with linear isotropic
     hardening; etype / law give the same stream of sizes and aspects for
@@ -578,7 +578,7 @@ def c4_instances(n=512, seed=4, scale=1.0, etype="tet4", law=PL):
     return out
 
 
-def c4_mesh(spec):
+def instance_mesh(spec):
     from akantu2_amd import mesh as M
     mesh = M.box_mesh(ETYPES[spec.get("etype", "tet4")], *spec["cells"])
     return mesh, M.clamped_face_x0(mesh), M.initial_velocity(mesh, 1e-2, spec["seed"])
@@ -594,7 +594,7 @@ def _fill_batch(precision, specs):
               plastic=law == PL)
     ids = []
     for s in specs:
-        mesh, blocked, v0 = c4_mesh(s)
+        mesh, blocked, v0 = instance_mesh(s)
         ids.append(b.add(mesh, law, blocked=blocked, velocity=v0, dt_factor=0.5, **s["mat"]))
     return b, ids
 
@@ -622,7 +622,7 @@ def _time_batch(b, warmup, steps, tune):
 def run_batch(args):
     steps = args.steps or 500
     law = args.law or (PL if args.etype == "tet4" else "neohookean")
-    specs = c4_instances(args.instances, scale=args.scale, etype=args.etype, law=law)
+    specs = batch_instances(args.instances, scale=args.scale, etype=args.etype, law=law)
     t = time.perf_counter()
     res = {}
     for prec in ("fp32", "fp64"):
@@ -698,7 +698,7 @@ def cpu_baseline_batch(spec, gpu_u, steps):
     """The oracle on one instance of the batch (the smallest), timed on one
     thread, and the FP32 batch's displacement error against it."""
     import oracle
-    mesh, blocked, v0 = c4_mesh(spec)
+    mesh, blocked, v0 = instance_mesh(spec)
     hex8 = spec.get("etype") == "hex8"
     law = spec.get("law", PL)
     o = oracle.OracleModel(oracle.HEX8 if hex8 else oracle.TET4, mesh.nodes, mesh.conn, law,

@@ -1,8 +1,8 @@
 """ctypes binding of the CPU FP64 oracle (oracle/akantu_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
-the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product package
-``akantu2_amd`` never imports it.
+the ``cpu_baseline`` leg of ``bench.py``, to compare results against.  The
+product package ``akantu2_amd`` never imports it.
 
 ``OracleModel`` owns numpy arrays in the reference's array-of-structures
 layouts and drives the C restatement of ``SolidMechanicsModel::solveStep``.

@@ -27,7 +27,7 @@ CHECKS = (20, STEPS)  # step counts at which results are read
 
 @pytest.fixture(scope="module")
 def specs():
-    s = bench.c4_instances(512, scale=1.0)
+    s = bench.batch_instances(512, scale=1.0)
     assert len(s) == 512
     return s
 
@@ -63,10 +63,10 @@ def full(specs):
     return res
 
 
-def test_c4_fp64_batch_equals_oracle_on_smallest(specs, full):
+def test_fp64_batch_equals_oracle_on_smallest(specs, full):
     small = np.argsort([np.prod(s["cells"]) for s in specs])[:4]
     for k in small:
-        mesh, blocked, v0 = bench.c4_mesh(specs[k])
+        mesh, blocked, v0 = bench.instance_mesh(specs[k])
         o, _ = make_oracle(mesh, blocked, v0, PL, **specs[k]["mat"])
         done = 0
         for n in CHECKS:
@@ -81,7 +81,7 @@ def test_c4_fp64_batch_equals_oracle_on_smallest(specs, full):
         assert o.energy("plastic") > 0, k
 
 
-def test_c4_fp32_batch_within_1e5_of_fp64_on_every_instance(full):
+def test_fp32_batch_within_1e5_of_fp64_on_every_instance(full):
     worst = {}
     for n in CHECKS:
         errs = np.array([relerr(full["fp32"][n][k]["u"], full["fp64"][n][k]["u"])
@@ -96,7 +96,7 @@ def test_c4_fp32_batch_within_1e5_of_fp64_on_every_instance(full):
         assert mx <= 1e-5, (n, k, mx)
 
 
-def test_c4_instances_alone_equal_in_mix(specs, full):
+def test_instances_alone_equal_in_mix(specs, full):
     rng = np.random.default_rng(44)
     sample = sorted(rng.choice(512, 4, replace=False).tolist())
     for prec in ("fp64", "fp32"):

@@ -31,8 +31,8 @@ PRECISIONS = ["fp64"]
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
-def test_c1_plain_fp64_1000_steps(precision):
-    """Config 1: 20x4x4 Kuhn Tet4 bar, E=1 nu=0.3 rho=1, clamped x=0, seeded
+def test_tet4_bar_plain_fp64_1000_steps(precision):
+    """The 20x4x4 Kuhn Tet4 bar, E=1 nu=0.3 rho=1, clamped x=0, seeded
     sin-enveloped initial velocity, dt = 0.5 dt_crit, 1000 steps."""
     mesh, blocked, v0 = workload("_tetrahedron_4", (20, 4, 4), seed=0)
     o, dt = make_oracle(mesh, blocked, v0, "elastic")
@@ -42,7 +42,7 @@ def test_c1_plain_fp64_1000_steps(precision):
     for k in range(10):
         o.solve_steps(100)
         g.solveSteps(100)
-        _compare(g, o, f"C1 step {(k + 1) * 100}")
+        _compare(g, o, f"Tet4 bar step {(k + 1) * 100}")
     if precision == "fp64":  # bit for bit
         assert np.array_equal(g.getDisplacement(), o.u)
         assert np.array_equal(g.getInternalForce(), o.f_int)
@@ -65,8 +65,8 @@ def test_laws_and_elements_jittered(etype, law, precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
-def test_c2_reduced_checked_every_100_steps(precision):
-    """Config 2 at reduced size: jittered Hex8 neo-Hookean, checked against
+def test_hex8_box_reduced_checked_every_100_steps(precision):
+    """The jittered Hex8 neo-Hookean box at reduced size, checked against
     the oracle at every 100th step over 1000 steps."""
     mesh, blocked, v0 = workload("_hexahedron_8", (20, 10, 10), jitter=0.1, seed=1)
     o, dt = make_oracle(mesh, blocked, v0, "neohookean")
@@ -74,7 +74,7 @@ def test_c2_reduced_checked_every_100_steps(precision):
     for k in range(10):
         o.solve_steps(100)
         g.solveSteps(100)
-        _compare(g, o, f"C2-reduced step {(k + 1) * 100}")
+        _compare(g, o, f"reduced Hex8 box step {(k + 1) * 100}")
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
@@ -98,7 +98,7 @@ def test_plastic_linear_isotropic_hardening(etype, precision):
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_per_element_parameters(precision):
-    """Config 3's per-element E (log-uniform [0.5, 2]) and nu (U[0.2, 0.35])."""
+    """The 60 M-element Tet4 box's per-element E (log-uniform [0.5, 2]) and nu (U[0.2, 0.35])."""
     mesh, blocked, v0 = workload("_tetrahedron_4", (10, 4, 4), jitter=0.1, seed=7)
     rng = np.random.default_rng(7)
     E = np.exp(rng.uniform(np.log(0.5), np.log(2.0), mesh.nb_elements))

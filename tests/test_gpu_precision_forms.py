@@ -38,7 +38,7 @@ def test_fp64_fast(etype, law, steps, tol):
 @pytest.mark.parametrize("etype,law,tol_u", [("_tetrahedron_4", "elastic", 1e-5),
                                              ("_hexahedron_8", "neohookean", 1e-5)])
 def test_fp32_form_error_against_fp64_oracle(etype, law, tol_u):
-    """Config 3 in miniature (jittered Tet4, per-element E and nu, linear
+    """The 60 M-element Tet4 box in miniature (jittered Tet4, per-element E and nu, linear
     elastic, 2000 steps) and the Hex8 neo-Hookean case (a serving-form request
     type): u within 1e-5 of the FP64 oracle.  Reports L2 and max relative error per field."""
     mesh, blocked, v0 = workload(etype, (20, 4, 4), jitter=0.1, seed=11)

@@ -368,7 +368,7 @@ def test_central_difference_single_dof_closed_form():
         assert u[0] == pytest.approx(np.cos(n * theta), abs=1e-12)
 
 
-def test_c1_energy_is_conserved():
+def test_tet4_bar_energy_is_conserved():
     m = M.box_mesh("_tetrahedron_4", 20, 4, 4)
     o = oracle.OracleModel(oracle.TET4, m.nodes, m.conn, "elastic")
     o.init_full()
